@@ -1,0 +1,175 @@
+/*
+ * snc_hip.h -- C ABI of the MI355X (gfx950) backend for SNC's event-driven
+ * spiking path.
+ *
+ * Plain C: POD structs, raw pointers and sizes, integer status codes. No C++
+ * types, exceptions or torch types cross this boundary. Every entry point is
+ * synchronous with respect to the host unless stated otherwise and returns
+ * SNC_OK (0) on success or a negative SNC_E_* code; snc_hip_last_error()
+ * returns a human-readable message for the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   snc_hip_available       <- snc::cuda::available()       include/snc/runtime.hpp:61-64,
+ *                                                             src/runtime/cuda_backend.cu:181-184
+ *   snc_hip_forward         <- snc::cuda::forward(...)       include/snc/runtime.hpp:65-69,
+ *                                                             src/runtime/cuda_backend.cu:186-361
+ *   snc_hip_plan_*          <- the per-call upload + step loop of cuda::forward, split so the
+ *                              graph and inputs stay resident in HBM across calls
+ *                              (cuda_backend.cu:236-336 re-uploads on every call)
+ *   snc_hip_train_*         <- snc::cudatrain::* session     include/snc/cuda_trainer.hpp:24-57
+ *   snc_hip_mlp_*           <- no counterpart: learned grow proposal that plugs into
+ *                              Connectome::structural_update's grow step
+ *                              (src/structure/connectome.cpp:164-211)
+ *
+ * Not thread-safe: one host thread per plan/session, as in the reference.
+ */
+#ifndef SNC_HIP_H_
+#define SNC_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define SNC_OK 0
+#define SNC_E_INVALID (-1)     /* bad argument / graph fails validation      */
+#define SNC_E_NODEVICE (-2)    /* no usable gfx950 device                     */
+#define SNC_E_HIP (-3)         /* a HIP runtime call failed                   */
+#define SNC_E_NOMEM (-4)       /* device allocation failed                    */
+#define SNC_E_UNSUPPORTED (-5) /* variant / config not supported by this build */
+
+/* ---- delivery variants (values follow snc::Backend, runtime.hpp:21-27,
+ *      extended with the hip-* strategies) ---------------------------------- */
+#define SNC_VARIANT_ATOMIC 5 /* hip-atomic: float atomics into the ring      */
+#define SNC_VARIANT_BUCKET 6 /* hip-bucket: per-target ordered fold, bit-exact */
+#define SNC_VARIANT_SORT 7   /* hip-sort: canonical expand + stable radix sort */
+#define SNC_VARIANT_AUTO 8   /* hip-auto: autotuner picks per graph           */
+
+/* ---- graph roles (GraphRole, snn_graph.hpp:29) ---------------------------- */
+#define SNC_ROLE_INTERNAL 0
+#define SNC_ROLE_INPUT 1
+#define SNC_ROLE_OUTPUT 2
+
+/* ---- reset modes --------------------------------------------------------- */
+#define SNC_RESET_TO_VALUE 0 /* v = reset on a spike (cpu_backend.cpp:135-138)  */
+#define SNC_RESET_SUBTRACT 1 /* v -= threshold on a spike; v held while refractory */
+
+/* Read-only view over SNNGraph's arrays (snn_graph.hpp:43-77). CSR over
+ * OUTGOING synapses, rows sorted by pre id. */
+typedef struct snc_graph_view {
+  int32_t num_neurons;
+  int32_t num_synapses;
+  const int32_t* row_ptr;  /* [N+1]                                     */
+  const int32_t* post_ids; /* [S]                                       */
+  const float* weights;    /* [S]  (signed values are allowed)          */
+  const int32_t* delays;   /* [S]  >= 1                                 */
+  const int8_t* sign;      /* [N]  +1 / -1 (applied on the pre)         */
+  const uint8_t* role;     /* [N]  SNC_ROLE_*                           */
+  const int32_t* channel;  /* [N]  input/output channel or -1           */
+} snc_graph_view;
+
+/* LIFParams (runtime.hpp:34-39) plus the reset mode. */
+typedef struct snc_lif {
+  float threshold;
+  float decay;
+  float reset;
+  int32_t refractory;
+  int32_t reset_mode; /* SNC_RESET_* */
+} snc_lif;
+
+/* InputEvent (encoders.hpp:21-26), identical layout. */
+typedef struct snc_event {
+  int32_t sample_id;
+  int32_t channel_id;
+  int32_t time;
+  float amplitude;
+} snc_event;
+
+/* RunStats (runtime.hpp:41-46) plus the variant that actually ran. */
+typedef struct snc_forward_stats {
+  int64_t spikes;
+  int64_t synaptic_events;
+  int32_t overflow; /* events dropped for lack of buffer space (always 0 here) */
+  int32_t used_variant;
+  double ms; /* device time of the step loop only (HIP events) */
+} snc_forward_stats;
+
+/* Optional outputs of a forward run; any pointer may be NULL. */
+typedef struct snc_forward_capture {
+  float* logits;               /* [num_output_channels] summed output spikes  */
+  int32_t* neuron_spikes;      /* [N] per-neuron spike counts                 */
+  uint32_t* raster;            /* [num_steps][ceil(N/32)] fired bitmap        */
+  float* final_v;              /* [N] membrane after the last step            */
+  int64_t* step_spikes;        /* [num_steps] spikes per step                 */
+} snc_forward_capture;
+
+typedef struct snc_hip_graph snc_hip_graph; /* device-resident graph      */
+typedef struct snc_hip_plan snc_hip_plan;   /* graph + inputs + state      */
+
+const char* snc_hip_last_error(void);
+
+/* 1 when a gfx950 device is usable, else 0 (cuda_backend.cu:181-184). */
+int snc_hip_available(void);
+
+/* Upload a graph once; it stays resident until snc_hip_graph_free. The graph
+ * is validated as SNNGraph::validate does (snn_graph.cpp:30-56). */
+int snc_hip_graph_upload(const snc_graph_view* g, snc_hip_graph** out);
+void snc_hip_graph_free(snc_hip_graph* g);
+
+/* Static properties of an uploaded graph used by the autotuner. */
+typedef struct snc_graph_profile {
+  int32_t num_neurons;
+  int64_t num_synapses;
+  int32_t max_delay;
+  int32_t ring_len;
+  int32_t max_in_degree;
+  int32_t max_out_degree;
+  double mean_in_degree;
+  double heavy_fraction; /* fraction of synapses onto targets with in-degree > 256 */
+} snc_graph_profile;
+int snc_hip_graph_profile_get(const snc_hip_graph* g, snc_graph_profile* out);
+
+/* Build a forward plan: the sample's events are resolved to per-step
+ * injections (build_schedule, cpu_backend.cpp:47-65) and uploaded, and the
+ * device state is allocated. `variant` is SNC_VARIANT_*. */
+int snc_hip_plan_create(snc_hip_graph* g, const snc_event* events, int64_t n_events,
+                        int32_t num_steps, const snc_lif* lif, int32_t variant,
+                        snc_hip_plan** out);
+/* Run the plan from a zero state. Inputs are already resident; stats->ms is
+ * the device time of the step loop. */
+int snc_hip_plan_run(snc_hip_plan* p, snc_forward_capture* cap, snc_forward_stats* stats);
+/* Same as plan_run but without synchronising or reading results back: the
+ * caller synchronises the device itself. For throughput timing. */
+int snc_hip_plan_launch(snc_hip_plan* p);
+/* The variant a plan resolved to (useful after SNC_VARIANT_AUTO). */
+int32_t snc_hip_plan_variant(const snc_hip_plan* p);
+/* Average device duration (ms) of the dominant kernel over the last run, and
+ * the number of its launches. */
+int snc_hip_plan_kernel_time(const snc_hip_plan* p, double* avg_ms, int64_t* launches);
+void snc_hip_plan_free(snc_hip_plan* p);
+
+/* Per-kernel device time of one instrumented run: eager launches with a HIP
+ * event around every kernel on the plan's stream. kernel_ms_total[k] and
+ * launches[k] are filled for each class k = SNC_KERNEL_*. */
+#define SNC_KERNEL_FOLD_HEAVY 0
+#define SNC_KERNEL_LIF 1
+#define SNC_KERNEL_EXPAND 2
+#define SNC_KERNEL_SORT 3
+#define SNC_KERNEL_PERSISTENT 4
+#define SNC_KERNEL_CLASSES 5
+int snc_hip_plan_profile(snc_hip_plan* p, double* kernel_ms_total, int64_t* launches);
+
+/* Wait until all work the library queued on the device has finished. */
+int snc_hip_synchronize(void);
+
+/* One-shot forward, the shape of cuda::forward: upload, run, read back, free. */
+int snc_hip_forward(const snc_graph_view* g, const snc_event* events, int64_t n_events,
+                    int32_t num_steps, const snc_lif* lif, int32_t variant,
+                    snc_forward_capture* cap, snc_forward_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNC_HIP_H_ */

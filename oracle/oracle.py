@@ -1,0 +1,300 @@
+"""ctypes wrappers of the CPU oracle and of the compiled reference (tests only)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(_HERE, "liboracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "libsnc_ref.so")
+
+EVENT_DTYPE = np.dtype([("sample_id", "<i4"), ("channel_id", "<i4"), ("time", "<i4"),
+                        ("amplitude", "<f4")])
+
+
+class _GraphView(C.Structure):
+    _fields_ = [("num_neurons", C.c_int32), ("num_synapses", C.c_int32),
+                ("row_ptr", C.c_void_p), ("post_ids", C.c_void_p), ("weights", C.c_void_p),
+                ("delays", C.c_void_p), ("sign", C.c_void_p), ("role", C.c_void_p),
+                ("channel", C.c_void_p)]
+
+
+class _Lif(C.Structure):
+    _fields_ = [("threshold", C.c_float), ("decay", C.c_float), ("reset", C.c_float),
+                ("refractory", C.c_int32), ("reset_mode", C.c_int32)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("spikes", C.c_int64), ("synaptic_events", C.c_int64), ("overflow", C.c_int32),
+                ("used_variant", C.c_int32), ("ms", C.c_double)]
+
+
+class _Capture(C.Structure):
+    _fields_ = [("logits", C.c_void_p), ("neuron_spikes", C.c_void_p), ("raster", C.c_void_p),
+                ("final_v", C.c_void_p), ("step_spikes", C.c_void_p)]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+_orc = None
+_ref = None
+
+
+def oracle_lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"{ORACLE_PATH} missing: run make -C oracle")
+        _orc = C.CDLL(ORACLE_PATH)
+        _orc.oracle_forward.restype = C.c_int
+        _orc.oracle_forward.argtypes = [C.POINTER(_GraphView), C.c_void_p, C.c_int64, C.c_int32,
+                                        C.POINTER(_Lif), C.c_int32, C.POINTER(_Capture),
+                                        C.POINTER(_Stats)]
+        _orc.oracle_encode.restype = C.c_int
+        _orc.oracle_encode.argtypes = [C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_float,
+                                       C.c_float, C.c_float, C.c_uint64, C.c_int32, C.c_void_p,
+                                       C.c_int64, C.POINTER(C.c_int64)]
+        _orc.oracle_make_synthetic.restype = None
+        _orc.oracle_make_synthetic.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                               C.c_uint64, C.c_void_p, C.c_void_p]
+        _orc.oracle_philox4x32.restype = None
+        _orc.oracle_philox4x32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _orc.oracle_splitmix64.restype = C.c_uint64
+        _orc.oracle_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
+        _bind_train(_orc)
+    return _orc
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("compiled reference not present")
+        _ref = C.CDLL(REF_PATH)
+        V, I32, F, I64, U64, D = C.c_void_p, C.c_int32, C.c_float, C.c_int64, C.c_uint64, C.c_double
+        sig = {
+            "ref_make_static_snc": (V, [V, I32, I32, F, I32, F, U64]),
+            "ref_make_random_sparse": (V, [V, I32, I32, F, I32, F, U64]),
+            "ref_make_dense": (V, [V, I32, F, I32]),
+            "ref_graph_sizes": (None, [V, V, V]),
+            "ref_graph_copy": (None, [V] * 8),
+            "ref_graph_validate": (C.c_int, [I32, I32] + [V] * 7),
+            "ref_graph_free": (None, [V]),
+            "ref_forward": (C.c_int, [I32, I32] + [V] * 7 + [V, I64, I32, F, F, F, I32, I32,
+                                                             V, V, V, V]),
+            "ref_encode": (I64, [I32, V, I32, I32, F, F, F, U64, I32, V, I64]),
+            "ref_make_synthetic": (None, [I32, I32, I32, F, U64, V, V]),
+            "ref_trainer_create": (V, [I32, I32] + [V] * 7 + [I32, F, F, F, I32, I32, F, F, F,
+                                                             F, F, F, F, F, I32, U64]),
+            "ref_trainer_free": (None, [V]),
+            "ref_trainer_get_weights": (None, [V, V]),
+            "ref_trainer_set_weights": (None, [V, V, I64]),
+            "ref_trainer_reset_stats": (None, [V]),
+            "ref_trainer_stats": (None, [V, V, V]),
+            "ref_trainer_train_epoch": (D, [V, V, V, I32, I32, I32, I32, V, V, V]),
+            "ref_trainer_evaluate": (D, [V, V, V, I32, I32, I32]),
+            "ref_connectome_create": (V, [V, I32, I32, I32, F, U64]),
+            "ref_connectome_free": (None, [V]),
+            "ref_connectome_to_graph": (V, [V]),
+            "ref_connectome_num_synapses": (I32, [V]),
+            "ref_connectome_edge_weights": (None, [V, V]),
+            "ref_connectome_set_edge_weights": (None, [V, V, I64]),
+            "ref_connectome_update": (None, [V, V, I64, V, I64, I32, I32, F, F, I32, U64, V, V]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(_ref, k)
+            f.restype = r
+            f.argtypes = a
+    return _ref
+
+
+# ---------------------------------------------------------------------------
+# graph helpers (duck-typed on snc_amd.graph.SNNGraph: row_ptr, post_ids, ...)
+# ---------------------------------------------------------------------------
+
+def _view(g) -> _GraphView:
+    return _GraphView(g.num_neurons, g.num_synapses, _p(g.row_ptr), _p(g.post_ids),
+                      _p(g.weights), _p(g.delays), _p(g.sign), _p(g.role), _p(g.channel))
+
+
+def _graph_args(g):
+    return [g.num_neurons, g.num_synapses, _p(g.row_ptr), _p(g.post_ids), _p(g.weights),
+            _p(g.delays), _p(g.sign), _p(g.role), _p(g.channel)]
+
+
+def _lif(lif) -> _Lif:
+    return _Lif(lif.threshold, lif.decay, lif.reset, lif.refractory,
+                getattr(lif, "reset_mode", 0))
+
+
+def forward(g, events, num_steps, lif, use_omp=False, capture_raster=True):
+    """Oracle run_cpu: dict with logits, spikes, events, ms, neuron_spikes,
+    raster [T][W] uint32, final_v, step_spikes."""
+    ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+    n = g.num_neurons
+    n_out = int(g.channel[g.role == 2].max() + 1) if (g.role == 2).any() else 0
+    logits = np.zeros(max(n_out, 1), dtype=np.float32)
+    nspk = np.zeros(n, dtype=np.int32)
+    W = (n + 31) // 32
+    raster = np.zeros((num_steps, W), dtype=np.uint32) if capture_raster else None
+    fv = np.zeros(n, dtype=np.float32)
+    steps = np.zeros(num_steps, dtype=np.int64)
+    cap = _Capture(_p(logits), _p(nspk), _p(raster), _p(fv), _p(steps))
+    st = _Stats()
+    lc = _lif(lif)
+    rc = oracle_lib().oracle_forward(C.byref(_view(g)), _p(ev) if ev.size else None, ev.size,
+                                     num_steps, C.byref(lc), 1 if use_omp else 0, C.byref(cap),
+                                     C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle_forward failed: {rc}")
+    return dict(logits=logits[:n_out], spikes=st.spikes, events=st.synaptic_events, ms=st.ms,
+                neuron_spikes=nspk, raster=raster, final_v=fv, step_spikes=steps)
+
+
+def encode(enc, x, num_steps, gain=1.0, max_rate=0.5, threshold=0.0, seed=0, sample_id=0):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = C.c_int64()
+    lib = oracle_lib()
+    lib.oracle_encode(enc, _p(x), x.size, num_steps, gain, max_rate, threshold, seed, sample_id,
+                      None, 0, C.byref(n))
+    out = np.zeros(n.value, dtype=EVENT_DTYPE)
+    lib.oracle_encode(enc, _p(x), x.size, num_steps, gain, max_rate, threshold, seed, sample_id,
+                      _p(out) if n.value else None, n.value, C.byref(n))
+    return out
+
+
+def make_synthetic(n, dim, classes, noise, seed):
+    X = np.zeros((n, dim), dtype=np.float32)
+    y = np.zeros(n, dtype=np.int32)
+    oracle_lib().oracle_make_synthetic(n, dim, classes, noise, seed, _p(X), _p(y))
+    return X, y
+
+
+def philox4x32(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    oracle_lib().oracle_philox4x32(_p(c), _p(k), _p(o))
+    return o
+
+
+def splitmix64(seed, count):
+    s = C.c_uint64(seed)
+    lib = oracle_lib()
+    return [lib.oracle_splitmix64(C.byref(s)) for _ in range(count)]
+
+
+# ---------------------------------------------------------------------------
+# compiled reference
+# ---------------------------------------------------------------------------
+
+class _RefGraph:
+    def __init__(self, row_ptr, post_ids, weights, delays, sign, role, channel):
+        self.row_ptr, self.post_ids, self.weights, self.delays = row_ptr, post_ids, weights, delays
+        self.sign, self.role, self.channel = sign, role, channel
+        self.num_neurons = row_ptr.size - 1
+        self.num_synapses = post_ids.size
+
+
+def _ref_graph_take(h):
+    lib = ref_lib()
+    n, m = C.c_int32(), C.c_int32()
+    lib.ref_graph_sizes(h, C.byref(n), C.byref(m))
+    n, m = n.value, m.value
+    arrs = [np.zeros(n + 1, np.int32), np.zeros(m, np.int32), np.zeros(m, np.float32),
+            np.zeros(m, np.int32), np.zeros(n, np.int8), np.zeros(n, np.uint8),
+            np.zeros(n, np.int32)]
+    lib.ref_graph_copy(h, *[_p(a) for a in arrs])
+    lib.ref_graph_free(h)
+    return _RefGraph(*arrs)
+
+
+def ref_static_snc(layers, budget, weight=0.5, delay=1, inhib=0.2, seed=1):
+    lay = np.ascontiguousarray(layers, dtype=np.int32)
+    return _ref_graph_take(ref_lib().ref_make_static_snc(_p(lay), lay.size, budget, weight, delay,
+                                                         inhib, seed))
+
+
+def ref_random_sparse(layers, budget, weight=0.5, delay=1, inhib=0.2, seed=1):
+    lay = np.ascontiguousarray(layers, dtype=np.int32)
+    return _ref_graph_take(ref_lib().ref_make_random_sparse(_p(lay), lay.size, budget, weight,
+                                                            delay, inhib, seed))
+
+
+def ref_dense(layers, weight=0.5, delay=1):
+    lay = np.ascontiguousarray(layers, dtype=np.int32)
+    return _ref_graph_take(ref_lib().ref_make_dense(_p(lay), lay.size, weight, delay))
+
+
+def ref_validate(g) -> bool:
+    return bool(ref_lib().ref_graph_validate(*_graph_args(g)))
+
+
+def ref_forward(g, events, num_steps, lif, use_omp=False):
+    ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+    n_out = int(g.channel[g.role == 2].max() + 1) if (g.role == 2).any() else 0
+    logits = np.zeros(max(n_out, 1), dtype=np.float32)
+    sp, evn, ms = C.c_int64(), C.c_int64(), C.c_double()
+    ref_lib().ref_forward(*_graph_args(g), _p(ev) if ev.size else None, ev.size, num_steps,
+                          lif.threshold, lif.decay, lif.reset, lif.refractory,
+                          1 if use_omp else 0, _p(logits), C.byref(sp), C.byref(evn),
+                          C.byref(ms))
+    return dict(logits=logits[:n_out], spikes=sp.value, events=evn.value, ms=ms.value)
+
+
+def ref_encode(enc, x, num_steps, gain=1.0, max_rate=0.5, threshold=0.0, seed=0, sample_id=0):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    lib = ref_lib()
+    n = lib.ref_encode(enc, _p(x), x.size, num_steps, gain, max_rate, threshold, seed, sample_id,
+                       None, 0)
+    out = np.zeros(n, dtype=EVENT_DTYPE)
+    lib.ref_encode(enc, _p(x), x.size, num_steps, gain, max_rate, threshold, seed, sample_id,
+                   _p(out) if n else None, n)
+    return out
+
+
+def ref_make_synthetic(n, dim, classes, noise, seed):
+    X = np.zeros((n, dim), dtype=np.float32)
+    y = np.zeros(n, dtype=np.int32)
+    ref_lib().ref_make_synthetic(n, dim, classes, noise, seed, _p(X), _p(y))
+    return X, y
+
+
+def ref_raster(g, events, num_steps, lif):
+    """Per-step raster of every non-INPUT neuron from the reference's public
+    API alone: each non-input neuron becomes an OUTPUT with its own channel
+    (role changes decoding only, cpu_backend.cpp:149-151) and the run is
+    repeated for T = 1..num_steps; consecutive logit differences are the
+    spikes of the last step (forward is causal and deterministic)."""
+    probe_role = g.role.copy()
+    probe_ch = g.channel.copy()
+    ids = np.nonzero(g.role != 1)[0]
+    probe_role[ids] = 2
+    probe_ch[ids] = np.arange(ids.size, dtype=np.int32)
+    pg = _RefGraph(g.row_ptr, g.post_ids, g.weights, g.delays, g.sign, probe_role, probe_ch)
+    prev = np.zeros(ids.size, dtype=np.float32)
+    out = np.zeros((num_steps, ids.size), dtype=np.uint8)
+    spikes = np.zeros(num_steps, dtype=np.int64)
+    prev_sp = 0
+    for t in range(1, num_steps + 1):
+        r = ref_forward(pg, events, t, lif)
+        out[t - 1] = (r["logits"] - prev).astype(np.uint8)
+        spikes[t - 1] = r["spikes"] - prev_sp
+        prev, prev_sp = r["logits"], r["spikes"]
+    return ids, out, spikes
+
+
+# ---------------------------------------------------------------------------
+# trainer (bound in _bind_train once oracle_train.cpp provides it)
+# ---------------------------------------------------------------------------
+
+def _bind_train(lib):
+    pass

@@ -1,0 +1,626 @@
+// C ABI of the forward path: device graph upload, forward plans, one-shot
+// forward. Host code; the kernels live in forward.hip / sort.hip / persist.hip.
+//
+// Reference contract kept (src/runtime/cuda_backend.cu:186-361): synchronous,
+// validated input, non-zero status on any allocation or launch failure so the
+// caller can fall back, stats.ms covering only the step loop. Differences: the
+// graph is uploaded once and stays resident (snc_hip_graph_upload), and a plan
+// keeps one sample's inputs resident across runs.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "autotune.hpp"
+#include "common.hpp"
+#include "device_graph.hpp"
+#include "forward_plan.hpp"
+#include "persist.hpp"
+#include "snc_hip.h"
+
+namespace snc_amd {
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return fail(SNC_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+  }
+  return SNC_OK;
+}
+
+template <typename T>
+int dupload(T** p, const T* h, size_t count) {
+  int rc = dalloc(p, count);
+  if (rc) return rc;
+  if (count && hipMemcpy(*p, h, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(SNC_E_HIP, "hipMemcpy H2D failed");
+  return SNC_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+bool device_ok() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return false;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return false;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+}  // namespace
+
+DevGraph::~DevGraph() {
+  dfree(row_ptr);
+  dfree(post);
+  dfree(wsig);
+  dfree(delay);
+  dfree(cpos);
+  dfree(ccol);
+  dfree(cval);
+  dfree(cdel);
+  dfree(heavy_ids);
+  dfree(is_heavy);
+}
+
+}  // namespace snc_amd
+
+using namespace snc_amd;
+
+struct snc_hip_plan;
+static int sort_alloc(snc_hip_plan* P);
+
+struct snc_hip_plan {
+  snc_hip_graph* G = nullptr;
+  FwdPlanCore core;
+  StepArgs a{};
+  int T = 0;
+  snc_lif lif{};
+  int variant = SNC_VARIANT_ATOMIC;
+  bool persistent = false;
+  PersistConfig pcfg{};
+  std::vector<int32_t> h_inj_ptr;
+  // owned device buffers
+  int32_t *inj_ptr = nullptr, *inj_nid = nullptr;
+  float* inj_amp = nullptr;
+  float *v = nullptr, *ext = nullptr, *ring = nullptr;
+  int32_t *refr = nullptr, *fired = nullptr, *fcnt = nullptr, *nspk = nullptr;
+  uint8_t* dirty = nullptr;
+  uint32_t* bits = nullptr;
+  size_t n_bits_words = 0;
+  unsigned long long *blk_spikes = nullptr, *blk_events = nullptr;
+  uint32_t* raster = nullptr;
+  uint32_t *keys = nullptr, *keys_alt = nullptr;
+  int32_t *vals = nullptr, *vals_alt = nullptr, *hist = nullptr, *ev_ptr = nullptr;
+  size_t sort_capacity = 0;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // [raster off, on]
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  double kernel_avg_ms = 0.0;
+  int64_t kernel_launches = 0;
+
+  ~snc_hip_plan() {
+    for (auto& g : graph_exec)
+      if (g) (void)hipGraphExecDestroy(g);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (core.stream) (void)hipStreamDestroy(core.stream);
+    dfree(inj_ptr);
+    dfree(inj_nid);
+    dfree(inj_amp);
+    dfree(v);
+    dfree(ext);
+    dfree(ring);
+    dfree(refr);
+    dfree(fired);
+    dfree(fcnt);
+    dfree(nspk);
+    dfree(dirty);
+    dfree(bits);
+    dfree(blk_spikes);
+    dfree(blk_events);
+    dfree(raster);
+    dfree(keys);
+    dfree(keys_alt);
+    dfree(vals);
+    dfree(vals_alt);
+    dfree(hist);
+    dfree(ev_ptr);
+  }
+};
+
+// Scratch of the hip-sort strategy, sized for the worst step (every neuron
+// fires): S event keys and values, double-buffered, plus offsets.
+static int sort_alloc(snc_hip_plan* P) {
+  const DevGraph& g = P->G->g;
+  const size_t cap = (size_t)std::max<int64_t>(1, g.S);
+  P->sort_capacity = cap;
+  int rc = SNC_OK;
+  if (!rc) rc = dalloc(&P->keys, cap);
+  if (!rc) rc = dalloc(&P->keys_alt, cap);
+  if (!rc) rc = dalloc(&P->vals, cap);
+  if (!rc) rc = dalloc(&P->vals_alt, cap);
+  if (!rc) rc = dalloc(&P->hist, sort_hist_words(g.n));
+  if (!rc) rc = dalloc(&P->ev_ptr, (size_t)g.n + 2);
+  return rc;
+}
+
+extern "C" {
+
+int snc_hip_available(void) { return device_ok() ? 1 : 0; }
+
+int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
+  if (!gv || !out) return fail(SNC_E_INVALID, "null argument");
+  *out = nullptr;
+  if (!device_ok()) return fail(SNC_E_NODEVICE, "no gfx950 device");
+  const std::string err = validate_view(*gv);
+  if (!err.empty()) return fail(SNC_E_INVALID, "graph validation failed: " + err);
+  const int n = gv->num_neurons;
+  const int64_t S = gv->num_synapses;
+  std::unique_ptr<snc_hip_graph> H(new snc_hip_graph());
+  DevGraph& g = H->g;
+  g.n = n;
+  g.S = S;
+  int md = 1;
+  for (int64_t e = 0; e < S; ++e) md = std::max(md, (int)gv->delays[e]);
+  if (md > 255) return fail(SNC_E_UNSUPPORTED, "delays above 255 steps are not supported");
+  g.max_delay = md;
+  g.L = md + 1;
+
+  std::vector<float> wsig(S);
+  std::vector<uint8_t> del8(S);
+  g.out_degree.assign(n, 0);
+  g.in_degree.assign(n, 0);
+  for (int i = 0; i < n; ++i) {
+    const float s = (float)gv->sign[i];
+    g.out_degree[i] = gv->row_ptr[i + 1] - gv->row_ptr[i];
+    for (int e = gv->row_ptr[i]; e < gv->row_ptr[i + 1]; ++e) {
+      wsig[e] = s * gv->weights[e];
+      del8[e] = (uint8_t)gv->delays[e];
+      g.in_degree[gv->post_ids[e]]++;
+    }
+  }
+  // Padded incoming columns: the CSR walk (pre ascending, e ascending)
+  // appends to each target's column, so columns come out in CPU fold order.
+  std::vector<int32_t> ccol((size_t)n + 1, 0);
+  int64_t P = 0;
+  for (int j = 0; j < n; ++j) {
+    ccol[j] = (int32_t)P;
+    P += ((int64_t)g.in_degree[j] + 31) / 32 * 32;
+    if (P > 0x7fffff00LL) return fail(SNC_E_UNSUPPORTED, "graph too large for 32-bit columns");
+  }
+  ccol[n] = (int32_t)P;
+  g.P = P;
+  std::vector<int32_t> cpos(S), fillc(n, 0);
+  std::vector<float> cval((size_t)P, 0.0f);
+  std::vector<uint8_t> cdel((size_t)P, 0);
+  for (int64_t e = 0; e < S; ++e) {
+    const int j = gv->post_ids[e];
+    const int32_t p = ccol[j] + fillc[j]++;
+    cpos[e] = p;
+    cval[p] = wsig[e];
+    cdel[p] = del8[e];
+  }
+  std::vector<int32_t> heavy;
+  std::vector<uint8_t> is_heavy(n, 0);
+  int64_t heavy_syn = 0;
+  for (int j = 0; j < n; ++j)
+    if (g.in_degree[j] > g.heavy_threshold) {
+      heavy.push_back(j);
+      is_heavy[j] = 1;
+      heavy_syn += g.in_degree[j];
+    }
+  g.n_heavy = (int)heavy.size();
+
+  int rc = SNC_OK;
+  if (!rc) rc = dupload(&g.row_ptr, gv->row_ptr, (size_t)n + 1);
+  if (!rc) rc = dupload(&g.post, gv->post_ids, (size_t)S);
+  if (!rc) rc = dupload(&g.wsig, wsig.data(), (size_t)S);
+  if (!rc) rc = dupload(&g.delay, del8.data(), (size_t)S);
+  if (!rc) rc = dupload(&g.cpos, cpos.data(), (size_t)S);
+  if (!rc) rc = dupload(&g.ccol, ccol.data(), (size_t)n + 1);
+  if (!rc) rc = dupload(&g.cval, cval.data(), (size_t)P);
+  if (!rc) rc = dupload(&g.cdel, cdel.data(), (size_t)P);
+  if (!rc) rc = dupload(&g.heavy_ids, heavy.data(), heavy.size());
+  if (!rc) rc = dupload(&g.is_heavy, is_heavy.data(), (size_t)n);
+  if (rc) return rc;
+
+  g.role.assign(gv->role, gv->role + n);
+  g.channel.assign(gv->channel, gv->channel + n);
+  int n_in_ch = 0;
+  for (int i = 0; i < n; ++i) {
+    if (g.role[i] == SNC_ROLE_OUTPUT) g.n_out_channels = std::max(g.n_out_channels, g.channel[i] + 1);
+    if (g.role[i] == SNC_ROLE_INPUT) n_in_ch = std::max(n_in_ch, g.channel[i] + 1);
+  }
+  g.chan_to_inputs.assign(n_in_ch, {});
+  for (int i = 0; i < n; ++i)
+    if (g.role[i] == SNC_ROLE_INPUT && g.channel[i] >= 0) g.chan_to_inputs[g.channel[i]].push_back(i);
+
+  snc_graph_profile& pf = g.profile;
+  pf.num_neurons = n;
+  pf.num_synapses = S;
+  pf.max_delay = md;
+  pf.ring_len = g.L;
+  pf.max_in_degree = n ? *std::max_element(g.in_degree.begin(), g.in_degree.end()) : 0;
+  pf.max_out_degree = n ? *std::max_element(g.out_degree.begin(), g.out_degree.end()) : 0;
+  pf.mean_in_degree = n ? (double)S / n : 0.0;
+  pf.heavy_fraction = S ? (double)heavy_syn / (double)S : 0.0;
+  *out = H.release();
+  return SNC_OK;
+}
+
+void snc_hip_graph_free(snc_hip_graph* g) { delete g; }
+
+int snc_hip_graph_profile_get(const snc_hip_graph* g, snc_graph_profile* out) {
+  if (!g || !out) return fail(SNC_E_INVALID, "null argument");
+  *out = g->g.profile;
+  return SNC_OK;
+}
+
+int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_events,
+                        int32_t num_steps, const snc_lif* lif, int32_t variant,
+                        snc_hip_plan** out) {
+  if (!G || !lif || !out || (n_events > 0 && !events) || num_steps < 0)
+    return fail(SNC_E_INVALID, "bad plan arguments");
+  *out = nullptr;
+  if (lif->reset_mode != SNC_RESET_TO_VALUE && lif->reset_mode != SNC_RESET_SUBTRACT)
+    return fail(SNC_E_INVALID, "unknown reset mode");
+  const DevGraph& g = G->g;
+  std::unique_ptr<snc_hip_plan> P(new snc_hip_plan());
+  P->G = G;
+  P->T = num_steps;
+  P->lif = *lif;
+  if (variant == SNC_VARIANT_AUTO) variant = autotune_variant(g, n_events, num_steps);
+  if (variant != SNC_VARIANT_ATOMIC && variant != SNC_VARIANT_BUCKET && variant != SNC_VARIANT_SORT)
+    return fail(SNC_E_INVALID, "unknown variant");
+  P->variant = variant;
+  P->core.variant = variant;
+  const int n = g.n;
+
+  // Per-step injections, resolved channel -> INPUT neurons in the reference's
+  // order (cpu_backend.cpp:50-63) and folded per neuron in that order, so the
+  // device writes one pre-summed ext value per (step, neuron).
+  std::vector<std::vector<std::pair<int32_t, float>>> by_step(num_steps);
+  const int n_in_ch = (int)g.chan_to_inputs.size();
+  for (int64_t k = 0; k < n_events; ++k) {
+    const snc_event& e = events[k];
+    if (e.time < 0 || e.time >= num_steps) continue;
+    if (e.channel_id < 0 || e.channel_id >= n_in_ch) continue;
+    for (int nid : g.chan_to_inputs[e.channel_id]) by_step[e.time].push_back({nid, e.amplitude});
+  }
+  std::vector<int32_t> ip((size_t)num_steps + 1, 0), inid;
+  std::vector<float> iamp;
+  for (int t = 0; t < num_steps; ++t) {
+    auto& lst = by_step[t];
+    std::stable_sort(lst.begin(), lst.end(),
+                     [](const auto& x, const auto& y) { return x.first < y.first; });
+    for (size_t k = 0; k < lst.size();) {
+      float acc = 0.0f;
+      size_t q = k;
+      for (; q < lst.size() && lst[q].first == lst[k].first; ++q) acc += lst[q].second;
+      inid.push_back(lst[k].first);
+      iamp.push_back(acc);
+      k = q;
+    }
+    ip[t + 1] = (int32_t)inid.size();
+    P->core.max_inj_per_step = std::max<int64_t>(P->core.max_inj_per_step, ip[t + 1] - ip[t]);
+  }
+  P->h_inj_ptr = ip;
+
+  int rc = SNC_OK;
+  if (!rc) rc = dupload(&P->inj_ptr, ip.data(), ip.size());
+  if (!rc) rc = dupload(&P->inj_nid, inid.data(), inid.size());
+  if (!rc) rc = dupload(&P->inj_amp, iamp.data(), iamp.size());
+  if (!rc) rc = dalloc(&P->v, (size_t)n);
+  if (!rc) rc = dalloc(&P->ext, (size_t)n);
+  if (!rc) rc = dalloc(&P->ring, (size_t)g.L * n);
+  if (!rc) rc = dalloc(&P->refr, (size_t)n);
+  if (!rc) rc = dalloc(&P->fired, (size_t)n);
+  if (!rc) rc = dalloc(&P->fcnt, 2);
+  if (!rc) rc = dalloc(&P->nspk, (size_t)n);
+  if (!rc) rc = dalloc(&P->dirty, ((size_t)n + 3) / 4 * 4);  // cleared in 4-byte words
+  P->n_bits_words = (size_t)(g.P / 32);
+  if (!rc && variant == SNC_VARIANT_BUCKET) rc = dalloc(&P->bits, P->n_bits_words);
+  P->core.lif_grid = std::max(1, (n + 255) / 256);
+  P->core.expand_grid = std::max(1, std::min(2048, (n + 255) / 256));
+  P->core.n_heavy = variant == SNC_VARIANT_BUCKET ? g.n_heavy : 0;
+  if (!rc) rc = dalloc(&P->blk_spikes, (size_t)std::max(P->core.lif_grid, 4096));
+  if (!rc) rc = dalloc(&P->blk_events, (size_t)std::max(P->core.expand_grid, 4096));
+  if (!rc && variant == SNC_VARIANT_SORT) rc = sort_alloc(P.get());
+  if (rc) return rc;
+  if (hipStreamCreateWithFlags(&P->core.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&P->ev0) != hipSuccess || hipEventCreate(&P->ev1) != hipSuccess)
+    return fail(SNC_E_HIP, "stream/event creation failed");
+
+  StepArgs& a = P->a;
+  a.n = n;
+  a.L = g.L;
+  a.T = num_steps;
+  a.words = (n + 31) / 32;
+  a.lif = {lif->threshold, lif->decay, lif->reset, lif->refractory,
+           lif->reset_mode == SNC_RESET_SUBTRACT ? 1 : 0};
+  a.row_ptr = g.row_ptr;
+  a.post = g.post;
+  a.wsig = g.wsig;
+  a.delay = g.delay;
+  a.cpos = g.cpos;
+  a.ccol = g.ccol;
+  a.cval = g.cval;
+  a.cdel = g.cdel;
+  a.heavy_ids = g.heavy_ids;
+  a.inj_ptr = P->inj_ptr;
+  a.inj_nid = P->inj_nid;
+  a.inj_amp = P->inj_amp;
+  a.v = P->v;
+  a.refr = P->refr;
+  a.ext = P->ext;
+  a.ring = P->ring;
+  a.dirty = P->dirty;
+  a.bits = P->bits;
+  a.fired = P->fired;
+  a.fcnt = P->fcnt;
+  a.nspk = P->nspk;
+  a.blk_spikes = P->blk_spikes;
+  a.blk_events = P->blk_events;
+  a.raster = nullptr;
+  a.keys = P->keys;
+  a.keys_alt = P->keys_alt;
+  a.vals = P->vals;
+  a.vals_alt = P->vals_alt;
+  a.hist = P->hist;
+  a.ev_ptr = P->ev_ptr;
+
+  P->persistent = persist_choose(g, variant, P->pcfg);
+  *out = P.release();
+  return SNC_OK;
+}
+
+int32_t snc_hip_plan_variant(const snc_hip_plan* p) { return p ? p->variant : -1; }
+
+void snc_hip_plan_free(snc_hip_plan* p) { delete p; }
+
+}  // extern "C"
+
+namespace {
+
+// Clear all per-run state with fill kernels (no memset nodes), then stage the
+// first step's injections.
+int reset_state(snc_hip_plan* P, bool raster) {
+  const DevGraph& g = P->G->g;
+  hipStream_t st = P->core.stream;
+  const int n = g.n;
+  int rc = SNC_OK;
+  auto fill = [&](void* ptr, size_t bytes) {
+    if (!rc && ptr && bytes) rc = fwd_launch_fill(st, ptr, 0u, (bytes + 3) / 4);
+  };
+  fill(P->v, sizeof(float) * n);
+  fill(P->ext, sizeof(float) * n);
+  fill(P->ring, sizeof(float) * (size_t)g.L * n);
+  fill(P->refr, sizeof(int32_t) * n);
+  fill(P->fcnt, sizeof(int32_t) * 2);
+  fill(P->nspk, sizeof(int32_t) * n);
+  fill(P->dirty, ((size_t)n + 3) / 4 * 4);
+  fill(P->bits, sizeof(uint32_t) * P->n_bits_words);
+  fill(P->blk_spikes, sizeof(unsigned long long) * std::max(P->core.lif_grid, 4096));
+  fill(P->blk_events, sizeof(unsigned long long) * std::max(P->core.expand_grid, 4096));
+  if (raster) fill(P->raster, sizeof(uint32_t) * (size_t)P->T * P->a.words);
+  if (!rc) rc = fwd_launch_first_injection(P->core, P->a);
+  return rc;
+}
+
+int ensure_raster(snc_hip_plan* P) {
+  if (P->raster) return SNC_OK;
+  return dalloc(&P->raster, (size_t)std::max(1, P->T) * P->a.words);
+}
+
+int launch_chain(snc_hip_plan* P, const StepArgs& a) {
+  for (int t = 0; t < P->T; ++t) {
+    int rc = P->variant == SNC_VARIANT_SORT ? sort_launch_step(P->core, a, t)
+                                           : fwd_launch_step(P->core, a, t);
+    if (rc) return rc;
+  }
+  return SNC_OK;
+}
+
+bool use_graphs() {
+  const char* e = std::getenv("SNC_HIP_GRAPH");
+  return !(e && e[0] == '0');
+}
+
+int launch_loop(snc_hip_plan* P, bool raster) {
+  StepArgs a = P->a;
+  a.raster = raster ? P->raster : nullptr;
+  if (P->persistent) return persist_launch(P->G->g, P->core, a, P->pcfg);
+  if (!use_graphs() || P->T == 0) return launch_chain(P, a);
+  hipGraphExec_t& ge = P->graph_exec[raster ? 1 : 0];
+  if (!ge) {
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(P->core.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail(SNC_E_HIP, "hipStreamBeginCapture failed");
+    int rc = launch_chain(P, a);
+    hipError_t e = hipStreamEndCapture(P->core.stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess || !graph) return fail(SNC_E_HIP, "hipStreamEndCapture failed");
+    e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      ge = nullptr;
+      return fail(SNC_E_HIP, "hipGraphInstantiate failed");
+    }
+  }
+  return hipGraphLaunch(ge, P->core.stream) == hipSuccess ? SNC_OK
+                                                           : fail(SNC_E_HIP, "hipGraphLaunch failed");
+}
+
+}  // namespace
+
+extern "C" {
+
+int snc_hip_plan_launch(snc_hip_plan* P) {
+  if (!P) return fail(SNC_E_INVALID, "null plan");
+  int rc = reset_state(P, false);
+  if (!rc) rc = launch_loop(P, false);
+  return rc;
+}
+
+int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stats* stats) {
+  if (!P || !stats) return fail(SNC_E_INVALID, "null argument");
+  const DevGraph& g = P->G->g;
+  const bool want_raster = cap && (cap->raster || cap->step_spikes);
+  int rc = SNC_OK;
+  if (want_raster) rc = ensure_raster(P);
+  if (!rc) rc = reset_state(P, want_raster);
+  if (rc) return rc;
+  hipStream_t st = P->core.stream;
+  if (hipEventRecord(P->ev0, st) != hipSuccess) return fail(SNC_E_HIP, "hipEventRecord failed");
+  rc = launch_loop(P, want_raster);
+  if (rc) return rc;
+  if (hipEventRecord(P->ev1, st) != hipSuccess) return fail(SNC_E_HIP, "hipEventRecord failed");
+  if (hipStreamSynchronize(st) != hipSuccess) return fail(SNC_E_HIP, "step loop failed");
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, P->ev0, P->ev1);
+  P->last_ms = ms;
+
+  const int n = g.n;
+  std::vector<unsigned long long> bs(std::max(P->core.lif_grid, 4096)), be(std::max(P->core.expand_grid, 4096));
+  std::vector<int32_t> nspk(n);
+  if (hipMemcpy(bs.data(), P->blk_spikes, bs.size() * sizeof(bs[0]), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(be.data(), P->blk_events, be.size() * sizeof(be[0]), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(nspk.data(), P->nspk, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(SNC_E_HIP, "result readback failed");
+  std::memset(stats, 0, sizeof(*stats));
+  for (auto x : bs) stats->spikes += (int64_t)x;
+  for (auto x : be) stats->synaptic_events += (int64_t)x;
+  stats->used_variant = P->variant;
+  stats->ms = ms;
+  if (cap) {
+    if (cap->neuron_spikes) std::copy(nspk.begin(), nspk.end(), cap->neuron_spikes);
+    if (cap->logits) {
+      // Output spikes summed per channel in float, as the CPU decode does
+      // (cpu_backend.cpp:149-151); exact while counts stay below 2^24.
+      std::fill(cap->logits, cap->logits + g.n_out_channels, 0.0f);
+      for (int i = 0; i < n; ++i)
+        if (g.role[i] == SNC_ROLE_OUTPUT && g.channel[i] >= 0)
+          cap->logits[g.channel[i]] += (float)nspk[i];
+    }
+    if (cap->final_v &&
+        hipMemcpy(cap->final_v, P->v, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(SNC_E_HIP, "voltage readback failed");
+    if (want_raster) {
+      std::vector<uint32_t> r((size_t)P->T * P->a.words);
+      if (!r.empty() && hipMemcpy(r.data(), P->raster, r.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(SNC_E_HIP, "raster readback failed");
+      if (cap->raster) std::copy(r.begin(), r.end(), cap->raster);
+      if (cap->step_spikes)
+        for (int t = 0; t < P->T; ++t) {
+          int64_t c = 0;
+          for (int w = 0; w < P->a.words; ++w) c += __builtin_popcount(r[(size_t)t * P->a.words + w]);
+          cap->step_spikes[t] = c;
+        }
+    }
+  }
+  return SNC_OK;
+}
+
+int snc_hip_plan_kernel_time(const snc_hip_plan* p, double* avg_ms, int64_t* launches) {
+  if (!p || !avg_ms || !launches) return fail(SNC_E_INVALID, "null argument");
+  if (p->persistent) {
+    *avg_ms = p->last_ms;
+    *launches = 1;
+  } else {
+    *avg_ms = p->T > 0 ? p->last_ms / p->T : 0.0;
+    *launches = p->T;
+  }
+  return SNC_OK;
+}
+
+int snc_hip_synchronize(void) {
+  return hipDeviceSynchronize() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "hipDeviceSynchronize failed");
+}
+
+int snc_hip_plan_profile(snc_hip_plan* P, double* kernel_ms, int64_t* launches) {
+  if (!P || !kernel_ms || !launches) return fail(SNC_E_INVALID, "null argument");
+  for (int k = 0; k < SNC_KERNEL_CLASSES; ++k) {
+    kernel_ms[k] = 0.0;
+    launches[k] = 0;
+  }
+  int rc = reset_state(P, false);
+  if (rc) return rc;
+  hipStream_t st = P->core.stream;
+  hipEvent_t ev[4];
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(SNC_E_HIP, "hipEventCreate failed");
+  StepArgs a = P->a;
+  a.raster = nullptr;
+  auto span = [&](int k, hipEvent_t b, hipEvent_t e) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b, e) == hipSuccess) {
+      kernel_ms[k] += ms;
+      launches[k] += 1;
+    }
+  };
+  if (P->persistent) {
+    (void)hipEventRecord(ev[0], st);
+    rc = persist_launch(P->G->g, P->core, a, P->pcfg);
+    (void)hipEventRecord(ev[1], st);
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
+    if (!rc) span(SNC_KERNEL_PERSISTENT, ev[0], ev[1]);
+  } else {
+    const bool heavy = P->variant == SNC_VARIANT_BUCKET && P->core.n_heavy > 0;
+    for (int t = 0; t < P->T && !rc; ++t) {
+      (void)hipEventRecord(ev[0], st);
+      if (P->variant == SNC_VARIANT_SORT) {
+        rc = sort_launch_step(P->core, a, t);
+        (void)hipEventRecord(ev[3], st);
+        if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
+        if (!rc) span(SNC_KERNEL_SORT, ev[0], ev[3]);
+        continue;
+      }
+      rc = fwd_launch_heavy(P->core, a, t);
+      (void)hipEventRecord(ev[1], st);
+      if (!rc) rc = fwd_launch_lif(P->core, a, t);
+      (void)hipEventRecord(ev[2], st);
+      if (!rc) rc = fwd_launch_expand(P->core, a, t);
+      (void)hipEventRecord(ev[3], st);
+      if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
+      if (rc) break;
+      if (heavy && t > 0) span(SNC_KERNEL_FOLD_HEAVY, ev[0], ev[1]);
+      span(SNC_KERNEL_LIF, ev[1], ev[2]);
+      span(SNC_KERNEL_EXPAND, ev[2], ev[3]);
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int snc_hip_forward(const snc_graph_view* gv, const snc_event* events, int64_t n_events,
+                    int32_t num_steps, const snc_lif* lif, int32_t variant,
+                    snc_forward_capture* cap, snc_forward_stats* stats) {
+  snc_hip_graph* G = nullptr;
+  int rc = snc_hip_graph_upload(gv, &G);
+  if (rc) return rc;
+  snc_hip_plan* P = nullptr;
+  rc = snc_hip_plan_create(G, events, n_events, num_steps, lif, variant, &P);
+  if (!rc) rc = snc_hip_plan_run(P, cap, stats);
+  snc_hip_plan_free(P);
+  snc_hip_graph_free(G);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Forward plan: device buffers and kernel argument block of one (graph,
+// sample, variant) forward configuration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "device_graph.hpp"
+#include "snc_hip.h"
+
+namespace snc_amd {
+
+struct LifK {
+  float threshold, decay, reset;
+  int refractory;
+  int subtract;
+};
+
+// Everything a step kernel touches; passed by value.
+struct StepArgs {
+  int n, L, T, words;
+  LifK lif;
+  // graph
+  const int32_t* row_ptr;
+  const int32_t* post;
+  const float* wsig;
+  const uint8_t* delay;
+  const int32_t* cpos;
+  const int32_t* ccol;
+  const float* cval;
+  const uint8_t* cdel;
+  const int32_t* heavy_ids;
+  // inputs (per-step injection CSR, pre-folded per neuron)
+  const int32_t* inj_ptr;
+  const int32_t* inj_nid;
+  const float* inj_amp;
+  // state
+  float* v;
+  int32_t* refr;
+  float* ext;
+  float* ring;
+  uint8_t* dirty;
+  uint32_t* bits;
+  int32_t* fired;
+  int32_t* fcnt;  // [2], indexed by step parity
+  int32_t* nspk;  // [n] per-neuron spike counts
+  unsigned long long* blk_spikes;
+  unsigned long long* blk_events;
+  uint32_t* raster;  // [T][words] or null
+  // sort strategy scratch
+  uint32_t* keys;
+  uint32_t* keys_alt;
+  int32_t* vals;
+  int32_t* vals_alt;
+  int32_t* hist;
+  int32_t* ev_ptr;
+};
+
+struct FwdPlanCore {
+  hipStream_t stream = nullptr;
+  int variant = SNC_VARIANT_ATOMIC;
+  int lif_grid = 1, expand_grid = 1;
+  int n_heavy = 0;
+  int64_t max_inj_per_step = 0;
+};
+
+int fwd_launch_fill(hipStream_t st, void* p, uint32_t value, size_t words);
+int fwd_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t);
+int fwd_launch_heavy(const FwdPlanCore& pc, const StepArgs& a, int t);
+int fwd_launch_lif(const FwdPlanCore& pc, const StepArgs& a, int t);
+int fwd_launch_expand(const FwdPlanCore& pc, const StepArgs& a, int t);
+int fwd_launch_first_injection(const FwdPlanCore& pc, const StepArgs& a);
+
+// hip-sort (sort.hip)
+size_t sort_hist_words(int n);
+int sort_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t);
+
+}  // namespace snc_amd
