@@ -103,6 +103,10 @@ struct snc_hip_plan {
   size_t n_bits_words = 0;
   unsigned long long *blk_spikes = nullptr, *blk_events = nullptr;
   uint32_t* raster = nullptr;
+  uint32_t* fbits = nullptr;
+  int32_t* heavy_blk_ptr = nullptr;
+  int halves = 1;
+  size_t dirty_stride = 0;
   uint32_t *keys = nullptr, *keys_alt = nullptr;
   int32_t *vals = nullptr, *vals_alt = nullptr, *hist = nullptr, *ev_ptr = nullptr;
   size_t sort_capacity = 0;
@@ -133,6 +137,8 @@ struct snc_hip_plan {
     dfree(blk_spikes);
     dfree(blk_events);
     dfree(raster);
+    dfree(fbits);
+    dfree(heavy_blk_ptr);
     dfree(keys);
     dfree(keys_alt);
     dfree(vals);
@@ -155,6 +161,10 @@ static int sort_alloc(snc_hip_plan* P) {
   if (!rc) rc = dalloc(&P->vals_alt, cap);
   if (!rc) rc = dalloc(&P->hist, sort_hist_words(g.n));
   if (!rc) rc = dalloc(&P->ev_ptr, (size_t)g.n + 2);
+  if (!rc) rc = dalloc(&P->fbits, ((size_t)g.n + 31) / 32);
+  int bits = 1;
+  while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)g.n * (uint64_t)g.L) ++bits;
+  P->core.sort_bits = bits;
   return rc;
 }
 
@@ -224,6 +234,7 @@ int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
       heavy_syn += g.in_degree[j];
     }
   g.n_heavy = (int)heavy.size();
+  g.heavy_host = heavy;
 
   int rc = SNC_OK;
   if (!rc) rc = dupload(&g.row_ptr, gv->row_ptr, (size_t)n + 1);
@@ -320,26 +331,42 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   }
   P->h_inj_ptr = ip;
 
+  // One persistent launch per run when the device and graph allow it; its
+  // ext, bitmap and dirty flags are double-buffered by step parity.
+  P->persistent = persist_choose(g, variant, P->pcfg);
+  const int halves = P->persistent ? 2 : 1;
+  const size_t dirty_stride = ((size_t)n + 3) / 4 * 4;
   int rc = SNC_OK;
   if (!rc) rc = dupload(&P->inj_ptr, ip.data(), ip.size());
   if (!rc) rc = dupload(&P->inj_nid, inid.data(), inid.size());
   if (!rc) rc = dupload(&P->inj_amp, iamp.data(), iamp.size());
   if (!rc) rc = dalloc(&P->v, (size_t)n);
-  if (!rc) rc = dalloc(&P->ext, (size_t)n);
+  if (!rc) rc = dalloc(&P->ext, (size_t)n * halves);
   if (!rc) rc = dalloc(&P->ring, (size_t)g.L * n);
   if (!rc) rc = dalloc(&P->refr, (size_t)n);
   if (!rc) rc = dalloc(&P->fired, (size_t)n);
   if (!rc) rc = dalloc(&P->fcnt, 2);
   if (!rc) rc = dalloc(&P->nspk, (size_t)n);
-  if (!rc) rc = dalloc(&P->dirty, ((size_t)n + 3) / 4 * 4);  // cleared in 4-byte words
+  if (!rc) rc = dalloc(&P->dirty, dirty_stride * halves);  // cleared in 4-byte words
   P->n_bits_words = (size_t)(g.P / 32);
-  if (!rc && variant == SNC_VARIANT_BUCKET) rc = dalloc(&P->bits, P->n_bits_words);
+  if (!rc && variant == SNC_VARIANT_BUCKET) rc = dalloc(&P->bits, P->n_bits_words * halves);
   P->core.lif_grid = std::max(1, (n + 255) / 256);
   P->core.expand_grid = std::max(1, std::min(2048, (n + 255) / 256));
   P->core.n_heavy = variant == SNC_VARIANT_BUCKET ? g.n_heavy : 0;
   if (!rc) rc = dalloc(&P->blk_spikes, (size_t)std::max(P->core.lif_grid, 4096));
   if (!rc) rc = dalloc(&P->blk_events, (size_t)std::max(P->core.expand_grid, 4096));
+  if (variant == SNC_VARIANT_SORT && (uint64_t)n * (uint64_t)g.L >= (1ull << 32))
+    return fail(SNC_E_UNSUPPORTED, "hip-sort keys need n * ring_len < 2^32");
   if (!rc && variant == SNC_VARIANT_SORT) rc = sort_alloc(P.get());
+  if (!rc && P->persistent) {
+    const std::vector<int32_t> hp = persist_heavy_ptr(g, g.heavy_host, P->pcfg);
+    rc = dupload(&P->heavy_blk_ptr, hp.data(), hp.size());
+    P->pcfg.heavy_blk_ptr = P->heavy_blk_ptr;
+    P->pcfg.bits_words = (int64_t)P->n_bits_words;
+    P->pcfg.dirty_stride = (int64_t)dirty_stride;
+  }
+  P->halves = halves;
+  P->dirty_stride = dirty_stride;
   if (rc) return rc;
   if (hipStreamCreateWithFlags(&P->core.stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&P->ev0) != hipSuccess || hipEventCreate(&P->ev1) != hipSuccess)
@@ -382,8 +409,8 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   a.vals_alt = P->vals_alt;
   a.hist = P->hist;
   a.ev_ptr = P->ev_ptr;
+  a.fbits = P->fbits;
 
-  P->persistent = persist_choose(g, variant, P->pcfg);
   *out = P.release();
   return SNC_OK;
 }
@@ -407,13 +434,13 @@ int reset_state(snc_hip_plan* P, bool raster) {
     if (!rc && ptr && bytes) rc = fwd_launch_fill(st, ptr, 0u, (bytes + 3) / 4);
   };
   fill(P->v, sizeof(float) * n);
-  fill(P->ext, sizeof(float) * n);
+  fill(P->ext, sizeof(float) * n * P->halves);
   fill(P->ring, sizeof(float) * (size_t)g.L * n);
   fill(P->refr, sizeof(int32_t) * n);
   fill(P->fcnt, sizeof(int32_t) * 2);
   fill(P->nspk, sizeof(int32_t) * n);
-  fill(P->dirty, ((size_t)n + 3) / 4 * 4);
-  fill(P->bits, sizeof(uint32_t) * P->n_bits_words);
+  fill(P->dirty, P->dirty_stride * P->halves);
+  fill(P->bits, sizeof(uint32_t) * P->n_bits_words * P->halves);
   fill(P->blk_spikes, sizeof(unsigned long long) * std::max(P->core.lif_grid, 4096));
   fill(P->blk_events, sizeof(unsigned long long) * std::max(P->core.expand_grid, 4096));
   if (raster) fill(P->raster, sizeof(uint32_t) * (size_t)P->T * P->a.words);

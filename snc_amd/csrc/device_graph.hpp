@@ -46,6 +46,7 @@ struct DevGraph {
   std::vector<int32_t> in_degree, out_degree;
   int n_out_channels = 0;
   std::vector<std::vector<int32_t>> chan_to_inputs;  // INPUT neurons per channel, ascending
+  std::vector<int32_t> heavy_host;                   // heavy target ids, ascending
   snc_graph_profile profile{};
 
   ~DevGraph();

@@ -36,6 +36,7 @@
 
 #include "common.hpp"
 #include "device_graph.hpp"
+#include "fold.hpp"
 #include "forward_plan.hpp"
 #include "wave.hpp"
 
@@ -59,23 +60,6 @@ __global__ void k_scatter_inj(const int32_t* __restrict__ inj_ptr,
   for (int k = b + (int)(blockIdx.x * blockDim.x + threadIdx.x); k < e;
        k += (int)(gridDim.x * blockDim.x))
     ext[inj_nid[k]] = inj_amp[k];
-}
-
-__device__ __forceinline__ bool lif_step(float& v, int& refr, float ext, float slot,
-                                         const LifK& p) {
-  if (refr > 0) {
-    --refr;
-    if (!p.subtract) v = p.reset;
-    return false;
-  }
-  const float x = __fadd_rn(__fadd_rn(__fmul_rn(v, p.decay), ext), slot);
-  if (x >= p.threshold) {
-    v = p.subtract ? __fsub_rn(x, p.threshold) : p.reset;
-    refr = p.refractory;
-    return true;
-  }
-  v = x;
-  return false;
 }
 
 // Appends the wave's fired neurons to the fired list, records the raster
@@ -109,39 +93,6 @@ __device__ __forceinline__ void block_accumulate(int wave_count, unsigned long l
   if (threadIdx.x == 0 && s_sum) *slot += (unsigned long long)s_sum;
 }
 
-// Ordered fold of one light target's pending deposits (made at step t-1) into
-// its ring slots: walk the presence bits of its column in ascending order and
-// add each value to slot (t-1 + delay) % L. A running accumulator is kept per
-// slot run; switching slots writes it back, so every (target, slot) sum is
-// the sequential sum in column order.
-__device__ __forceinline__ void fold_light(int i, int n, int L, int tprev,
-                                           const int32_t* __restrict__ ccol,
-                                           const float* __restrict__ cval,
-                                           const uint8_t* __restrict__ cdel,
-                                           uint32_t* __restrict__ bits, float* __restrict__ ring) {
-  const int w0 = ccol[i] >> 5, w1 = ccol[i + 1] >> 5;
-  int cur = -1;
-  float acc = 0.0f;
-  for (int w = w0; w < w1; ++w) {
-    uint32_t b = bits[w];
-    if (!b) continue;
-    bits[w] = 0u;
-    while (b) {
-      const int p = (w << 5) | __builtin_ctz(b);
-      b &= b - 1u;
-      int s = tprev + (int)cdel[p];
-      if (s >= L) s -= L;
-      if (s != cur) {
-        if (cur >= 0) ring[(size_t)cur * n + i] = acc;
-        cur = s;
-        acc = ring[(size_t)s * n + i];
-      }
-      acc = __fadd_rn(acc, cval[p]);
-    }
-  }
-  if (cur >= 0) ring[(size_t)cur * n + i] = acc;
-}
-
 // Integrate/fire for every neuron (step t, slot tm = t % L). With FOLD the
 // lane first folds its target's pending bucket (hip-bucket).
 template <bool FOLD>
@@ -163,79 +114,25 @@ __global__ __launch_bounds__(256) void k_lif(StepArgs a, int t, int tm) {
     *slotp = 0.0f;
     if (f) a.nspk[i] += 1;
   }
+  if (a.fbits) {  // every wave rewrites its own words, zeros included
+    const uint64_t m = ballot(f);
+    const int first = i - lane_id();
+    if (lane_id() == 0 && first < a.n) {
+      a.fbits[first >> 5] = (uint32_t)m;
+      if ((first >> 5) + 1 < a.words) a.fbits[(first >> 5) + 1] = (uint32_t)(m >> 32);
+    }
+  }
   const int cnt = wave_emit_fired(f, i, a.fired, a.fcnt + (t & 1),
                                   a.raster ? a.raster + (size_t)t * a.words : nullptr, a.words);
   block_accumulate(cnt, a.blk_spikes + blockIdx.x);
 }
 
-// Wave-per-heavy-target ordered fold (hip-bucket). The wave gathers the set
-// positions of the target's column in ascending order into LDS, then lane s
-// folds, in that order, every value bound for ring slot s (lanes cover slots
-// s, s+64, ... for rings longer than 64).
-constexpr int kHeavyChunk = 2048;
-constexpr int kMaxSlotsPerLane = 4;  // rings up to 256 slots
-
+// Wave-per-heavy-target ordered fold (hip-bucket), one wave per block.
 __global__ __launch_bounds__(64) void k_fold_heavy(StepArgs a, int tprev) {
   __shared__ float s_val[kHeavyChunk];
   __shared__ uint8_t s_slot[kHeavyChunk];
-  const int h = a.heavy_ids[blockIdx.x];
-  if (!a.dirty[h]) return;
-  const int lane = lane_id();
-  const int L = a.L, n = a.n;
-  float acc[kMaxSlotsPerLane];
-#pragma unroll
-  for (int k = 0; k < kMaxSlotsPerLane; ++k) {
-    const int s = lane + k * kWave;
-    acc[k] = s < L ? a.ring[(size_t)s * n + h] : 0.0f;
-  }
-  int fill = 0;
-  auto drain = [&]() {
-    for (int q = 0; q < fill; ++q) {
-      const int s = s_slot[q];
-      const float x = s_val[q];
-#pragma unroll
-      for (int k = 0; k < kMaxSlotsPerLane; ++k)
-        if (s == lane + k * kWave) acc[k] = __fadd_rn(acc[k], x);
-    }
-    fill = 0;
-  };
-  const int w0 = a.ccol[h] >> 5, w1 = a.ccol[h + 1] >> 5;
-  for (int wb = w0; wb < w1; wb += kWave) {
-    const int w = wb + lane;
-    uint32_t b = 0;
-    if (w < w1) {
-      b = a.bits[w];
-      if (b) a.bits[w] = 0u;
-    }
-    const int c = __popc(b);
-    const int incl = wave_inclusive_scan(c);
-    const int total = __shfl(incl, kWave - 1, kWave);
-    if (total == 0) continue;
-    if (fill + total > kHeavyChunk) {
-      __syncthreads();
-      drain();
-      __syncthreads();
-    }
-    int o = fill + incl - c;
-    while (b) {
-      const int p = (w << 5) | __builtin_ctz(b);
-      b &= b - 1u;
-      int s = tprev + (int)a.cdel[p];
-      if (s >= L) s -= L;
-      s_slot[o] = (uint8_t)s;
-      s_val[o] = a.cval[p];
-      ++o;
-    }
-    fill += total;
-  }
-  __syncthreads();
-  drain();
-#pragma unroll
-  for (int k = 0; k < kMaxSlotsPerLane; ++k) {
-    const int s = lane + k * kWave;
-    if (s < L) a.ring[(size_t)s * n + h] = acc[k];
-  }
-  if (lane == 0) a.dirty[h] = 0;
+  fold_heavy_wave<kHeavyChunk>(a.heavy_ids[blockIdx.x], tprev, a.L, a.n, a.ccol, a.cval, a.cdel,
+                               a.bits, a.dirty, a.ring, s_val, s_slot);
 }
 
 // Expansion of step t's fired rows, one wave per row, lanes striding the row

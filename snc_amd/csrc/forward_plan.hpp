@@ -49,6 +49,7 @@ struct StepArgs {
   unsigned long long* blk_spikes;
   unsigned long long* blk_events;
   uint32_t* raster;  // [T][words] or null
+  uint32_t* fbits;   // [words] fired bitmap of the current step (hip-sort) or null
   // sort strategy scratch
   uint32_t* keys;
   uint32_t* keys_alt;
@@ -64,6 +65,7 @@ struct FwdPlanCore {
   int lif_grid = 1, expand_grid = 1;
   int n_heavy = 0;
   int64_t max_inj_per_step = 0;
+  int sort_bits = 0;  // key bits of hip-sort: ceil(log2(n * L))
 };
 
 int fwd_launch_fill(hipStream_t st, void* p, uint32_t value, size_t words);
