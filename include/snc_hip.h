@@ -169,6 +169,33 @@ int snc_hip_forward(const snc_graph_view* g, const snc_event* events, int64_t n_
                     int32_t num_steps, const snc_lif* lif, int32_t variant,
                     snc_forward_capture* cap, snc_forward_stats* stats);
 
+/* ---- e-prop training (cudatrain::*, include/snc/cuda_trainer.hpp:24-57) ---- */
+
+#define SNC_READOUT_SPIKING 0 /* logits = output spike counts (trainer.cpp:122-123)   */
+#define SNC_READOUT_LEAKY 1   /* outputs never fire; logits = mean membrane over T_b   */
+
+#define SNC_ENC_DIRECT 0
+#define SNC_ENC_POISSON 1 /* on-device Philox Bernoulli, see snc_hip_train_batch */
+
+/* TrainConfig (include/snc/trainer.hpp:30-48) plus the readout mode. */
+typedef struct snc_train_config {
+  int32_t num_steps; /* T when per-sample lengths are not given              */
+  snc_lif lif;
+  float lr;
+  float w_init;          /* weights start uniform [0, w_init] (trainer.cpp:15-18) */
+  float w_max;
+  float surrogate_scale; /* gamma of the triangular surrogate                */
+  float feedback_scale;  /* sd of the random feedback matrix B               */
+  int32_t train_hidden;
+  int32_t readout_mode;  /* SNC_READOUT_*                                    */
+  float readout_psi;     /* surrogate of a leaky readout neuron (constant)   */
+  int32_t encoder;       /* SNC_ENC_* for feature inputs                     */
+  float enc_gain;
+  float enc_max_rate;
+  float enc_threshold;
+  uint64_t seed;
+} snc_train_config;
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,28 @@ void oracle_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 /* SplitMix64 (Steele, Lea, Flood, OOPSLA'14): next value, advancing *state. */
 uint64_t oracle_splitmix64(uint64_t* state);
 
+/* ---- e-prop trainer (oracle_train.cpp) ---------------------------------- */
+void* oracle_trainer_create(const snc_graph_view* g, const snc_train_config* cfg);
+void oracle_trainer_free(void* h);
+void oracle_trainer_get_weights(void* h, float* w);
+void oracle_trainer_set_weights(void* h, const float* w);
+void oracle_trainer_get_feedback(void* h, float* B);
+void oracle_trainer_reset_stats(void* h);
+void oracle_trainer_stats(void* h, int64_t* syn_deliv, int64_t* neuron_fires);
+int64_t oracle_trainer_run_sample(void* h, const snc_event* ev, int64_t n_ev, int32_t sample,
+                                  int32_t T, float* logits, double* elig, uint32_t* raster,
+                                  int64_t* events, int32_t record_stats);
+double oracle_trainer_train_epoch_ref(void* h, const float* X, const int32_t* y, int32_t n,
+                                      int32_t dim, int32_t epoch, double* loss, int64_t* spikes,
+                                      int64_t* events);
+void oracle_trainer_minibatch(void* h, const snc_event* ev, int64_t n_ev, const int32_t* lengths,
+                              const int32_t* labels, int32_t B, int32_t apply, float* logits_out,
+                              uint32_t* raster_out, int32_t t_max, int64_t* spikes,
+                              int64_t* events);
+int64_t oracle_encode_philox(const float* x, int32_t n_ch, int32_t T, float gain, float max_rate,
+                             float thr, uint64_t seed, int32_t sample, int32_t epoch,
+                             snc_event* out, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -296,5 +296,182 @@ def ref_raster(g, events, num_steps, lif):
 # trainer (bound in _bind_train once oracle_train.cpp provides it)
 # ---------------------------------------------------------------------------
 
+class TrainConfig(C.Structure):
+    """snc_train_config (include/snc_hip.h)."""
+    _fields_ = [("num_steps", C.c_int32), ("lif", _Lif), ("lr", C.c_float),
+                ("w_init", C.c_float), ("w_max", C.c_float), ("surrogate_scale", C.c_float),
+                ("feedback_scale", C.c_float), ("train_hidden", C.c_int32),
+                ("readout_mode", C.c_int32), ("readout_psi", C.c_float), ("encoder", C.c_int32),
+                ("enc_gain", C.c_float), ("enc_max_rate", C.c_float),
+                ("enc_threshold", C.c_float), ("seed", C.c_uint64)]
+
+
+def train_config(num_steps=30, lif=None, lr=0.02, w_init=0.3, w_max=4.0, gamma=0.3,
+                 feedback=1.0, train_hidden=True, readout_mode=0, readout_psi=1.0, encoder=1,
+                 gain=1.0, max_rate=0.5, enc_threshold=0.0, seed=1):
+    lc = _lif(lif) if lif is not None else _Lif(1.0, 0.9, 0.0, 1, 0)
+    return TrainConfig(num_steps, lc, lr, w_init, w_max, gamma, feedback,
+                       1 if train_hidden else 0, readout_mode, readout_psi, encoder, gain,
+                       max_rate, enc_threshold, seed)
+
+
 def _bind_train(lib):
-    pass
+    V, I32, I64, F, D, U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_uint64
+    sig = {
+        "oracle_trainer_create": (V, [C.POINTER(_GraphView), C.POINTER(TrainConfig)]),
+        "oracle_trainer_free": (None, [V]),
+        "oracle_trainer_get_weights": (None, [V, V]),
+        "oracle_trainer_set_weights": (None, [V, V]),
+        "oracle_trainer_get_feedback": (None, [V, V]),
+        "oracle_trainer_reset_stats": (None, [V]),
+        "oracle_trainer_stats": (None, [V, V, V]),
+        "oracle_trainer_run_sample": (I64, [V, V, I64, I32, I32, V, V, V, V, I32]),
+        "oracle_trainer_train_epoch_ref": (D, [V, V, V, I32, I32, I32, V, V, V]),
+        "oracle_trainer_minibatch": (None, [V, V, I64, V, V, I32, I32, V, V, I32, V, V]),
+        "oracle_encode_philox": (I64, [V, I32, I32, F, F, F, U64, I32, I32, V, I64]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+
+
+class Trainer:
+    """Oracle e-prop trainer over a graph (reference Trainer + extensions)."""
+
+    def __init__(self, g, cfg: TrainConfig):
+        self.g = g
+        self.cfg = cfg
+        self.S = g.num_synapses
+        n_out = int(g.channel[g.role == 2].max() + 1) if (g.role == 2).any() else 0
+        self.classes = max(1, n_out)
+        self.n_internal = int((g.role == 0).sum())
+        self._h = oracle_lib().oracle_trainer_create(C.byref(_view(g)), C.byref(cfg))
+
+    def __del__(self):
+        try:
+            oracle_lib().oracle_trainer_free(self._h)
+        except Exception:
+            pass
+
+    @property
+    def weights(self):
+        w = np.zeros(self.S, np.float32)
+        oracle_lib().oracle_trainer_get_weights(self._h, _p(w))
+        return w
+
+    @weights.setter
+    def weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        oracle_lib().oracle_trainer_set_weights(self._h, _p(w))
+
+    def feedback(self):
+        B = np.zeros(self.n_internal * self.classes, np.float32)
+        oracle_lib().oracle_trainer_get_feedback(self._h, _p(B))
+        return B
+
+    def reset_stats(self):
+        oracle_lib().oracle_trainer_reset_stats(self._h)
+
+    def stats(self):
+        a = np.zeros(self.S, np.int64)
+        b = np.zeros(self.g.num_neurons, np.int64)
+        oracle_lib().oracle_trainer_stats(self._h, _p(a), _p(b))
+        return a, b
+
+    def run_sample(self, events, T, sample=-1, elig=True, raster=False, record_stats=False):
+        ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        logits = np.zeros(self.classes, np.float32)
+        E = np.zeros(self.S, np.float64) if elig else None
+        W = (self.g.num_neurons + 31) // 32
+        R = np.zeros((T, W), np.uint32) if raster else None
+        evn = C.c_int64()
+        sp = oracle_lib().oracle_trainer_run_sample(self._h, _p(ev) if ev.size else None, ev.size,
+                                                    sample, T, _p(logits), _p(E), _p(R),
+                                                    C.byref(evn), 1 if record_stats else 0)
+        return dict(logits=logits, elig=E, raster=R, spikes=sp, events=evn.value)
+
+    def train_epoch_ref(self, X, y, epoch):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        loss, sp, ev = C.c_double(), C.c_int64(), C.c_int64()
+        acc = oracle_lib().oracle_trainer_train_epoch_ref(self._h, _p(X), _p(y), X.shape[0],
+                                                          X.shape[1], epoch, C.byref(loss),
+                                                          C.byref(sp), C.byref(ev))
+        return dict(train_acc=acc, loss=loss.value, spikes=sp.value, events=ev.value)
+
+    def minibatch(self, events, lengths, labels, apply=True, raster=False):
+        ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+        lengths = np.ascontiguousarray(lengths, np.int32)
+        labels = np.ascontiguousarray(labels, np.int32)
+        Bn = lengths.size
+        logits = np.zeros((Bn, self.classes), np.float32)
+        t_max = int(lengths.max()) if Bn else 0
+        W = (self.g.num_neurons + 31) // 32
+        R = np.zeros((Bn, t_max, W), np.uint32) if raster else None
+        sp, evn = C.c_int64(), C.c_int64()
+        oracle_lib().oracle_trainer_minibatch(self._h, _p(ev) if ev.size else None, ev.size,
+                                              _p(lengths), _p(labels), Bn, 1 if apply else 0,
+                                              _p(logits), _p(R), t_max, C.byref(sp),
+                                              C.byref(evn))
+        return dict(logits=logits, raster=R, spikes=sp.value, events=evn.value)
+
+
+def encode_philox(x, T, gain=1.0, max_rate=0.2, threshold=0.0, seed=1, sample=0, epoch=0):
+    x = np.ascontiguousarray(x, np.float32)
+    lib = oracle_lib()
+    k = lib.oracle_encode_philox(_p(x), x.size, T, gain, max_rate, threshold, seed, sample, epoch,
+                                 None, 0)
+    out = np.zeros(k, dtype=EVENT_DTYPE)
+    lib.oracle_encode_philox(_p(x), x.size, T, gain, max_rate, threshold, seed, sample, epoch,
+                             _p(out) if k else None, k)
+    return out
+
+
+class RefTrainer:
+    """The reference's own Trainer (compiled reference), for goldens."""
+
+    def __init__(self, g, cfg: TrainConfig):
+        self.g = g
+        self.S = g.num_synapses
+        L = cfg.lif
+        self._h = ref_lib().ref_trainer_create(*_graph_args(g), cfg.num_steps, L.threshold,
+                                               L.decay, L.reset, L.refractory, cfg.encoder,
+                                               cfg.enc_gain, cfg.enc_max_rate, cfg.enc_threshold,
+                                               cfg.lr, cfg.w_init, cfg.w_max, cfg.surrogate_scale,
+                                               cfg.feedback_scale, cfg.train_hidden, cfg.seed)
+
+    def __del__(self):
+        try:
+            ref_lib().ref_trainer_free(self._h)
+        except Exception:
+            pass
+
+    @property
+    def weights(self):
+        w = np.zeros(self.S, np.float32)
+        ref_lib().ref_trainer_get_weights(self._h, _p(w))
+        return w
+
+    @weights.setter
+    def weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        ref_lib().ref_trainer_set_weights(self._h, _p(w), w.size)
+
+    def stats(self):
+        a = np.zeros(self.S, np.int64)
+        b = np.zeros(self.g.num_neurons, np.int64)
+        ref_lib().ref_trainer_stats(self._h, _p(a), _p(b))
+        return a, b
+
+    def reset_stats(self):
+        ref_lib().ref_trainer_reset_stats(self._h)
+
+    def train_epoch(self, X, y, epoch, classes):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        loss, sp, ev = C.c_double(), C.c_int64(), C.c_int64()
+        acc = ref_lib().ref_trainer_train_epoch(self._h, _p(X), _p(y), X.shape[0], X.shape[1],
+                                                classes, epoch, C.byref(loss), C.byref(sp),
+                                                C.byref(ev))
+        return dict(train_acc=acc, loss=loss.value, spikes=sp.value, events=ev.value)

@@ -94,8 +94,31 @@ def main():
     np.savez_compressed(os.path.join(HERE, "synthetic.npz"), X=X, y=y)
     print("wrote encode.npz synthetic.npz")
 
-    if hasattr(sys.modules[__name__], "write_train"):
-        write_train()  # noqa: F821
+    write_train()
+
+
+def write_train():
+    """snc::Trainer epochs on a small static-snc graph: weights and stats."""
+    for name, w_init, lr, enc in [("train_spiking.npz", 0.3, 0.08, 1),
+                                  ("train_direct.npz", 0.25, 0.05, 0)]:
+        g = G.static_snc([64, 48, 8], 1500, 0.5, 1, 0.2, 4)
+        X, y = O.ref_make_synthetic(12, 64, 8, 0.35, 3)
+        cfg = O.train_config(num_steps=20, lr=lr, w_init=w_init, w_max=4.0, gamma=0.3,
+                             feedback=1.0, encoder=enc, max_rate=0.5, seed=5)
+        r = O.RefTrainer(g, cfg)
+        out = graph_arrays(g)
+        out.update(X=X, y=y, num_steps=np.int32(20), lr=np.float32(lr), w_init=np.float32(w_init),
+                   encoder=np.int32(enc), seed=np.uint64(5), w0=r.weights)
+        for ep in (1, 2):
+            st = r.train_epoch(X, y, ep, 8)
+            syn, neur = r.stats()
+            out[f"w_{ep}"] = r.weights
+            out[f"syn_deliv_{ep}"] = syn
+            out[f"neuron_fires_{ep}"] = neur
+            out[f"epoch_{ep}"] = np.array([st["train_acc"], st["loss"], st["spikes"],
+                                           st["events"]], dtype=np.float64)
+        np.savez_compressed(os.path.join(HERE, name), **out)
+        print("wrote", name)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,174 @@
+"""The e-prop oracle pinned against the reference Trainer and closed forms.
+
+- golden fixtures from the reference's compiled snc::Trainer: weights after
+  each epoch, per-synapse deliveries, per-neuron fires, accuracy/loss/totals
+- live differential epochs against the compiled reference when present
+- closed forms: the eligibility of a one-synapse chain, minibatch linearity
+  (B = 1 equals the per-sample step; B samples equal the mean of the
+  per-sample deltas from the same start), leaky-readout geometric series,
+  the Philox encoder's rate and its counter layout
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN
+from helpers import events, make_graph
+from snc_amd.graph import SNNGraph
+from snc_amd.runtime import LIFParams
+
+
+def _fixture_trainer(name):
+    z = dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))
+    g = SNNGraph(z["row_ptr"], z["post_ids"], z["weights"], z["delays"], z["sign"], z["role"],
+                 z["channel"])
+    cfg = O.train_config(num_steps=int(z["num_steps"]), lr=float(z["lr"]),
+                         w_init=float(z["w_init"]), w_max=4.0, gamma=0.3, feedback=1.0,
+                         encoder=int(z["encoder"]), max_rate=0.5, seed=int(z["seed"]))
+    return g, cfg, z
+
+
+@pytest.mark.parametrize("name", ["train_spiking.npz", "train_direct.npz"])
+def test_trainer_matches_reference_goldens(name):
+    g, cfg, z = _fixture_trainer(name)
+    t = O.Trainer(g, cfg)
+    assert np.array_equal(t.weights, z["w0"])
+    for ep in (1, 2):
+        st = t.train_epoch_ref(z["X"], z["y"], ep)
+        acc, loss, spikes, evn = z[f"epoch_{ep}"]
+        assert st["train_acc"] == acc and st["loss"] == loss
+        assert (st["spikes"], st["events"]) == (int(spikes), int(evn))
+        assert np.array_equal(t.weights, z[f"w_{ep}"])
+        syn, neur = t.stats()
+        assert np.array_equal(syn, z[f"syn_deliv_{ep}"])
+        assert np.array_equal(neur, z[f"neuron_fires_{ep}"])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not present")
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_trainer_differential_vs_compiled_reference(seed):
+    from snc_amd import graph as G
+    g = G.static_snc([40, 30, 20, 5], 900, 0.5, 2, 0.2, seed)
+    X, y = O.ref_make_synthetic(10, 40, 5, 0.3, seed)
+    cfg = O.train_config(num_steps=15 + seed, lr=0.1, w_init=0.35, gamma=0.3, feedback=0.7,
+                         encoder=seed % 3, max_rate=0.6, seed=seed,
+                         lif=LIFParams(1.0, 0.85, 0.1 * (seed - 1), seed % 3))
+    a, r = O.Trainer(g, cfg), O.RefTrainer(g, cfg)
+    for ep in (1, 2):
+        sa, sr = a.train_epoch_ref(X, y, ep), r.train_epoch(X, y, ep, 5)
+        assert sa == sr
+        assert np.array_equal(a.weights, r.weights)
+        assert all(np.array_equal(p, q) for p, q in zip(a.stats(), r.stats()))
+
+
+def test_eligibility_of_one_synapse_chain():
+    # Input a -> output b, weight w, decay d = 1/2, threshold 1. With b far
+    # below threshold its surrogate is psi = gamma (1 - |v - 1|) = gamma v.
+    d = 0.5
+    g = make_graph(2, [(0, 1, 0.5, 1)], roles=[1, 2], channels=[0, 0])
+    cfg = O.train_config(num_steps=6, lif=LIFParams(1.0, d, 0.0, 1), gamma=0.4, w_init=0.0,
+                         readout_mode=0)
+    t = O.Trainer(g, cfg)
+    t.weights = np.array([0.0], np.float32)
+    r = t.run_sample(events([(0, 0, 1.0), (0, 3, 1.0)]), 6)
+    assert r["elig"][0] == 0.0  # b never moves: v = 0, psi = 0
+    t.weights = np.array([0.5], np.float32)
+    r = t.run_sample(events([(0, 0, 1.0)]), 3)
+    # a fires at 0: tr_a = 1/2, 1/4, 1/8. b: v = 0, 1/2, 1/4 so psi = 0,
+    # gamma/2, gamma/4 (float32). E = sum psi * tr in double.
+    g32 = np.float32(0.4)
+    psi1 = np.float32(g32 * np.float32(1 - 0.5))
+    psi2 = np.float32(g32 * np.float32(1 - 0.75))
+    assert r["elig"][0] == float(psi1) * 0.25 + float(psi2) * 0.125
+
+
+def test_minibatch_of_one_equals_per_sample_step():
+    from snc_amd import graph as G
+    g = G.static_snc([30, 20, 4], 400, 0.5, 1, 0.2, 2)
+    cfg = O.train_config(num_steps=25, lr=0.2, w_init=0.4, encoder=0, seed=3)
+    X, y = O.ref_make_synthetic(4, 30, 4, 0.3, 8)
+    a, b = O.Trainer(g, cfg), O.Trainer(g, cfg)
+    ev = O.encode(0, X[0], 25)
+    r = a.run_sample(ev, 25)
+    # per-sample step by hand through the minibatch path with B = 1
+    b.minibatch(ev, [25], [int(y[0])])
+    # reference step: w -= (lr / T * L) * E with clamp; minibatch uses the
+    # product (lr/T * L) * float(E) then divides by B = 1: same float ops
+    # except the reference computes (lr/T * L) * E as scale * L * E.
+    w0 = a.weights
+    assert np.all(np.isfinite(b.weights))
+    moved = b.weights != w0
+    assert moved.any()
+    np.testing.assert_allclose(b.weights - w0, _per_sample_delta(a, ev, 25, int(y[0]), r),
+                               rtol=1e-6, atol=6e-8)  # w is float32: ulp(0.4) ~ 3e-8
+
+
+def _per_sample_delta(t, ev, T, label, run):
+    """Delta of the per-sample rule computed in float64 from the oracle's
+    logits and eligibilities: -(lr/T) * L_post * E."""
+    lg = run["logits"].astype(np.float64)
+    p = np.exp(lg - lg.max())
+    p /= p.sum()
+    delta = p - np.eye(t.classes)[label]
+    g = t.g
+    Bm = t.feedback().reshape(-1, t.classes).astype(np.float64)
+    Ls = np.zeros(g.num_neurons)
+    internal = np.nonzero(g.role == 0)[0]
+    Ls[internal] = Bm @ delta
+    outs = np.nonzero(g.role == 2)[0]
+    Ls[outs] = delta[g.channel[outs]]
+    return -(t.cfg.lr / T) * Ls[g.post_ids] * run["elig"]
+
+
+def test_minibatch_is_mean_of_per_sample_deltas():
+    from snc_amd import graph as G
+    g = G.static_snc([30, 20, 4], 400, 0.5, 1, 0.2, 2)
+    cfg = O.train_config(num_steps=25, lr=0.2, w_init=0.4, encoder=0, seed=3)
+    X, y = O.ref_make_synthetic(6, 30, 4, 0.3, 8)
+    lengths = np.array([25, 18, 30, 22, 25, 19], np.int32)
+    evs = []
+    for b in range(6):
+        e = O.encode(0, X[b], int(lengths[b]))
+        e["sample_id"] = b
+        evs.append(e)
+    ev = np.concatenate(evs)
+    t = O.Trainer(g, cfg)
+    w0 = t.weights
+    deltas = []
+    for b in range(6):
+        run = t.run_sample(ev, int(lengths[b]), sample=b)
+        deltas.append(_per_sample_delta(t, ev, int(lengths[b]), int(y[b]), run))
+    t.minibatch(ev, lengths, y)
+    np.testing.assert_allclose(t.weights - w0, np.mean(deltas, axis=0), rtol=2e-5, atol=6e-8)
+
+
+def test_leaky_readout_is_mean_membrane():
+    # Output b integrates a's spikes (weight w, decay d) and never fires:
+    # with a firing at step 0 only, v_b(t) = w d^(t-1) for t >= 1, so the
+    # logit is w (1 - d^(T-1)) / ((1 - d) T).
+    d = 0.5
+    g = make_graph(2, [(0, 1, 0.75, 1)], roles=[1, 2], channels=[0, 0])
+    cfg = O.train_config(num_steps=8, lif=LIFParams(1.0, d, 0.0, 1), readout_mode=1)
+    t = O.Trainer(g, cfg)
+    t.weights = np.array([0.75], np.float32)
+    r = t.run_sample(events([(0, 0, 1.0)]), 8)
+    T = 8
+    assert r["logits"][0] == np.float32(0.75 * (1 - d ** (T - 1)) / ((1 - d) * T))
+    assert r["spikes"] == 1  # only a
+
+
+def test_philox_encoder_counter_layout_and_rate():
+    x = np.linspace(0, 1, 97).astype(np.float32)
+    ev = O.encode_philox(x, 400, max_rate=0.3, seed=11, sample=5, epoch=2)
+    # every event satisfies its own counter's draw
+    for k in range(0, ev.size, max(1, ev.size // 25)):
+        c, t = int(ev["channel_id"][k]), int(ev["time"][k])
+        r = O.philox4x32([c // 4, t, 5, 2], [11, 0])
+        assert r[c % 4] < np.uint32(min(1.0, np.float32(x[c]) * np.float32(0.3)) * 2.0 ** 32)
+        assert ev["sample_id"][k] == 5
+    # rates follow p = x * max_rate
+    counts = np.bincount(ev["channel_id"], minlength=97) / 400
+    assert np.abs(counts - x * 0.3).max() < 0.08
+    assert counts[0] == 0
