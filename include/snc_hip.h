@@ -196,6 +196,70 @@ typedef struct snc_train_config {
   uint64_t seed;
 } snc_train_config;
 
+#define SNC_TRAIN_PLAIN 0   /* one sample at a time, per-step kernel launches       */
+#define SNC_TRAIN_BATCHED 1 /* whole minibatch in one persistent launch, one
+                               workgroup per sample running its own T_b steps    */
+
+#define SNC_ELIG_F32 0 /* eligibility accumulated in float (fused multiply-add)    */
+#define SNC_ELIG_F64 1 /* in double, exactly the CPU trainer's sums (trainer.cpp:137) */
+
+typedef struct snc_hip_train snc_hip_train;
+
+/* cudatrain::create (cuda_trainer.cu:190-273): device session for a frozen
+ * graph. Weights start uniform [0, w_init], then the feedback matrix
+ * B ~ N(0, feedback_scale), both from one mt19937_64(seed) stream as the CPU
+ * Trainer draws them (trainer.cpp:10-41). batch = largest minibatch;
+ * max_steps = longest sample. elig = SNC_ELIG_*. */
+int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, int32_t batch,
+                         int32_t max_steps, int32_t elig, snc_hip_train** out);
+void snc_hip_train_destroy(snc_hip_train* s);
+
+/* cudatrain::set_weights / get_weights (signed, CSR order). Unlike the
+ * reference, a size mismatch is reported (SNC_E_INVALID), not ignored. */
+int snc_hip_train_set_weights(snc_hip_train* s, const float* w, int64_t n);
+int snc_hip_train_get_weights(const snc_hip_train* s, float* w, int64_t n);
+
+/* Upload a whole dataset once; later steps index it. Features: [n][dim]
+ * values in [0, 1] encoded on the device (cfg.encoder: Philox Poisson with
+ * counter (channel / 4, step, sample index, epoch), or Direct). Events:
+ * per-sample InputEvent lists (sample_id = dataset index). lengths[i] is T_i
+ * (NULL: cfg.num_steps for every sample). */
+int snc_hip_train_upload_features(snc_hip_train* s, const float* X, int32_t n, int32_t dim,
+                                  const int32_t* labels, const int32_t* lengths);
+int snc_hip_train_upload_events(snc_hip_train* s, const snc_event* ev, int64_t n_events,
+                                int32_t n, const int32_t* labels, const int32_t* lengths);
+
+typedef struct snc_train_stats {
+  int64_t spikes;
+  int64_t synaptic_events;
+  double loss_sum; /* sum over samples of -log softmax(logits)[label]          */
+  int32_t correct;
+  int32_t samples;
+  double ms;       /* device time of the step's launches                          */
+} snc_train_stats;
+
+/* One minibatch: the samples idx[0..B) of the uploaded dataset run from the
+ * current weights; when apply is set the weights then move by the
+ * minibatch-mean e-prop update (cuda_trainer.cu:163-172, with lr / T_b per
+ * sample): w <- clamp(w - sum_b (lr / T_b * L_b[post]) * E_b / B, +-w_max).
+ * logits (optional) receives [B][classes]; raster (optional) [B][T_max][ceil(N/32)]. */
+int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t epoch,
+                       int32_t mode, int32_t apply, float* logits, uint32_t* raster,
+                       snc_train_stats* st);
+
+/* cudatrain::train_epoch / evaluate (cuda_trainer.cu:374-435) over the
+ * uploaded dataset: shuffled minibatches (mt19937_64 seeded with
+ * 0x9e3779b97f4a7c15 * (epoch + 1)); evaluate runs without updates. */
+int snc_hip_train_epoch(snc_hip_train* s, int32_t epoch, int32_t mode, snc_train_stats* st);
+int snc_hip_train_evaluate(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t mode,
+                           snc_train_stats* st);
+
+/* cudatrain::reset_stats / collect_stats: per-synapse deliveries and
+ * per-neuron fires accumulated over training steps (not evaluation). */
+int snc_hip_train_reset_stats(snc_hip_train* s);
+int snc_hip_train_collect_stats(const snc_hip_train* s, int64_t* syn_deliveries,
+                                int64_t* neuron_fires);
+
 #ifdef __cplusplus
 }
 #endif

@@ -209,6 +209,16 @@ def lib():
     return _lib
 
 
+def register_signatures(sigs: dict):
+    """Add entry points bound by another module (binds now if already loaded)."""
+    _SIGS.update(sigs)
+    if _lib is not None:
+        for name, (res, args) in sigs.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+
+
 def exported_symbols():
     return list(_SIGS)
 

@@ -68,6 +68,64 @@ def c1_graph(width=500, height=400, seed=1) -> G.SNNGraph:
                              all_input=True, seed=seed)
 
 
+@dataclass
+class TrainWorkload:
+    name: str
+    graph: G.SNNGraph
+    cfg: object            # snc_amd.train.TrainConfig
+    batch: int
+    max_steps: int
+    kind: str              # "features" or "events"
+    X: np.ndarray | None
+    events: np.ndarray | None
+    labels: np.ndarray
+    lengths: np.ndarray
+    n_train: int
+    init_weights: np.ndarray | None = None  # signed start weights (set_weights)
+
+
+def _lengths(n, lo, hi, seed):
+    from .rng_py import splitmix_uniform_ints
+    return splitmix_uniform_ints(n, lo, hi, seed)
+
+
+def c2(n_train=2048, n_test=512, batch=256, seed=1, w_std=0.2) -> TrainWorkload:
+    """784 -> 512 recurrent LIF (32 input + 56 recurrent partners per hidden
+    neuron, 2-D locality) -> 10 leaky readout, ~50k synapses; prototype images
+    with 20 % pixel noise, Poisson p = pixel/255 * 0.2, T_b ~ U[50, 200].
+    Training starts from the graph's signed N(0, 0.2) weights: the reference's
+    all-positive uniform [0, w_init] start is bistable on this recurrent graph
+    (silent at w_init <= 0.04, above 40 % firing at >= 0.07)."""
+    from .train import READOUT_LEAKY, TrainConfig
+    g = G.sheet_graph([784, 512, 10], 32, recurrent_fan_in=56, dense_readout=True, radius=3.0,
+                      sheet=28.0, w_std=w_std, delay=1, seed=seed)
+    n = n_train + n_test
+    Xu, y = I.prototype_images(n, 784, 10, 0.7, 0.2, seed=seed + 10)
+    X = Xu.astype(np.float32) / np.float32(255.0)
+    lengths = _lengths(n, 50, 200, seed + 20)
+    cfg = TrainConfig(num_steps=200, lif=lif_tau20(), lr=1.0, w_init=0.0, w_max=4.0,
+                      surrogate_scale=0.3, feedback_scale=1.0, readout_mode=READOUT_LEAKY,
+                      readout_psi=1.0, encoder=1, enc_gain=1.0, enc_max_rate=0.2, seed=seed)
+    return TrainWorkload("c2_batched_eprop_784-512r-10", g, cfg, batch, 200, "features", X,
+                         None, y, lengths, n_train, g.weights.copy())
+
+
+def c3(n_train=1024, n_test=256, batch=128, seed=1) -> TrainWorkload:
+    """700-channel event streams (~5k events, 20 Gaussian centres, sd 15,
+    T ~ U[80, 120]) -> 1024 recurrent LIF (10 % input and recurrent density,
+    delays 0..30 mapped to 1..31) -> 20 leaky readout."""
+    from .train import READOUT_LEAKY, TrainConfig
+    g = G.random_recurrent_graph(700, 1024, 20, 0.1, 0.1, 1, 31, 1.0 / math.sqrt(102.0),
+                                 seed=seed)
+    n = n_train + n_test
+    ev, lengths, y = I.event_streams(n, 700, 20, 20, 15.0, 5000, 80, 120, seed=seed + 30)
+    cfg = TrainConfig(num_steps=120, lif=lif_tau20(), lr=1.0, w_init=0.0, w_max=4.0,
+                      surrogate_scale=0.3, feedback_scale=1.0, readout_mode=READOUT_LEAKY,
+                      readout_psi=1.0, encoder=0, seed=seed)
+    return TrainWorkload("c3_delay_rich_eprop_700-1024r-20", g, cfg, batch, 120, "events", None,
+                         ev, y, lengths, n_train, g.weights.copy())
+
+
 def c1(width=500, height=400, num_steps=1000, seed=1, reset_mode=L.RESET_TO_VALUE):
     g = c1_graph(width, height, seed)
     drive = I.background_drive(g.num_neurons, num_steps, 0.01, 1.0, seed + 1)

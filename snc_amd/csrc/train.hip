@@ -1,0 +1,543 @@
+// e-prop training on gfx950: the batched persistent path and the plain
+// per-sample path, sharing one definition of the math.
+//
+// Per sample and step (CPU Trainer order, src/training/trainer.cpp:91-151):
+//   inject -> LIF + surrogate psi -> trace -> eligibility E_s += psi_post tr_pre
+//   -> delivery of fired rows with the trainer's signed weights.
+// Delivery is the ordered bucket fold of the forward path: fired rows set
+// presence bits over each target's incoming column (ordered pre, CSR
+// position), then every target adds its bits in column order, so rasters are
+// the CPU's bit for bit. Eligibility is accumulated per (sample, synapse) in
+// double (the CPU's exact sums) or float.
+//
+// After a sample's T_b steps: delta = softmax(logits) - onehot(label)
+// (trainer.cpp:158-180), learning signal L (outputs: delta, hidden: B delta,
+// trainer.cpp:182-195) and the per-synapse contribution
+// C_{b,s} = (lr / T_b * L_post) * float(E_{b,s}). A separate kernel sums C over
+// the minibatch in sample order and applies w <- clamp(w - sum / B), the
+// minibatch-mean rule of the reference GPU trainer (cuda_trainer.cu:163-172).
+//
+// Batched (SNC_TRAIN_BATCHED): one workgroup per sample runs all of its T_b
+// steps in a single launch -- neuron state, fired bitmap, delivery buckets and
+// (when it fits) the delay ring live in LDS; uneven lengths need no masking.
+// Plain (SNC_TRAIN_PLAIN): the same phases as grid-wide kernels, one sample
+// at a time, six launches per step -- the shape of the reference's trainer.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "fold.hpp"
+#include "forward_plan.hpp"
+#include "rng.hpp"
+#include "train.hpp"
+#include "wave.hpp"
+
+namespace snc_amd {
+
+constexpr int kTrainThreads = 1024;
+
+// ---------------------------------------------------------------------------
+// shared device pieces
+// ---------------------------------------------------------------------------
+
+// Drive of input neuron j at step t of dataset sample `sample`.
+__device__ __forceinline__ float input_drive(const TrainArgs& a, int j, int sample, int t) {
+  const int c = a.channel[j];
+  if (c < 0 || c >= a.dim) return 0.0f;
+  const float x = a.X[(size_t)sample * a.dim + c];
+  if (!(x > a.enc_thr)) return 0.0f;
+  if (a.encoder == SNC_ENC_DIRECT) return x * a.gain;
+  const Philox4 r = philox4x32_10((uint32_t)(c >> 2), (uint32_t)t, (uint32_t)sample,
+                                  (uint32_t)a.epoch, a.seed_lo, a.seed_hi);
+  const float p = fminf(1.0f, x * a.max_rate);
+  return r.v[c & 3] < prob_to_u32((double)p) ? a.gain : 0.0f;
+}
+
+// LIF + surrogate of one neuron (trainer.cpp:96-125 plus the reset-mode and
+// leaky-readout extensions). Returns fired; writes psi.
+__device__ __forceinline__ bool train_lif(const TrainArgs& a, int j, float& v, int& refr,
+                                          float ext, float slot, float& psi) {
+  const float thr = a.lif.threshold;
+  if (a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT) {
+    v = __fadd_rn(__fadd_rn(__fmul_rn(a.lif.decay, v), ext), slot);
+    psi = a.readout_psi;
+    return false;
+  }
+  float vpre;
+  bool fired = false, in_refr = false;
+  if (refr > 0) {
+    --refr;
+    in_refr = true;
+    if (!a.lif.subtract) v = a.lif.reset;
+    vpre = a.lif.subtract ? v : a.lif.reset;
+  } else {
+    v = __fadd_rn(__fadd_rn(__fmul_rn(a.lif.decay, v), ext), slot);
+    vpre = v;
+    if (v >= thr) {
+      v = a.lif.subtract ? __fsub_rn(v, thr) : a.lif.reset;
+      refr = a.lif.refractory;
+      fired = true;
+    }
+  }
+  const float d = __fdiv_rn(fabsf(__fsub_rn(vpre, thr)), thr > 0.0f ? thr : 1.0f);
+  psi = (a.lif.subtract && in_refr) ? 0.0f
+                                    : (d < 1.0f ? __fmul_rn(a.gamma, __fsub_rn(1.0f, d)) : 0.0f);
+  return fired;
+}
+
+__device__ __forceinline__ float trace_step(float decay, float tr, bool fired) {
+  return __fadd_rn(__fmul_rn(decay, tr), __fmul_rn(__fsub_rn(1.0f, decay), fired ? 1.0f : 0.0f));
+}
+
+template <typename ET>
+__device__ __forceinline__ void elig_add(ET* E, size_t k, float psi, float tr);
+template <>
+__device__ __forceinline__ void elig_add<double>(double* E, size_t k, float psi, float tr) {
+  // psi * tr is exact in double, so one rounding: the CPU's double sum.
+  E[k] = fma((double)psi, (double)tr, E[k]);
+}
+template <>
+__device__ __forceinline__ void elig_add<float>(float* E, size_t k, float psi, float tr) {
+  E[k] = fmaf(psi, tr, E[k]);
+}
+
+// End of sample: delta, loss, prediction, learning signal into Ls[n].
+// Runs on the whole block; s_delta holds `classes` floats.
+__device__ void sample_epilogue(const TrainArgs& a, int b, const float* logits_in, float* Ls,
+                                float* s_delta, int label, int n, int tid, int nthreads) {
+  const int K = a.classes;
+  if (tid == 0) {
+    float mx = logits_in[0];
+    for (int k = 1; k < K; ++k) mx = fmaxf(mx, logits_in[k]);
+    double sum = 0.0;
+    for (int k = 0; k < K; ++k) sum += (double)expf(__fsub_rn(logits_in[k], mx));
+    for (int k = 0; k < K; ++k) {
+      const float p = __fdiv_rn(expf(__fsub_rn(logits_in[k], mx)), (float)sum);
+      s_delta[k] = __fsub_rn(p, k == label ? 1.0f : 0.0f);
+    }
+    double ls = 0.0;
+    for (int k = 0; k < K; ++k) ls += exp((double)__fsub_rn(logits_in[k], mx));
+    int pred = 0;
+    for (int k = 1; k < K; ++k)
+      if (logits_in[k] > logits_in[pred]) pred = k;
+    a.loss[b] = -(double)__fsub_rn(logits_in[label], mx) + log(ls);
+    a.correct[b] = pred == label ? 1 : 0;
+    for (int k = 0; k < K; ++k) a.logits[(size_t)b * K + k] = logits_in[k];
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += nthreads) {
+    float l = 0.0f;
+    const int c = a.channel[j];
+    if (a.role[j] == SNC_ROLE_OUTPUT && c >= 0 && c < K) {
+      l = s_delta[c];
+    } else if (a.train_hidden && a.internal_row[j] >= 0) {
+      const float* br = a.Bfb + (size_t)a.internal_row[j] * K;
+      float s = 0.0f;
+      for (int k = 0; k < K; ++k) s = __fadd_rn(s, __fmul_rn(br[k], s_delta[k]));
+      l = s;
+    }
+    Ls[j] = l;
+  }
+}
+
+template <typename ET>
+__device__ __forceinline__ void write_contrib(const TrainArgs& a, int b, const ET* E,
+                                              const float* Ls, float scale, int tid, int nthreads,
+                                              size_t stride_base) {
+  for (int s = tid; s < a.S; s += nthreads) {
+    const float e = (float)E[stride_base + s];
+    a.C[(size_t)b * a.S + s] = __fmul_rn(__fmul_rn(scale, Ls[a.post[s]]), e);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batched persistent kernel: blockIdx.x = position in the minibatch
+// ---------------------------------------------------------------------------
+
+template <typename ET, bool RING_LDS>
+__global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = a.n, L = a.L, tid = threadIdx.x;
+  float* s_v = (float*)smem;
+  float* s_tr = s_v + n;
+  float* s_psi = s_tr + n;
+  float* s_ext = s_psi + n;
+  int* s_refr = (int*)(s_ext + n);
+  int* s_fired = s_refr + n;                      // fired list
+  uint32_t* s_fw = (uint32_t*)(s_fired + n);      // fired bitmap [words]
+  uint32_t* s_bits = s_fw + a.words;              // delivery buckets [bits_words]
+  float* s_logit = (float*)(s_bits + a.bits_words);  // [classes]
+  float* s_delta = s_logit + a.classes;              // [classes]
+  float* s_ring = s_delta + a.classes;               // [L][n] when RING_LDS
+  __shared__ int s_nf;
+  __shared__ unsigned long long s_ev, s_sp;
+
+  const int b = blockIdx.x;
+  const int sample = a.idx[b];
+  const int T = a.lengths[sample];
+  const int label = a.labels[sample];
+  float* ring = RING_LDS ? s_ring : a.ring_g + (size_t)b * L * n;
+  ET* E = (ET*)a.E + (size_t)b * a.S;
+
+  for (int j = tid; j < n; j += kTrainThreads) {
+    s_v[j] = 0.0f;
+    s_tr[j] = 0.0f;
+    s_ext[j] = 0.0f;
+    s_refr[j] = 0;
+  }
+  for (int k = tid; k < a.bits_words; k += kTrainThreads) s_bits[k] = 0u;
+  for (int k = tid; k < a.classes; k += kTrainThreads) s_logit[k] = 0.0f;
+  for (size_t k = tid; k < (size_t)L * n; k += kTrainThreads) ring[k] = 0.0f;
+  for (int s = tid; s < a.S; s += kTrainThreads) E[s] = (ET)0;
+  if (tid == 0) {
+    s_nf = 0;
+    s_ev = 0;
+    s_sp = 0;
+  }
+  __syncthreads();
+
+  const int ev0 = a.ev_base ? a.ev_base[sample] : 0;
+  int tm = 0;
+  for (int t = 0; t < T; ++t) {
+    // inject
+    if (a.ev_base) {
+      for (int k = a.ev_step_ptr[ev0 + t] + tid; k < a.ev_step_ptr[ev0 + t + 1];
+           k += kTrainThreads)
+        s_ext[a.ev_nid[k]] = a.ev_amp[k];
+    } else {
+      for (int q = tid; q < a.n_inputs; q += kTrainThreads) {
+        const int j = a.inputs[q];
+        const float x = input_drive(a, j, sample, t);
+        if (x != 0.0f) s_ext[j] = x;
+      }
+    }
+    __syncthreads();
+    // LIF + psi, fired bitmap and list
+    const float* slot = ring + (size_t)tm * n;
+    for (int j0 = 0; j0 < n; j0 += kTrainThreads) {
+      const int j = j0 + tid;
+      bool f = false;
+      if (j < n) {
+        float v = s_v[j];
+        int r = s_refr[j];
+        float psi;
+        f = train_lif(a, j, v, r, s_ext[j], slot[j], psi);
+        s_v[j] = v;
+        s_refr[j] = r;
+        s_psi[j] = psi;
+        s_ext[j] = 0.0f;
+        ((float*)slot)[j] = 0.0f;
+        if (f) {
+          if (a.neur_fire) atomicAdd(a.neur_fire + j, 1ull);
+          if (!a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT && a.channel[j] >= 0)
+            atomicAdd(&s_logit[a.channel[j]], 1.0f);
+        }
+        if (a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT && a.channel[j] >= 0)
+          s_logit[a.channel[j]] = __fadd_rn(s_logit[a.channel[j]], v);  // one readout per channel
+      }
+      const uint64_t m = ballot(f);
+      if (lane_id() == 0 && j - lane_id() < n) {
+        const int w0 = (j - lane_id()) >> 5;
+        s_fw[w0] = (uint32_t)m;
+        if (w0 + 1 < a.words) s_fw[w0 + 1] = (uint32_t)(m >> 32);
+      }
+      if (m) {
+        int pos = 0;
+        if (lane_id() == 0) pos = atomicAdd(&s_nf, __popcll(m));
+        pos = __shfl(pos, 0, kWave);
+        if (f) s_fired[pos + rank_below(m)] = j;
+      }
+    }
+    __syncthreads();
+    if (a.raster)
+      for (int w = tid; w < a.words; w += kTrainThreads)
+        a.raster[((size_t)b * a.T_max + t) * a.words + w] = s_fw[w];
+    // trace
+    for (int j = tid; j < n; j += kTrainThreads)
+      s_tr[j] = trace_step(a.lif.decay, s_tr[j], (s_fw[j >> 5] >> (j & 31)) & 1u);
+    __syncthreads();
+    // eligibility over all synapses; skipped where the pre trace is zero,
+    // as the CPU skips those rows (trainer.cpp:135-136).
+    for (int s = tid; s < a.S; s += kTrainThreads) {
+      const float tp = s_tr[a.src[s]];
+      if (tp != 0.0f) elig_add<ET>(E, s, s_psi[a.post[s]], tp);
+    }
+    // delivery: fired rows set bucket bits
+    const int nf = s_nf;
+    const int wave = tid >> 6, lane = lane_id();
+    unsigned long long ev = 0;
+    for (int k = wave; k < nf; k += kTrainThreads / kWave) {
+      const int i = s_fired[k];
+      const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
+      for (int e = rb + lane; e < re; e += kWave) {
+        const int p = a.cpos[e];
+        atomicOr(&s_bits[p >> 5], 1u << (p & 31));
+      }
+      ev += (unsigned long long)(re - rb);
+    }
+    if (lane == 0 && ev) atomicAdd(&s_ev, ev);
+    __syncthreads();
+    // ordered fold into the ring
+    for (int j = tid; j < n; j += kTrainThreads) {
+      const int w0 = a.ccol[j] >> 5, w1 = a.ccol[j + 1] >> 5;
+      int cur = -1;
+      float acc = 0.0f;
+      for (int w = w0; w < w1; ++w) {
+        uint32_t bw = s_bits[w];
+        if (!bw) continue;
+        s_bits[w] = 0u;
+        while (bw) {
+          const int p = (w << 5) | __builtin_ctz(bw);
+          bw &= bw - 1u;
+          int s = tm + (int)a.cdel[p];
+          if (s >= L) s -= L;
+          if (s != cur) {
+            if (cur >= 0) ring[(size_t)cur * n + j] = acc;
+            cur = s;
+            acc = ring[(size_t)s * n + j];
+          }
+          acc = __fadd_rn(acc, a.wc[p]);
+        }
+      }
+      if (cur >= 0) ring[(size_t)cur * n + j] = acc;
+    }
+    if (tid == 0) {
+      s_sp += (unsigned long long)nf;
+      s_nf = 0;
+    }
+    tm = tm + 1 == L ? 0 : tm + 1;
+    __syncthreads();
+  }
+
+  if (a.readout_leaky && tid < a.classes)
+    s_logit[tid] = __fdiv_rn(s_logit[tid], (float)max(1, T));
+  __syncthreads();
+  sample_epilogue(a, b, s_logit, s_psi, s_delta, label, n, tid, kTrainThreads);
+  __syncthreads();
+  if (a.C) {
+    const float scale = __fdiv_rn(a.lr, (float)max(1, T));
+    write_contrib<ET>(a, b, E, s_psi, scale, tid, kTrainThreads, 0);
+  }
+  if (tid == 0) {
+    a.blk_spikes[b] = s_sp;
+    a.blk_events[b] = s_ev;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plain path: grid-wide kernels, one sample at a time
+// ---------------------------------------------------------------------------
+
+__global__ void k_p_inject(TrainArgs a, PlainState ps, int sample, int t) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.ev_base) {
+    const int e0 = a.ev_base[sample];
+    const int k = a.ev_step_ptr[e0 + t] + q;
+    if (k < a.ev_step_ptr[e0 + t + 1]) ps.ext[a.ev_nid[k]] = a.ev_amp[k];
+  } else if (q < a.n_inputs) {
+    const int j = a.inputs[q];
+    const float x = input_drive(a, j, sample, t);
+    if (x != 0.0f) ps.ext[j] = x;
+  }
+}
+
+__global__ void k_p_lif(TrainArgs a, PlainState ps, int t, int tm) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  bool f = false;
+  if (j < a.n) {
+    float v = ps.v[j];
+    int r = ps.refr[j];
+    float psi;
+    float* slot = ps.ring + (size_t)tm * a.n + j;
+    f = train_lif(a, j, v, r, ps.ext[j], *slot, psi);
+    ps.v[j] = v;
+    ps.refr[j] = r;
+    ps.psi[j] = psi;
+    ps.ext[j] = 0.0f;
+    *slot = 0.0f;
+    if (f) {
+      if (a.neur_fire) atomicAdd(a.neur_fire + j, 1ull);
+      if (!a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT && a.channel[j] >= 0)
+        atomicAdd(&ps.logit[a.channel[j]], 1.0f);
+    }
+    if (a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT && a.channel[j] >= 0)
+      ps.logit[a.channel[j]] = __fadd_rn(ps.logit[a.channel[j]], v);
+  }
+  const uint64_t m = ballot(f);
+  if (lane_id() == 0 && j - lane_id() < a.n) {
+    const int w0 = (j - lane_id()) >> 5;
+    ps.fw[w0] = (uint32_t)m;
+    if (w0 + 1 < a.words) ps.fw[w0 + 1] = (uint32_t)(m >> 32);
+  }
+  if (m) {
+    int pos = 0;
+    if (lane_id() == 0) pos = atomicAdd(ps.nf, __popcll(m));
+    pos = __shfl(pos, 0, kWave);
+    if (f) ps.fired[pos + rank_below(m)] = j;
+  }
+}
+
+__global__ void k_p_trace(TrainArgs a, PlainState ps, int b, int t) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < a.n) ps.tr[j] = trace_step(a.lif.decay, ps.tr[j], (ps.fw[j >> 5] >> (j & 31)) & 1u);
+  if (a.raster && j < a.words) a.raster[((size_t)b * a.T_max + t) * a.words + j] = ps.fw[j];
+}
+
+template <typename ET>
+__global__ void k_p_elig(TrainArgs a, PlainState ps) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.S) return;
+  const float tp = ps.tr[a.src[s]];
+  if (tp != 0.0f) elig_add<ET>((ET*)ps.E, s, ps.psi[a.post[s]], tp);
+}
+
+__global__ void k_p_deliver(TrainArgs a, PlainState ps) {
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = gtid >> 6, lane = lane_id(), nw = (gridDim.x * blockDim.x) >> 6;
+  const int nf = *ps.nf;
+  unsigned long long ev = 0;
+  for (int k = wave; k < nf; k += nw) {
+    const int i = ps.fired[k];
+    const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
+    for (int e = rb + lane; e < re; e += kWave) {
+      const int p = a.cpos[e];
+      atomicOr(ps.bits + (p >> 5), 1u << (p & 31));
+    }
+    ev += (unsigned long long)(re - rb);
+  }
+  if (lane == 0 && ev) atomicAdd(ps.events, ev);
+}
+
+__global__ void k_p_fold(TrainArgs a, PlainState ps, int tm) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j == 0) {
+    *ps.spikes += (unsigned long long)*ps.nf;
+    *ps.nf = 0;  // the fold runs after delivery consumed the list
+  }
+  if (j >= a.n) return;
+  fold_light(j, a.n, a.L, tm, a.ccol, a.wc, a.cdel, ps.bits, ps.ring);
+}
+
+__global__ __launch_bounds__(kTrainThreads) void k_p_finish(TrainArgs a, PlainState ps, int b,
+                                                            int T) {
+  __shared__ float s_delta[64];
+  __shared__ float s_logit[64];
+  const int tid = threadIdx.x;
+  if (tid < a.classes)
+    s_logit[tid] = a.readout_leaky ? __fdiv_rn(ps.logit[tid], (float)max(1, T)) : ps.logit[tid];
+  __syncthreads();
+  sample_epilogue(a, b, s_logit, ps.L, s_delta, a.labels[a.idx[b]], a.n, tid, kTrainThreads);
+}
+
+template <typename ET>
+__global__ void k_p_contrib(TrainArgs a, PlainState ps, int b, int T) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.S) return;
+  const float scale = __fdiv_rn(a.lr, (float)max(1, T));
+  const float e = (float)((const ET*)ps.E)[s];
+  a.C[(size_t)b * a.S + s] = __fmul_rn(__fmul_rn(scale, ps.L[a.post[s]]), e);
+}
+
+// ---------------------------------------------------------------------------
+// update
+// ---------------------------------------------------------------------------
+
+// w_s <- clamp(w_s - (sum_b C_{b,s}) / B, +-w_max), b ascending.
+__global__ void k_apply(TrainArgs a, float* w, int B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.S) return;
+  float acc = 0.0f;
+  for (int b = 0; b < B; ++b) acc = __fadd_rn(acc, a.C[(size_t)b * a.S + s]);
+  const float nw = __fsub_rn(w[s], __fdiv_rn(acc, (float)B));
+  w[s] = fminf(fmaxf(nw, -a.w_max), a.w_max);
+}
+
+// Trainer weights in padded column order for the ordered fold.
+__global__ void k_gather_wc(const float* __restrict__ w, const int32_t* __restrict__ csc_e,
+                            float* __restrict__ wc, int64_t P) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int e = csc_e[p];
+  wc[p] = e >= 0 ? w[e] : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static inline int blocks_for(int64_t n, int t) { return (int)std::max<int64_t>(1, (n + t - 1) / t); }
+
+size_t train_lds_bytes(const TrainArgs& a, bool ring_lds) {
+  size_t b = sizeof(float) * 4 * (size_t)a.n + sizeof(int) * 2 * (size_t)a.n +
+             sizeof(uint32_t) * ((size_t)a.words + a.bits_words) + sizeof(float) * 2 * a.classes;
+  if (ring_lds) b += sizeof(float) * (size_t)a.L * a.n;
+  return (b + 15) / 16 * 16;
+}
+
+int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, hipStream_t st) {
+  const size_t lds = train_lds_bytes(a, ring_lds);
+  const void* fn = f64 ? (ring_lds ? (const void*)k_train_batched<double, true>
+                                   : (const void*)k_train_batched<double, false>)
+                       : (ring_lds ? (const void*)k_train_batched<float, true>
+                                   : (const void*)k_train_batched<float, false>);
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    return fail(SNC_E_UNSUPPORTED, "LDS request of the batched trainer rejected");
+  TrainArgs aa = a;
+  void* args[] = {(void*)&aa};
+  if (hipLaunchKernel(fn, dim3(B), dim3(kTrainThreads), args, lds, st) != hipSuccess)
+    return fail(SNC_E_HIP, "batched trainer launch failed");
+  return SNC_OK;
+}
+
+int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int b, int sample, int T,
+                              bool f64, hipStream_t st) {
+  const int n = a.n, L = a.L;
+  const int gn = blocks_for(n, 256), gs = blocks_for(a.S, 256);
+  const int gi = blocks_for(std::max<int64_t>(a.n_inputs, a.max_inj_per_step), 256);
+  int tm = 0;
+  for (int t = 0; t < T; ++t) {
+    hipLaunchKernelGGL(k_p_inject, dim3(gi), dim3(256), 0, st, a, ps, sample, t);
+    hipLaunchKernelGGL(k_p_lif, dim3(gn), dim3(256), 0, st, a, ps, t, tm);
+    hipLaunchKernelGGL(k_p_trace, dim3(std::max(gn, blocks_for(a.words, 256))), dim3(256), 0, st,
+                       a, ps, b, t);
+    if (f64)
+      hipLaunchKernelGGL(k_p_elig<double>, dim3(gs), dim3(256), 0, st, a, ps);
+    else
+      hipLaunchKernelGGL(k_p_elig<float>, dim3(gs), dim3(256), 0, st, a, ps);
+    hipLaunchKernelGGL(k_p_deliver, dim3(std::min(gn, 1024)), dim3(256), 0, st, a, ps);
+    hipLaunchKernelGGL(k_p_fold, dim3(gn), dim3(256), 0, st, a, ps, tm);
+    tm = tm + 1 == L ? 0 : tm + 1;
+  }
+  hipLaunchKernelGGL(k_p_finish, dim3(1), dim3(kTrainThreads), 0, st, a, ps, b, T);
+  if (a.C) {
+    if (f64)
+      hipLaunchKernelGGL(k_p_contrib<double>, dim3(gs), dim3(256), 0, st, a, ps, b, T);
+    else
+      hipLaunchKernelGGL(k_p_contrib<float>, dim3(gs), dim3(256), 0, st, a, ps, b, T);
+  }
+  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "plain trainer launch failed");
+}
+
+int train_launch_apply(const TrainArgs& a, float* w, int B, const int32_t* csc_e, float* wc,
+                       int64_t P, hipStream_t st) {
+  hipLaunchKernelGGL(k_apply, dim3(blocks_for(a.S, 256)), dim3(256), 0, st, a, w, B);
+  hipLaunchKernelGGL(k_gather_wc, dim3(blocks_for(P, 256)), dim3(256), 0, st, w, csc_e, wc, P);
+  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "update launch failed");
+}
+
+int train_launch_gather(const float* w, const int32_t* csc_e, float* wc, int64_t P,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_wc, dim3(blocks_for(P, 256)), dim3(256), 0, st, w, csc_e, wc, P);
+  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "gather launch failed");
+}
+
+}  // namespace snc_amd

@@ -1,0 +1,78 @@
+// e-prop training: kernel argument blocks and launchers (train.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "forward_plan.hpp"
+#include "snc_hip.h"
+
+namespace snc_amd {
+
+struct TrainArgs {
+  int n, S, L, classes, words, bits_words, n_inputs, dim;
+  int readout_leaky;
+  float readout_psi;
+  LifK lif;
+  float gamma, lr, w_max;
+  int train_hidden;
+  int encoder;
+  float gain, max_rate, enc_thr;
+  uint32_t seed_lo, seed_hi;
+  int epoch, T_max;
+  int64_t max_inj_per_step;
+  // graph
+  const int32_t* row_ptr;
+  const int32_t* post;
+  const int32_t* src;  // pre of each synapse (CSR order)
+  const int32_t* cpos;
+  const int32_t* ccol;
+  const uint8_t* cdel;
+  const float* wc;     // trainer weights in padded column order
+  const uint8_t* role;
+  const int32_t* channel;
+  const int32_t* internal_row;
+  const float* Bfb;    // [n_internal][classes]
+  const int32_t* inputs;  // INPUT neuron ids
+  // dataset
+  const float* X;          // [n_data][dim] or null
+  const int32_t* labels;   // [n_data]
+  const int32_t* lengths;  // [n_data]
+  const int32_t* ev_base;      // [n_data] offset into ev_step_ptr, or null (features)
+  const int32_t* ev_step_ptr;  // per (sample, step) injection ranges
+  const int32_t* ev_nid;
+  const float* ev_amp;
+  // batch
+  const int32_t* idx;  // [B] dataset indices
+  void* E;             // [B][S] float or double
+  float* C;            // [B][S] contributions, or null (no update)
+  float* ring_g;       // [B][L][n] when the ring is not in LDS
+  float* logits;       // [B][classes]
+  double* loss;        // [B]
+  int32_t* correct;    // [B]
+  unsigned long long* blk_spikes;  // [B]
+  unsigned long long* blk_events;  // [B]
+  unsigned long long* neur_fire;   // [n] or null
+  uint32_t* raster;                // [B][T_max][words] or null
+};
+
+// Device state of the plain path (one sample at a time).
+struct PlainState {
+  float *v, *tr, *psi, *ext, *ring, *logit, *L;
+  int32_t *refr, *fired, *nf;
+  uint32_t *fw, *bits;
+  void* E;  // [S]
+  unsigned long long *spikes, *events;
+};
+
+size_t train_lds_bytes(const TrainArgs& a, bool ring_lds);
+int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, hipStream_t st);
+int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int b, int sample, int T,
+                              bool f64, hipStream_t st);
+int train_launch_apply(const TrainArgs& a, float* w, int B, const int32_t* csc_e, float* wc,
+                       int64_t P, hipStream_t st);
+int train_launch_gather(const float* w, const int32_t* csc_e, float* wc, int64_t P,
+                        hipStream_t st);
+
+}  // namespace snc_amd

@@ -1,0 +1,156 @@
+"""GPU e-prop parity through the C ABI (snc_hip_train_*).
+
+Against the CPU oracle's minibatch step from the same weights and inputs:
+rasters and logits bit-exact, per-neuron fires identical, and FP32 weight
+deltas within 1e-5 relative (absolute floor 6e-8, ~2 ulp of the weights:
+softmax uses the device expf). The plain per-sample path and the batched
+persistent path compute the same sums in the same order, so they agree bit
+for bit with each other.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from snc_amd import graph as G
+from snc_amd import inputs as I
+from snc_amd.runtime import LIFParams
+from snc_amd.train import (BATCHED, ELIG_F32, ELIG_F64, PLAIN, READOUT_LEAKY, READOUT_SPIKING,
+                           HipTrainSession, TrainConfig)
+
+pytestmark = pytest.mark.gpu
+
+W_RTOL, W_ATOL = 1e-5, 6e-8
+
+
+def _cfgs(T, readout=READOUT_SPIKING, reset_mode=0, encoder=0, lr=0.2, w_init=0.4, seed=3,
+          decay=0.9, max_rate=0.5):
+    lif = LIFParams(1.0, decay, 0.0, 1, reset_mode)
+    gpu = TrainConfig(num_steps=T, lif=lif, lr=lr, w_init=w_init, w_max=4.0, surrogate_scale=0.3,
+                      feedback_scale=1.0, readout_mode=readout, readout_psi=1.0, encoder=encoder,
+                      enc_max_rate=max_rate, seed=seed)
+    orc = O.train_config(num_steps=T, lif=lif, lr=lr, w_init=w_init, w_max=4.0, gamma=0.3,
+                         feedback=1.0, readout_mode=readout, readout_psi=1.0, encoder=encoder,
+                         max_rate=max_rate, seed=seed)
+    return gpu, orc
+
+
+def _events_for(X, lengths, enc=0, max_rate=0.5, seed=3, epoch=0, ids=None):
+    evs = []
+    for b in range(len(lengths)):
+        sid = b if ids is None else int(ids[b])
+        if enc == 0:
+            e = O.encode(0, X[b], int(lengths[b]))
+        else:
+            e = O.encode_philox(X[b], int(lengths[b]), max_rate=max_rate, seed=seed, sample=sid,
+                                epoch=epoch)
+        e["sample_id"] = b
+        evs.append(e)
+    return np.concatenate(evs)
+
+
+def _compare(sess, orc_tr, w0, idx, ev_orc, lengths, labels, mode, want_exact_w=False):
+    st, logits, raster = sess.step(idx, epoch=0, mode=mode, apply=True, want_logits=True,
+                                   want_raster=True)
+    ref = orc_tr.minibatch(ev_orc, lengths, labels, apply=True, raster=True)
+    assert np.array_equal(logits, ref["logits"])
+    Tm = int(lengths.max())
+    assert np.array_equal(raster[:, :Tm], ref["raster"])
+    assert st.spikes == ref["spikes"] and st.synaptic_events == ref["events"]
+    wg, wo = sess.get_weights(), orc_tr.weights
+    if want_exact_w:
+        assert np.array_equal(wg, wo)
+    np.testing.assert_allclose(wg - w0, wo - w0, rtol=W_RTOL, atol=W_ATOL)
+    return wg
+
+
+@pytest.mark.parametrize("mode", [BATCHED, PLAIN])
+@pytest.mark.parametrize("elig", [ELIG_F64, ELIG_F32])
+def test_minibatch_step_matches_oracle(mode, elig):
+    g = G.static_snc([64, 48, 8], 1500, 0.5, 1, 0.2, 4)
+    B = 12
+    X, y = O.ref_make_synthetic(B, 64, 8, 0.35, 3)
+    lengths = np.array([20, 17, 25, 20, 9, 31, 20, 22, 14, 20, 28, 19], np.int32)
+    gcfg, ocfg = _cfgs(32)
+    sess = HipTrainSession(g, gcfg, batch=B, max_steps=32, elig=elig)
+    orc = O.Trainer(g, ocfg)
+    w0 = orc.weights
+    assert np.array_equal(sess.get_weights(), w0)  # same mt19937_64 draws
+    ev = _events_for(X, lengths)
+    sess.upload_events(ev, B, y, lengths)
+    _compare(sess, orc, w0, np.arange(B), ev, lengths, y, mode)
+    syn, neur = sess.collect_stats()
+    osyn, oneur = orc.stats()
+    assert np.array_equal(neur, oneur) and np.array_equal(syn, osyn)
+
+
+def test_plain_and_batched_are_bit_identical():
+    g = G.sheet_graph([100, 64, 10], 16, recurrent_fan_in=8, radius=3.0, sheet=10.0, w_std=0.3,
+                      max_delay=5, seed=2)
+    B = 16
+    X, y = I.prototype_images(B, 100, 10, 0.6, 0.2, seed=4)
+    Xf = X.astype(np.float32) / np.float32(255.0)
+    lengths = np.random.default_rng(1).integers(10, 40, B).astype(np.int32)
+    gcfg, _ = _cfgs(40, readout=READOUT_LEAKY, reset_mode=1, encoder=1, max_rate=0.4)
+    out = []
+    for mode in (PLAIN, BATCHED):
+        s = HipTrainSession(g, gcfg, batch=B, max_steps=40)
+        s.upload_features(Xf, y, lengths)
+        st, lg, ras = s.step(np.arange(B), epoch=2, mode=mode, want_logits=True, want_raster=True)
+        out.append((s.get_weights(), lg, ras, st.spikes, st.synaptic_events))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("readout,reset_mode", [(READOUT_LEAKY, 1), (READOUT_SPIKING, 1),
+                                                 (READOUT_LEAKY, 0)])
+def test_device_poisson_and_extensions_match_oracle(readout, reset_mode):
+    """Philox encoding on the device, leaky readout, subtract reset, uneven T."""
+    g = G.sheet_graph([196, 96, 10], 24, recurrent_fan_in=12, radius=2.5, sheet=14.0,
+                      w_std=0.3, delay=1, seed=7)
+    B = 10
+    Xu, y = I.prototype_images(B, 196, 10, 0.6, 0.2, seed=9)
+    X = Xu.astype(np.float32) / np.float32(255.0)
+    lengths = np.array([30, 45, 12, 50, 33, 40, 27, 50, 18, 44], np.int32)
+    ids = np.arange(100, 100 + B)  # dataset indices feed the Philox counters
+    gcfg, ocfg = _cfgs(50, readout=readout, reset_mode=reset_mode, encoder=1, lr=0.5,
+                       w_init=0.3, max_rate=0.3, decay=float(np.float32(np.exp(-1 / 20))))
+    sess = HipTrainSession(g, gcfg, batch=B, max_steps=50)
+    Xd = np.zeros((200, 196), np.float32)
+    Xd[ids] = X
+    yd = np.zeros(200, np.int32)
+    yd[ids] = y
+    Ld = np.full(200, 10, np.int32)
+    Ld[ids] = lengths
+    sess.upload_features(Xd, yd, Ld)
+    orc = O.Trainer(g, ocfg)
+    w0 = orc.weights
+    ev = _events_for(X, lengths, enc=1, max_rate=0.3, seed=3, epoch=0, ids=ids)
+    _compare(sess, orc, w0, ids, ev, lengths, y, BATCHED)
+
+
+def test_delay_rich_recurrent_ring_outside_lds():
+    """Recurrent 10 % graph with delays 1..31: the ring lives in global memory."""
+    g = G.random_recurrent_graph(70, 400, 20, 0.1, 0.1, 1, 31, 0.25, seed=5)
+    B = 6
+    ev, lengths, y = I.event_streams(B, channels=70, classes=20, n_centres=20, sd=3.0,
+                                     events_per_sample=300, t_min=40, t_max=60, seed=8)
+    gcfg, ocfg = _cfgs(60, reset_mode=1, lr=0.3, w_init=0.25)
+    sess = HipTrainSession(g, gcfg, batch=B, max_steps=60)
+    sess.upload_events(ev, B, y, lengths)
+    orc = O.Trainer(g, ocfg)
+    w0 = orc.weights
+    _compare(sess, orc, w0, np.arange(B), ev, lengths, y, BATCHED)
+
+
+def test_epoch_and_evaluate_run_and_learn_something():
+    g = G.static_snc([64, 48, 8], 1500, 0.5, 1, 0.2, 4)
+    X, y = O.ref_make_synthetic(256, 64, 8, 0.2, 3)
+    gcfg, _ = _cfgs(20, encoder=0, lr=1.0, w_init=0.3)
+    sess = HipTrainSession(g, gcfg, batch=32, max_steps=20)
+    sess.upload_features(X, y)
+    acc0 = sess.evaluate()
+    for ep in range(1, 6):
+        st = sess.train_epoch(ep)
+        assert st.samples == 256
+    acc1 = sess.evaluate()
+    assert acc1 >= acc0
