@@ -260,6 +260,37 @@ int snc_hip_train_reset_stats(snc_hip_train* s);
 int snc_hip_train_collect_stats(const snc_hip_train* s, int64_t* syn_deliveries,
                                 int64_t* neuron_fires);
 
+/* ---- learned grow proposal (no reference counterpart) ---------------------
+ * A 16 -> 64 -> 64 -> 1 MLP with BF16 weights and FP32 accumulation scores
+ * candidate synapses for the grow step of Connectome::structural_update
+ * (src/structure/connectome.cpp:164-211); the heuristic stays as fallback.
+ * Parameters: FP32 master vector of SNC_MLP_PARAMS floats laid out as
+ * W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3. */
+#define SNC_MLP_IN 16
+#define SNC_MLP_PARAMS 5313
+
+typedef struct snc_hip_mlp snc_hip_mlp;
+
+/* He-uniform weights, zero biases, from SplitMix64(seed). */
+int snc_hip_mlp_create(uint64_t seed, snc_hip_mlp** out);
+void snc_hip_mlp_destroy(snc_hip_mlp* m);
+int snc_hip_mlp_get_params(const snc_hip_mlp* m, float* params);
+int snc_hip_mlp_set_params(snc_hip_mlp* m, const float* params);
+
+/* Logits of M candidates (features [M][16], host arrays). *kernel_ms (may be
+ * NULL) is the device time of the scoring kernel alone, inputs resident. */
+int snc_hip_mlp_score(snc_hip_mlp* m, const float* features, int64_t M, float* scores,
+                      double* kernel_ms);
+
+/* Mean binary cross-entropy and its gradient w.r.t. the FP32 master
+ * parameters over M labelled examples (labels 0/1). */
+int snc_hip_mlp_grad(snc_hip_mlp* m, const float* features, const float* labels, int64_t M,
+                     float* grad, double* loss);
+
+/* Minibatch Adam on the GPU: `epochs` shuffled passes (SplitMix64(seed)). */
+int snc_hip_mlp_train(snc_hip_mlp* m, const float* features, const float* labels, int64_t M,
+                      int32_t epochs, int32_t batch, float lr, uint64_t seed, double* final_loss);
+
 #ifdef __cplusplus
 }
 #endif

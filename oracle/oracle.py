@@ -475,3 +475,41 @@ class RefTrainer:
                                                 classes, epoch, C.byref(loss), C.byref(sp),
                                                 C.byref(ev))
         return dict(train_acc=acc, loss=loss.value, spikes=sp.value, events=ev.value)
+
+
+# ---------------------------------------------------------------------------
+# grow-proposal MLP (oracle_mlp.cpp)
+# ---------------------------------------------------------------------------
+
+MLP_PARAMS = 5313
+
+
+def _mlp_bind():
+    lib = oracle_lib()
+    if not getattr(lib, "_mlp_bound", False):
+        lib.oracle_mlp_forward.restype = None
+        lib.oracle_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                           C.c_void_p]
+        lib.oracle_mlp_grad.restype = C.c_double
+        lib.oracle_mlp_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                        C.c_void_p]
+        lib._mlp_bound = True
+    return lib
+
+
+def mlp_forward(params, X, quantize=True):
+    p = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.zeros(X.shape[0], np.float64)
+    _mlp_bind().oracle_mlp_forward(_p(p), _p(X), X.shape[0], 1 if quantize else 0, _p(out))
+    return out
+
+
+def mlp_grad(params, X, y, quantize=True):
+    p = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = _mlp_bind().oracle_mlp_grad(_p(p), _p(X), _p(y), X.shape[0], 1 if quantize else 0,
+                                       _p(g))
+    return loss, g

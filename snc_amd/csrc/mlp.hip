@@ -1,0 +1,270 @@
+// Learned grow proposal: a 16 -> 64 -> 64 -> 1 MLP scoring candidate
+// synapses for the structural grow step (the hook replaces the candidate
+// choice of Connectome::structural_update, src/structure/connectome.cpp:164-211).
+//
+// Model (BF16 weights, FP32 accumulation):
+//   x  = bf16(features)                         16
+//   h1 = relu(W1 x + b1),  h1b = bf16(h1)        64
+//   h2 = relu(W2 h1b + b2)                       64
+//   s  = W3 h2 + b3                              logit of P(synapse survives)
+// Parameters are kept as an FP32 master copy (trained with Adam on BCE loss)
+// and quantised to BF16 (round to nearest even) for both passes.
+//
+// Inference runs on the matrix cores: every wave scores tiles of 16
+// candidates with v_mfma_f32_16x16x16 bf16 in the transposed form
+// H^T = W X^T -- the weight block is operand A, the activation tile operand B --
+// so the accumulator layout of one layer (lane l: rows 4*(l/16)+i, column
+// l%16) is exactly the B-operand layout of the next layer. Training runs one
+// example per lane on the VALU and writes per-example gradients param-major,
+// reduced per parameter in example order (deterministic).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "mlp.hpp"
+#include "wave.hpp"
+
+namespace snc_amd {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+__device__ __forceinline__ float q(float f) { return bf16_to_f32(f32_to_bf16(f)); }
+
+// ---------------------------------------------------------------------------
+// inference (MFMA)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_mlp_score(const MlpDev p, const float* __restrict__ X,
+                                                   int64_t M, float* __restrict__ out) {
+  const int lane = lane_id();
+  const int r = lane & 15;          // A row / B,D column within a 16-block
+  const int kq = (lane >> 4) * 4;   // k quad of the lane (A, B) / D row quad
+  // Weight blocks as A operands, resident in registers for the whole launch.
+  bf16x4 a1[4], a2[4][4];
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a1[ob][i] = (short)p.w1b[(ob * 16 + r) * kMlpIn + kq + i];
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob)
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        a2[ob][kb][i] = (short)p.w2b[(ob * 16 + r) * kMlpH + kb * 16 + kq + i];
+  float b1v[4][4], b2v[4][4], w3v[4][4];
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      b1v[ob][i] = p.b1[ob * 16 + kq + i];
+      b2v[ob][i] = p.b2[ob * 16 + kq + i];
+      w3v[ob][i] = bf16_to_f32(p.w3b[ob * 16 + kq + i]);
+    }
+  const float b3 = p.b3[0];
+
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t tiles = (M + 15) / 16;
+  for (int64_t tile = wave; tile < tiles; tile += nwaves) {
+    const int64_t cand = tile * 16 + r;
+    const bool valid = cand < M;
+    // B operand of layer 1: X^T, lane holds features kq..kq+3 of candidate r.
+    bf16x4 bx;
+    if (valid) {
+      const float4 x4 = *reinterpret_cast<const float4*>(X + cand * kMlpIn + kq);
+      bx[0] = (short)f32_to_bf16(x4.x);
+      bx[1] = (short)f32_to_bf16(x4.y);
+      bx[2] = (short)f32_to_bf16(x4.z);
+      bx[3] = (short)f32_to_bf16(x4.w);
+    } else {
+      bx[0] = bx[1] = bx[2] = bx[3] = 0;
+    }
+    bf16x4 h1[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob) {
+      f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ob], bx, c, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h1[ob][i] = (short)f32_to_bf16(fmaxf(c[i] + b1v[ob][i], 0.0f));
+    }
+    float part = 0.0f;
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob) {
+      f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[ob][kb], h1[kb], c, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) part += w3v[ob][i] * fmaxf(c[i] + b2v[ob][i], 0.0f);
+    }
+    // Lanes r, r+16, r+32, r+48 hold the four row quads of candidate r.
+    part += __shfl_xor(part, 16, kWave);
+    part += __shfl_xor(part, 32, kWave);
+    if (lane < 16 && valid) out[cand] = part + b3;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// training (VALU, one example per lane)
+// ---------------------------------------------------------------------------
+
+// Forward + backward of example e; writes its gradient to G[param][e] and
+// its loss to loss[e]. BF16 rounding is treated as identity in the backward
+// pass (straight-through).
+__global__ __launch_bounds__(256) void k_mlp_example_grad(const MlpDev p,
+                                                          const float* __restrict__ X,
+                                                          const float* __restrict__ y, int64_t M,
+                                                          float* __restrict__ G,
+                                                          float* __restrict__ loss) {
+  __shared__ float s_w1[kMlpH * kMlpIn], s_w2[kMlpH * kMlpH], s_w3[kMlpH], s_b1[kMlpH],
+      s_b2[kMlpH];
+  for (int k = threadIdx.x; k < kMlpH * kMlpIn; k += blockDim.x) s_w1[k] = bf16_to_f32(p.w1b[k]);
+  for (int k = threadIdx.x; k < kMlpH * kMlpH; k += blockDim.x) s_w2[k] = bf16_to_f32(p.w2b[k]);
+  for (int k = threadIdx.x; k < kMlpH; k += blockDim.x) {
+    s_w3[k] = bf16_to_f32(p.w3b[k]);
+    s_b1[k] = p.b1[k];
+    s_b2[k] = p.b2[k];
+  }
+  __syncthreads();
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M) return;
+  float x[kMlpIn];
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) x[k] = q(X[e * kMlpIn + k]);
+  float h1[kMlpH];
+#pragma unroll 4
+  for (int j = 0; j < kMlpH; ++j) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) s += s_w1[j * kMlpIn + k] * x[k];
+    h1[j] = q(fmaxf(s + s_b1[j], 0.0f));
+  }
+  float h2[kMlpH];
+  float out = p.b3[0];
+  for (int j = 0; j < kMlpH; ++j) {
+    float s = 0.0f;
+#pragma unroll 16
+    for (int k = 0; k < kMlpH; ++k) s += s_w2[j * kMlpH + k] * h1[k];
+    h2[j] = fmaxf(s + s_b2[j], 0.0f);
+    out += s_w3[j] * h2[j];
+  }
+  // mean binary cross-entropy with logits
+  const float t = y[e];
+  const float sg = 1.0f / (1.0f + expf(-out));
+  loss[e] = fmaxf(out, 0.0f) - out * t + log1pf(expf(-fabsf(out)));
+  const float dout = (sg - t) / (float)M;
+  G[(int64_t)kMlpOffB3 * M + e] = dout;
+  float dh1[kMlpH];
+#pragma unroll
+  for (int k = 0; k < kMlpH; ++k) dh1[k] = 0.0f;
+  for (int j = 0; j < kMlpH; ++j) {
+    G[(int64_t)(kMlpOffW3 + j) * M + e] = dout * h2[j];
+    const float d2 = h2[j] > 0.0f ? dout * s_w3[j] : 0.0f;
+    G[(int64_t)(kMlpOffB2 + j) * M + e] = d2;
+#pragma unroll 16
+    for (int k = 0; k < kMlpH; ++k) {
+      G[(int64_t)(kMlpOffW2 + j * kMlpH + k) * M + e] = d2 * h1[k];
+      dh1[k] += d2 * s_w2[j * kMlpH + k];
+    }
+  }
+  for (int k = 0; k < kMlpH; ++k) {
+    const float d1 = h1[k] > 0.0f ? dh1[k] : 0.0f;
+    G[(int64_t)(kMlpOffB1 + k) * M + e] = d1;
+#pragma unroll
+    for (int f = 0; f < kMlpIn; ++f) G[(int64_t)(kMlpOffW1 + k * kMlpIn + f) * M + e] = d1 * x[f];
+  }
+}
+
+// grad[p] = sum_e G[p][e] (examples in order); loss_out = sum_e loss[e] / M.
+__global__ void k_mlp_reduce(const float* __restrict__ G, const float* __restrict__ loss,
+                             int64_t M, float* __restrict__ grad, float* __restrict__ loss_out) {
+  const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pidx < kMlpParams) {
+    float s = 0.0f;
+    for (int64_t e = 0; e < M; ++e) s += G[(int64_t)pidx * M + e];
+    grad[pidx] = s;
+  }
+  if (pidx == kMlpParams) {
+    double s = 0.0;
+    for (int64_t e = 0; e < M; ++e) s += loss[e];
+    loss_out[0] = (float)(s / (double)M);
+  }
+}
+
+// Adam on the FP32 master parameters, then BF16 copies of the weights.
+__global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                           const float* __restrict__ g, float lr, float b1, float b2, float eps,
+                           float c1, float c2, MlpDev p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kMlpParams) return;
+  m[i] = b1 * m[i] + (1.0f - b1) * g[i];
+  v[i] = b2 * v[i] + (1.0f - b2) * g[i] * g[i];
+  w[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+  if (i < kMlpOffB1) p.w1b[i] = f32_to_bf16(w[i]);
+  else if (i >= kMlpOffW2 && i < kMlpOffB2) p.w2b[i - kMlpOffW2] = f32_to_bf16(w[i]);
+  else if (i >= kMlpOffW3 && i < kMlpOffB3) p.w3b[i - kMlpOffW3] = f32_to_bf16(w[i]);
+}
+
+__global__ void k_mlp_quantize(const float* __restrict__ w, MlpDev p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kMlpParams) return;
+  if (i < kMlpOffB1) p.w1b[i] = f32_to_bf16(w[i]);
+  else if (i >= kMlpOffW2 && i < kMlpOffB2) p.w2b[i - kMlpOffW2] = f32_to_bf16(w[i]);
+  else if (i >= kMlpOffW3 && i < kMlpOffB3) p.w3b[i - kMlpOffW3] = f32_to_bf16(w[i]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+int mlp_launch_score(const MlpDev& p, const float* X, int64_t M, float* out, int blocks,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_score, dim3(blocks), dim3(256), 0, st, p, X, M, out);
+  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "MLP score launch failed");
+}
+
+int mlp_launch_grad(const MlpDev& p, const float* X, const float* y, int64_t M, float* G,
+                    float* loss, float* grad, float* loss_out, hipStream_t st) {
+  const int blocks = (int)((M + 255) / 256);
+  hipLaunchKernelGGL(k_mlp_example_grad, dim3(blocks), dim3(256), 0, st, p, X, y, M, G, loss);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, G, loss,
+                     M, grad, loss_out);
+  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "MLP grad launch failed");
+}
+
+int mlp_launch_adam(float* w, float* m, float* v, const float* g, float lr, int step,
+                    const MlpDev& p, hipStream_t st) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, w, m, v, g, lr,
+                     b1, b2, eps, c1, c2, p);
+  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "MLP update launch failed");
+}
+
+int mlp_launch_quantize(const float* w, const MlpDev& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_quantize, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, w, p);
+  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "MLP quantize launch failed");
+}
+
+uint16_t mlp_bf16(float f) { return f32_to_bf16(f); }
+
+}  // namespace snc_amd
