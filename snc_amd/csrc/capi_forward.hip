@@ -105,6 +105,12 @@ struct snc_hip_plan {
   uint32_t* raster = nullptr;
   uint32_t* fbits = nullptr;
   int32_t* heavy_blk_ptr = nullptr;
+  int32_t *inj_blk_ptr = nullptr, *blk_nid = nullptr;
+  float *blk_amp = nullptr, *inj_heavy = nullptr;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  size_t n_blk_slots = 0;
+  int bucket_halves = 2;
   int halves = 1;
   size_t dirty_stride = 0;
   uint32_t *keys = nullptr, *keys_alt = nullptr;
@@ -139,6 +145,13 @@ struct snc_hip_plan {
     dfree(raster);
     dfree(fbits);
     dfree(heavy_blk_ptr);
+    dfree(inj_blk_ptr);
+    dfree(blk_nid);
+    dfree(blk_amp);
+    dfree(inj_heavy);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
     dfree(keys);
     dfree(keys_alt);
     dfree(vals);
@@ -331,12 +344,45 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   }
   P->h_inj_ptr = ip;
 
-  // One persistent launch per run when the device and graph allow it; its
-  // ext, bitmap and dirty flags are double-buffered by step parity.
+  // Fused per-step kernels: injections regrouped per (step, block of
+  // fused_block_neurons() neurons); hip-bucket steps heavy targets in their
+  // own blocks, so their injections go to a dense [step][hub] table.
+  const int nb = fused_block_neurons();
+  const int R = std::max(1, (n + nb - 1) / nb);
+  const bool hubs = variant == SNC_VARIANT_BUCKET && g.n_heavy > 0;
+  std::vector<int32_t> hub_index(hubs ? n : 0, -1);
+  for (int k = 0; hubs && k < g.n_heavy; ++k) hub_index[g.heavy_host[k]] = k;
+  std::vector<int32_t> blk_ptr((size_t)num_steps * (R + 1), 0), bnid;
+  std::vector<float> bamp, hinj(hubs ? (size_t)num_steps * g.n_heavy : 1, 0.0f);
+  for (int t = 0; t < num_steps; ++t) {
+    int32_t* bp = &blk_ptr[(size_t)t * (R + 1)];
+    int blk = 0;
+    bp[0] = (int32_t)bnid.size();
+    for (int k = ip[t]; k < ip[t + 1]; ++k) {  // entries sorted by neuron
+      const int j = inid[k];
+      if (hubs && hub_index[j] >= 0) {
+        hinj[(size_t)t * g.n_heavy + hub_index[j]] = iamp[k];
+        continue;
+      }
+      while (blk < j / nb) bp[++blk] = (int32_t)bnid.size();
+      bnid.push_back(j);
+      bamp.push_back(iamp[k]);
+    }
+    while (blk < R) bp[++blk] = (int32_t)bnid.size();
+  }
+
+  // One persistent launch per run when the device and graph allow it (and it
+  // is asked for); its ext is double-buffered by step parity. The bucket
+  // bitmap and dirty flags are double-buffered in every mode.
   P->persistent = persist_choose(g, variant, P->pcfg);
   const int halves = P->persistent ? 2 : 1;
+  const int bucket_halves = 2;
   const size_t dirty_stride = ((size_t)n + 3) / 4 * 4;
   int rc = SNC_OK;
+  if (!rc) rc = dupload(&P->inj_blk_ptr, blk_ptr.data(), blk_ptr.size());
+  if (!rc) rc = dupload(&P->blk_nid, bnid.data(), bnid.size());
+  if (!rc) rc = dupload(&P->blk_amp, bamp.data(), bamp.size());
+  if (!rc) rc = dupload(&P->inj_heavy, hinj.data(), hinj.size());
   if (!rc) rc = dupload(&P->inj_ptr, ip.data(), ip.size());
   if (!rc) rc = dupload(&P->inj_nid, inid.data(), inid.size());
   if (!rc) rc = dupload(&P->inj_amp, iamp.data(), iamp.size());
@@ -347,14 +393,15 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   if (!rc) rc = dalloc(&P->fired, (size_t)n);
   if (!rc) rc = dalloc(&P->fcnt, 2);
   if (!rc) rc = dalloc(&P->nspk, (size_t)n);
-  if (!rc) rc = dalloc(&P->dirty, dirty_stride * halves);  // cleared in 4-byte words
+  if (!rc) rc = dalloc(&P->dirty, dirty_stride * bucket_halves);  // cleared in 4-byte words
   P->n_bits_words = (size_t)(g.P / 32);
-  if (!rc && variant == SNC_VARIANT_BUCKET) rc = dalloc(&P->bits, P->n_bits_words * halves);
+  if (!rc && variant == SNC_VARIANT_BUCKET) rc = dalloc(&P->bits, P->n_bits_words * bucket_halves);
   P->core.lif_grid = std::max(1, (n + 255) / 256);
   P->core.expand_grid = std::max(1, std::min(2048, (n + 255) / 256));
   P->core.n_heavy = variant == SNC_VARIANT_BUCKET ? g.n_heavy : 0;
-  if (!rc) rc = dalloc(&P->blk_spikes, (size_t)std::max(P->core.lif_grid, 4096));
-  if (!rc) rc = dalloc(&P->blk_events, (size_t)std::max(P->core.expand_grid, 4096));
+  P->n_blk_slots = (size_t)std::max(4096, R + g.n_heavy + 1);
+  if (!rc) rc = dalloc(&P->blk_spikes, P->n_blk_slots);
+  if (!rc) rc = dalloc(&P->blk_events, P->n_blk_slots);
   if (variant == SNC_VARIANT_SORT && (uint64_t)n * (uint64_t)g.L >= (1ull << 32))
     return fail(SNC_E_UNSUPPORTED, "hip-sort keys need n * ring_len < 2^32");
   if (!rc && variant == SNC_VARIANT_SORT) rc = sort_alloc(P.get());
@@ -366,9 +413,13 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
     P->pcfg.dirty_stride = (int64_t)dirty_stride;
   }
   P->halves = halves;
+  P->bucket_halves = bucket_halves;
   P->dirty_stride = dirty_stride;
   if (rc) return rc;
   if (hipStreamCreateWithFlags(&P->core.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&P->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&P->join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&P->ev0) != hipSuccess || hipEventCreate(&P->ev1) != hipSuccess)
     return fail(SNC_E_HIP, "stream/event creation failed");
 
@@ -410,6 +461,15 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   a.hist = P->hist;
   a.ev_ptr = P->ev_ptr;
   a.fbits = P->fbits;
+  a.inj_blk_ptr = P->inj_blk_ptr;
+  a.blk_nid = P->blk_nid;
+  a.blk_amp = P->blk_amp;
+  a.inj_heavy = P->inj_heavy;
+  a.is_heavy = g.is_heavy;
+  a.n_reg_blocks = R;
+  a.n_heavy = hubs ? g.n_heavy : 0;
+  a.bits_half = (int64_t)P->n_bits_words;
+  a.dirty_half = (int64_t)dirty_stride;
 
   *out = P.release();
   return SNC_OK;
@@ -439,10 +499,10 @@ int reset_state(snc_hip_plan* P, bool raster) {
   fill(P->refr, sizeof(int32_t) * n);
   fill(P->fcnt, sizeof(int32_t) * 2);
   fill(P->nspk, sizeof(int32_t) * n);
-  fill(P->dirty, P->dirty_stride * P->halves);
-  fill(P->bits, sizeof(uint32_t) * P->n_bits_words * P->halves);
-  fill(P->blk_spikes, sizeof(unsigned long long) * std::max(P->core.lif_grid, 4096));
-  fill(P->blk_events, sizeof(unsigned long long) * std::max(P->core.expand_grid, 4096));
+  fill(P->dirty, P->dirty_stride * P->bucket_halves);
+  fill(P->bits, sizeof(uint32_t) * P->n_bits_words * P->bucket_halves);
+  fill(P->blk_spikes, sizeof(unsigned long long) * P->n_blk_slots);
+  fill(P->blk_events, sizeof(unsigned long long) * P->n_blk_slots);
   if (raster) fill(P->raster, sizeof(uint32_t) * (size_t)P->T * P->a.words);
   if (!rc) rc = fwd_launch_first_injection(P->core, P->a);
   return rc;
@@ -453,10 +513,26 @@ int ensure_raster(snc_hip_plan* P) {
   return dalloc(&P->raster, (size_t)std::max(1, P->T) * P->a.words);
 }
 
-int launch_chain(snc_hip_plan* P, const StepArgs& a) {
+// The step chain. While capturing, each step's hub kernel runs on a forked
+// side stream so it overlaps the regular step kernel; eagerly both go to the
+// plan's stream.
+int launch_chain(snc_hip_plan* P, const StepArgs& a, bool capturing) {
+  const bool fork = capturing && a.n_heavy > 0 && P->variant == SNC_VARIANT_BUCKET;
   for (int t = 0; t < P->T; ++t) {
-    int rc = P->variant == SNC_VARIANT_SORT ? sort_launch_step(P->core, a, t)
-                                           : fwd_launch_step(P->core, a, t);
+    int rc;
+    if (P->variant == SNC_VARIANT_SORT) {
+      rc = sort_launch_step(P->core, a, t);
+    } else if (fork) {
+      if (hipEventRecord(P->fork, P->core.stream) != hipSuccess ||
+          hipStreamWaitEvent(P->side, P->fork, 0) != hipSuccess)
+        return fail(SNC_E_HIP, "fork failed");
+      rc = fused_launch_step(P->core, a, t, P->side);
+      if (!rc && (hipEventRecord(P->join, P->side) != hipSuccess ||
+                  hipStreamWaitEvent(P->core.stream, P->join, 0) != hipSuccess))
+        rc = fail(SNC_E_HIP, "join failed");
+    } else {
+      rc = fused_launch_step(P->core, a, t, nullptr);
+    }
     if (rc) return rc;
   }
   return SNC_OK;
@@ -471,13 +547,13 @@ int launch_loop(snc_hip_plan* P, bool raster) {
   StepArgs a = P->a;
   a.raster = raster ? P->raster : nullptr;
   if (P->persistent) return persist_launch(P->G->g, P->core, a, P->pcfg);
-  if (!use_graphs() || P->T == 0) return launch_chain(P, a);
+  if (!use_graphs() || P->T == 0) return launch_chain(P, a, false);
   hipGraphExec_t& ge = P->graph_exec[raster ? 1 : 0];
   if (!ge) {
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(P->core.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return fail(SNC_E_HIP, "hipStreamBeginCapture failed");
-    int rc = launch_chain(P, a);
+    int rc = launch_chain(P, a, true);
     hipError_t e = hipStreamEndCapture(P->core.stream, &graph);
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -525,7 +601,7 @@ int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stat
   P->last_ms = ms;
 
   const int n = g.n;
-  std::vector<unsigned long long> bs(std::max(P->core.lif_grid, 4096)), be(std::max(P->core.expand_grid, 4096));
+  std::vector<unsigned long long> bs(P->n_blk_slots), be(P->n_blk_slots);
   std::vector<int32_t> nspk(n);
   if (hipMemcpy(bs.data(), P->blk_spikes, bs.size() * sizeof(bs[0]), hipMemcpyDeviceToHost) != hipSuccess ||
       hipMemcpy(be.data(), P->blk_events, be.size() * sizeof(be[0]), hipMemcpyDeviceToHost) != hipSuccess ||
@@ -609,7 +685,9 @@ int snc_hip_plan_profile(snc_hip_plan* P, double* kernel_ms, int64_t* launches) 
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
     if (!rc) span(SNC_KERNEL_PERSISTENT, ev[0], ev[1]);
   } else {
-    const bool heavy = P->variant == SNC_VARIANT_BUCKET && P->core.n_heavy > 0;
+    // Fused path: the hub kernel (hip-bucket) and the step kernel timed
+    // separately, one after the other on the plan's stream.
+    const bool heavy = P->variant == SNC_VARIANT_BUCKET && a.n_heavy > 0;
     for (int t = 0; t < P->T && !rc; ++t) {
       (void)hipEventRecord(ev[0], st);
       if (P->variant == SNC_VARIANT_SORT) {
@@ -619,17 +697,14 @@ int snc_hip_plan_profile(snc_hip_plan* P, double* kernel_ms, int64_t* launches) 
         if (!rc) span(SNC_KERNEL_SORT, ev[0], ev[3]);
         continue;
       }
-      rc = fwd_launch_heavy(P->core, a, t);
+      rc = fused_launch_hub(P->core, a, t, st);
       (void)hipEventRecord(ev[1], st);
-      if (!rc) rc = fwd_launch_lif(P->core, a, t);
+      if (!rc) rc = fused_launch_regular(P->core, a, t);
       (void)hipEventRecord(ev[2], st);
-      if (!rc) rc = fwd_launch_expand(P->core, a, t);
-      (void)hipEventRecord(ev[3], st);
       if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
       if (rc) break;
-      if (heavy && t > 0) span(SNC_KERNEL_FOLD_HEAVY, ev[0], ev[1]);
+      if (heavy) span(SNC_KERNEL_FOLD_HEAVY, ev[0], ev[1]);
       span(SNC_KERNEL_LIF, ev[1], ev[2]);
-      span(SNC_KERNEL_EXPAND, ev[2], ev[3]);
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);

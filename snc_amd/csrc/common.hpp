@@ -51,3 +51,8 @@ void pack_csr(int32_t n, std::vector<EdgeRec>& edges, bool sort_rows, GraphArray
 std::string validate_view(const snc_graph_view& g);
 
 }  // namespace snc_amd
+
+// Opaque handle of include/snc_gen.h: owned SNNGraph arrays.
+struct snc_graph_buf {
+  snc_amd::GraphArrays g;
+};

@@ -57,6 +57,16 @@ struct StepArgs {
   int32_t* vals_alt;
   int32_t* hist;
   int32_t* ev_ptr;
+  // fused step kernel
+  const int32_t* inj_blk_ptr;  // [T][n_reg_blocks + 1] per-(step, block) injection ranges
+  const int32_t* blk_nid;      // injections in (step, block) order, heavy targets excluded
+  const float* blk_amp;
+  const float* inj_heavy;      // [T][n_heavy] injections of heavy targets (hip-bucket)
+  const uint8_t* is_heavy;     // [n]
+  int n_reg_blocks;
+  int n_heavy;
+  int64_t bits_half;           // words of one bitmap half (step parity)
+  int64_t dirty_half;          // bytes of one dirty-flag half
 };
 
 struct FwdPlanCore {
@@ -74,6 +84,12 @@ int fwd_launch_heavy(const FwdPlanCore& pc, const StepArgs& a, int t);
 int fwd_launch_lif(const FwdPlanCore& pc, const StepArgs& a, int t);
 int fwd_launch_expand(const FwdPlanCore& pc, const StepArgs& a, int t);
 int fwd_launch_first_injection(const FwdPlanCore& pc, const StepArgs& a);
+
+// fused per-step kernels (step.hip); hub_stream may be null (same stream)
+int fused_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_t hub_stream);
+int fused_launch_hub(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_t st);
+int fused_launch_regular(const FwdPlanCore& pc, const StepArgs& a, int t);
+int fused_block_neurons();
 
 // hip-sort (sort.hip)
 size_t sort_hist_words(int n);

@@ -25,9 +25,6 @@
 #include "rng.hpp"
 #include "snc_gen.h"
 
-struct snc_graph_buf {
-  snc_amd::GraphArrays g;
-};
 struct snc_event_buf {
   std::vector<snc_event> ev;
 };
