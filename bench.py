@@ -15,8 +15,9 @@ Prints one JSON line (rank 0). Extra objects:
 
 Byte model (algorithmic, per step): 16 B per neuron (membrane read+write,
 ring slot read+clear), 9 B per synaptic event (post id 4, weight 4, delay 1),
-8 B per fired neuron (row_ptr pair). The LIF kernel is credited the neuron
-term, the expansion kernel the event and fired terms.
+8 B per fired neuron (row_ptr pair). Each kernel is credited the neurons it
+steps and the rows it expands: the hub kernel (hip-bucket) the heavy
+targets, the step kernel the rest (everything for hip-atomic).
 """
 from __future__ import annotations
 
@@ -38,24 +39,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, n, events, spikes, launches):
-    """Algorithmic bytes per launch of one kernel class over a run."""
-    if launches <= 0:
-        return 0.0
-    if name == "k_lif":
-        return 16.0 * n
-    if name == "k_expand":
-        return (9.0 * events + 8.0 * spikes) / launches
-    if name == "k_persistent":
-        return (16.0 * n * launches_steps[0] + 9.0 * events + 8.0 * spikes) / launches
-    if name == "sort_step":
-        return (16.0 * n * launches + 9.0 * events + 8.0 * spikes) / launches
-    if name == "k_fold_heavy":
-        return 0.0
-    return 0.0
+def kernel_units(g, prof, neuron_spikes, steps):
+    """Work per kernel class over one run: (neurons stepped, events expanded,
+    rows fired, launches). The hub kernel steps the heavy targets and expands
+    their own rows; the step kernel does the rest."""
+    import numpy as np
+
+    n = g.num_neurons
+    indeg = np.bincount(g.post_ids, minlength=n) if g.num_synapses else np.zeros(n, np.int64)
+    outdeg = np.diff(g.row_ptr).astype(np.int64)
+    spk = neuron_spikes.astype(np.int64)
+    hub = indeg > prof["heavy_threshold"]
+    ev_hub, f_hub = int((spk * outdeg)[hub].sum()), int(spk[hub].sum())
+    ev_all, f_all = int((spk * outdeg).sum()), int(spk.sum())
+    n_hub = int(hub.sum())
+    return {
+        "k_hub_step": (n_hub * steps, ev_hub, f_hub),
+        "k_step_with_hubs": ((n - n_hub) * steps, ev_all - ev_hub, f_all - f_hub),
+        "k_step": (n * steps, ev_all, f_all),
+        "sort_step": (n * steps, ev_all, f_all),
+        "k_persist": (n * steps, ev_all, f_all),
+    }
 
 
-launches_steps = [0]
+def model_bytes(neuron_steps, events, fired):
+    return 16.0 * neuron_steps + 9.0 * events + 8.0 * fired
 
 
 def cpu_baseline(w, repeats):
@@ -91,6 +99,106 @@ def cpu_baseline(w, repeats):
     return out
 
 
+def _train_session(w, elig):
+    from snc_amd.train import HipTrainSession
+
+    s = HipTrainSession(w.graph, w.cfg, batch=w.batch, max_steps=w.max_steps, elig=elig)
+    if w.init_weights is not None:
+        s.set_weights(w.init_weights)
+    n = w.labels.size
+    if w.kind == "features":
+        s.upload_features(w.X, w.labels, w.lengths)
+    else:
+        s.upload_events(w.events, n, w.labels, w.lengths)
+    return s
+
+
+def eprop_bench(w, epochs, plain_batches, elig):
+    """Batched persistent e-prop against the plain per-sample path on one
+    workload. Both start from the same weights; the plain path runs the first
+    `plain_batches` minibatches of epoch 1 and its weights are compared with
+    the batched path's after the same minibatches (same sums in the same
+    order, so they must be bit-identical; equal weights give equal accuracy).
+    samples/s are wall-clock rates of whole training steps (host loop,
+    launches, synchronisation and the per-step readback included)."""
+    import numpy as np
+    from snc_amd.train import BATCHED, PLAIN
+
+    n = w.labels.size
+    tr = np.arange(w.n_train, dtype=np.int32)
+    te = np.arange(w.n_train, n, dtype=np.int32)
+    rng = np.random.default_rng(12345)
+    order = rng.permutation(tr).astype(np.int32)
+    batches = [order[i:i + w.batch] for i in range(0, order.size, w.batch)]
+    nb = min(plain_batches, len(batches))
+
+    sb = _train_session(w, elig)
+    acc0 = sb.evaluate(te)
+    sb.step(batches[0], epoch=0, mode=BATCHED, apply=False)   # warm-up (no update)
+    synchronize_train()
+
+    def run(sess, mode, bl, epoch):
+        t0 = time.perf_counter()
+        dev, samples, spikes, events = 0.0, 0, 0, 0
+        for b in bl:
+            st, _, _ = sess.step(b, epoch=epoch, mode=mode, apply=True)
+            dev += st.ms
+            samples += st.samples
+            spikes += st.spikes
+            events += st.synaptic_events
+        return time.perf_counter() - t0, dev, samples, spikes, events
+
+    # plain path over the first nb minibatches
+    sp = _train_session(w, elig)
+    wall_p, dev_p, n_p, spk_p, ev_p = run(sp, PLAIN, batches[:nb], 1)
+    w_plain = sp.get_weights()
+    acc_plain = sp.evaluate(te)
+    sp.close()
+    # batched path over the same minibatches, then the rest of the epochs
+    wall_b, dev_b, n_b, spk_b, ev_b = run(sb, BATCHED, batches[:nb], 1)
+    w_batch = sb.get_weights()
+    acc_same = sb.evaluate(te)
+    identical = bool(np.array_equal(w_plain, w_batch))
+    if nb < len(batches):
+        r = run(sb, BATCHED, batches[nb:], 1)
+        wall_b, dev_b, n_b = wall_b + r[0], dev_b + r[1], n_b + r[2]
+        spk_b, ev_b = spk_b + r[3], ev_b + r[4]
+    curve = [sb.evaluate(te)]
+    for ep in range(2, epochs + 1):
+        o = np.random.default_rng(12345 + ep).permutation(tr).astype(np.int32)
+        r = run(sb, BATCHED, [o[i:i + w.batch] for i in range(0, o.size, w.batch)], ep)
+        wall_b, dev_b, n_b = wall_b + r[0], dev_b + r[1], n_b + r[2]
+        spk_b, ev_b = spk_b + r[3], ev_b + r[4]
+        curve.append(sb.evaluate(te))
+    sb.close()
+    bsps, psps = n_b / wall_b, n_p / wall_p
+    out = {
+        "workload": w.name, "neurons": w.graph.num_neurons, "synapses": w.graph.num_synapses,
+        "batch": w.batch, "train_samples": int(tr.size), "test_samples": int(te.size),
+        "lengths": [int(w.lengths.min()), int(w.lengths.max())],
+        "eligibility": "f64" if elig == 1 else "f32",
+        "batched": {"samples_per_s": bsps, "device_samples_per_s": n_b / (dev_b / 1e3),
+                    "samples": n_b, "epochs": epochs, "GSynOps/s": ev_b / wall_b / 1e9,
+                    "test_acc_per_epoch": curve},
+        "plain": {"samples_per_s": psps, "device_samples_per_s": n_p / (dev_p / 1e3),
+                  "samples": n_p, "GSynOps/s": ev_p / wall_p / 1e9},
+        "speedup_batched_vs_plain": bsps / psps,
+        "test_acc_initial": acc0,
+        "equal_accuracy_check": {"minibatches": nb, "weights_bit_identical": identical,
+                                 "test_acc_plain": acc_plain, "test_acc_batched": acc_same,
+                                 "acc_diff_pt": 100.0 * (acc_same - acc_plain)},
+    }
+    log(f"e-prop {w.name}: batched {bsps:.1f} samples/s, plain {psps:.1f} samples/s "
+        f"(x{bsps / psps:.1f}), identical={identical}, acc {acc0:.3f} -> {curve}")
+    return out
+
+
+def synchronize_train():
+    from snc_amd.runtime import synchronize
+
+    synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +212,11 @@ def main():
     ap.add_argument("--timesteps", type=int, default=1000)
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eprop-epochs", type=int, default=2)
+    ap.add_argument("--plain-batches", type=int, default=1,
+                    help="minibatches the plain per-sample path is timed on")
+    ap.add_argument("--no-eprop", action="store_true")
+    ap.add_argument("--eprop-only", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="build the workload and run K steps without extra measurements")
     args = ap.parse_args()
@@ -118,6 +231,18 @@ def main():
 
     if not available():
         raise SystemExit("HIP backend unavailable (no gfx950 device or library not built)")
+
+    def eprop_all():
+        res = {}
+        t = time.time()
+        res["c2"] = eprop_bench(K.c2(), args.eprop_epochs, args.plain_batches, 1)
+        res["c3"] = eprop_bench(K.c3(), 1, args.plain_batches, 1)
+        log(f"e-prop measurements: {time.time() - t:.1f}s")
+        return res
+
+    if args.eprop_only:
+        print(json.dumps({"eprop": eprop_all()}))
+        return
     t0 = time.time()
     w = K.c1(args.width, args.height, args.timesteps)
     g = w.graph
@@ -151,10 +276,11 @@ def main():
         return
 
     prof = plan.profile()
-    launches_steps[0] = w.num_steps
+    units = kernel_units(g, hg.profile(), ref.neuron_spikes, w.num_steps)
     kernels = {}
     for name, (tot, k) in prof.items():
-        by = kernel_bytes(name, g.num_neurons, events, spikes, k)
+        key = "k_step_with_hubs" if name == "k_step" and "k_hub_step" in prof else name
+        by = model_bytes(*units[key]) / k
         avg = tot / k
         kernels[name] = {"avg_ms": avg, "launches": k, "total_ms": tot,
                          "bytes_per_launch": by,
@@ -195,6 +321,8 @@ def main():
                       "spikes_per_step": spikes, "events_per_step": events,
                       "parallelism": "single-gpu"},
            "roofline": roofline, "variants": variants}
+    if not args.no_eprop:
+        out["eprop"] = eprop_all()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_repeats)
     print(json.dumps(out))

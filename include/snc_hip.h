@@ -127,7 +127,9 @@ typedef struct snc_graph_profile {
   int32_t max_in_degree;
   int32_t max_out_degree;
   double mean_in_degree;
-  double heavy_fraction; /* fraction of synapses onto targets with in-degree > 256 */
+  double heavy_fraction;  /* fraction of synapses onto heavy targets */
+  int32_t heavy_threshold; /* in-degree above which a target is heavy (hub) */
+  int32_t num_heavy;       /* number of heavy targets */
 } snc_graph_profile;
 int snc_hip_graph_profile_get(const snc_hip_graph* g, snc_graph_profile* out);
 
@@ -151,14 +153,20 @@ int snc_hip_plan_kernel_time(const snc_hip_plan* p, double* avg_ms, int64_t* lau
 void snc_hip_plan_free(snc_hip_plan* p);
 
 /* Per-kernel device time of one instrumented run: eager launches with a HIP
- * event around every kernel on the plan's stream. kernel_ms_total[k] and
- * launches[k] are filled for each class k = SNC_KERNEL_*. */
-#define SNC_KERNEL_FOLD_HEAVY 0
-#define SNC_KERNEL_LIF 1
-#define SNC_KERNEL_EXPAND 2
-#define SNC_KERNEL_SORT 3
-#define SNC_KERNEL_PERSISTENT 4
-#define SNC_KERNEL_CLASSES 5
+ * event around every kernel on the plan's stream (the hub and the regular
+ * step kernel one after the other, where the launched graph overlaps them).
+ * kernel_ms_total[k] and launches[k] are filled for each class
+ * k = SNC_KERNEL_*:
+ *   HUB_STEP    k_hub_step, one block per heavy target (hip-bucket)
+ *   STEP        k_step: fold, integrate/fire and expansion of the regular
+ *               neurons (all neurons for hip-atomic)
+ *   SORT_STEP   the whole kernel chain of one hip-sort step
+ *   PERSISTENT  k_persist, the whole run in one cooperative launch */
+#define SNC_KERNEL_HUB_STEP 0
+#define SNC_KERNEL_STEP 1
+#define SNC_KERNEL_SORT_STEP 2
+#define SNC_KERNEL_PERSISTENT 3
+#define SNC_KERNEL_CLASSES 4
 int snc_hip_plan_profile(snc_hip_plan* p, double* kernel_ms_total, int64_t* launches);
 
 /* Wait until all work the library queued on the device has finished. */
@@ -249,8 +257,11 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
 
 /* cudatrain::train_epoch / evaluate (cuda_trainer.cu:374-435) over the
  * uploaded dataset: shuffled minibatches (mt19937_64 seeded with
- * 0x9e3779b97f4a7c15 * (epoch + 1)); evaluate runs without updates. */
-int snc_hip_train_epoch(snc_hip_train* s, int32_t epoch, int32_t mode, snc_train_stats* st);
+ * 0x9e3779b97f4a7c15 * (epoch + 1)); evaluate runs without updates.
+ * train_epoch covers the samples idx[0..n) (the training split), or the whole
+ * uploaded pool when idx is NULL. */
+int snc_hip_train_epoch(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t epoch,
+                        int32_t mode, snc_train_stats* st);
 int snc_hip_train_evaluate(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t mode,
                            snc_train_stats* st);
 

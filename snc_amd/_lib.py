@@ -25,7 +25,7 @@ VARIANT_SORT = 7
 VARIANT_AUTO = 8
 
 ROLE_INTERNAL, ROLE_INPUT, ROLE_OUTPUT = 0, 1, 2
-KERNEL_NAMES = ["k_fold_heavy", "k_lif", "k_expand", "sort_step", "k_persistent"]
+KERNEL_NAMES = ["k_hub_step", "k_step", "sort_step", "k_persist"]
 RESET_TO_VALUE, RESET_SUBTRACT = 0, 1
 
 
@@ -102,6 +102,8 @@ class GraphProfile(C.Structure):
         ("max_out_degree", C.c_int32),
         ("mean_in_degree", C.c_double),
         ("heavy_fraction", C.c_double),
+        ("heavy_threshold", C.c_int32),
+        ("num_heavy", C.c_int32),
     ]
 
 

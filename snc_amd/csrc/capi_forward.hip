@@ -282,6 +282,8 @@ int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
   pf.max_out_degree = n ? *std::max_element(g.out_degree.begin(), g.out_degree.end()) : 0;
   pf.mean_in_degree = n ? (double)S / n : 0.0;
   pf.heavy_fraction = S ? (double)heavy_syn / (double)S : 0.0;
+  pf.heavy_threshold = g.heavy_threshold;
+  pf.num_heavy = g.n_heavy;
   *out = H.release();
   return SNC_OK;
 }
@@ -694,7 +696,7 @@ int snc_hip_plan_profile(snc_hip_plan* P, double* kernel_ms, int64_t* launches) 
         rc = sort_launch_step(P->core, a, t);
         (void)hipEventRecord(ev[3], st);
         if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
-        if (!rc) span(SNC_KERNEL_SORT, ev[0], ev[3]);
+        if (!rc) span(SNC_KERNEL_SORT_STEP, ev[0], ev[3]);
         continue;
       }
       rc = fused_launch_hub(P->core, a, t, st);
@@ -703,8 +705,8 @@ int snc_hip_plan_profile(snc_hip_plan* P, double* kernel_ms, int64_t* launches) 
       (void)hipEventRecord(ev[2], st);
       if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
       if (rc) break;
-      if (heavy) span(SNC_KERNEL_FOLD_HEAVY, ev[0], ev[1]);
-      span(SNC_KERNEL_LIF, ev[1], ev[2]);
+      if (heavy) span(SNC_KERNEL_HUB_STEP, ev[0], ev[1]);
+      span(SNC_KERNEL_STEP, ev[1], ev[2]);
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);

@@ -543,15 +543,23 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
   return SNC_OK;
 }
 
-int snc_hip_train_epoch(snc_hip_train* s, int32_t epoch, int32_t mode, snc_train_stats* st) {
+int snc_hip_train_epoch(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t epoch,
+                        int32_t mode, snc_train_stats* st) {
   if (!s || !s->n_data) return fail(SNC_E_INVALID, "no dataset uploaded");
-  std::vector<int32_t> order(s->n_data);
-  std::iota(order.begin(), order.end(), 0);
+  if (idx && n < 0) return fail(SNC_E_INVALID, "negative sample count");
+  std::vector<int32_t> order;
+  if (idx) {
+    order.assign(idx, idx + n);
+  } else {
+    order.resize(s->n_data);
+    std::iota(order.begin(), order.end(), 0);
+  }
   std::mt19937_64 rng(0x9e3779b97f4a7c15ULL * (uint64_t)(epoch + 1));
   std::shuffle(order.begin(), order.end(), rng);
+  const int total = (int)order.size();
   snc_train_stats tot{};
-  for (int off = 0; off < s->n_data; off += s->batch) {
-    const int B = std::min(s->batch, s->n_data - off);
+  for (int off = 0; off < total; off += s->batch) {
+    const int B = std::min(s->batch, total - off);
     snc_train_stats one{};
     int rc = snc_hip_train_step(s, order.data() + off, B, epoch, mode, 1, nullptr, nullptr, &one);
     if (rc) return rc;

@@ -174,10 +174,11 @@ class ForwardPlan:
 
     def profile(self) -> dict:
         """Per-kernel device time of one instrumented run: {name: (total_ms, launches)}."""
-        ms = (C.c_double * 5)()
-        k = (C.c_int64 * 5)()
+        nk = len(L.KERNEL_NAMES)
+        ms = (C.c_double * nk)()
+        k = (C.c_int64 * nk)()
         L.check(L.lib().snc_hip_plan_profile(self._h, ms, k))
-        return {L.KERNEL_NAMES[i]: (ms[i], k[i]) for i in range(5) if k[i]}
+        return {L.KERNEL_NAMES[i]: (ms[i], k[i]) for i in range(nk) if k[i]}
 
     def close(self):
         if self._h:

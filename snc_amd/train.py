@@ -93,7 +93,8 @@ _SIGS = {
     "snc_hip_train_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                      C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                      C.POINTER(C.c_uint32), C.POINTER(TrainStatsC)]),
-    "snc_hip_train_epoch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TrainStatsC)]),
+    "snc_hip_train_epoch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                      C.c_int32, C.POINTER(TrainStatsC)]),
     "snc_hip_train_evaluate": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                          C.POINTER(TrainStatsC)]),
     "snc_hip_train_reset_stats": (C.c_int, [C.c_void_p]),
@@ -185,9 +186,15 @@ class HipTrainSession:
                                            L.ptr(raster, C.c_uint32), C.byref(st)))
         return _stats(st), logits, raster
 
-    def train_epoch(self, epoch, mode=BATCHED) -> EpochStats:
+    def train_epoch(self, epoch, mode=BATCHED, idx=None) -> EpochStats:
+        """One shuffled pass over idx (default: the whole uploaded pool)."""
         st = TrainStatsC()
-        L.check(_bind().snc_hip_train_epoch(self._h, epoch, mode, C.byref(st)))
+        if idx is None:
+            L.check(_bind().snc_hip_train_epoch(self._h, None, 0, epoch, mode, C.byref(st)))
+        else:
+            idx = np.ascontiguousarray(idx, np.int32)
+            L.check(_bind().snc_hip_train_epoch(self._h, L.ptr(idx, C.c_int32), idx.size, epoch,
+                                                mode, C.byref(st)))
         return _stats(st)
 
     def evaluate(self, idx=None, mode=BATCHED) -> float:
