@@ -205,7 +205,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", default="hip-auto")
-    ap.add_argument("--compare", default="hip-atomic,hip-bucket",
+    ap.add_argument("--compare", default="hip-atomic,hip-bucket,hip-sort,hip-auto-exact",
                     help="other variants to time (reported under 'variants')")
     ap.add_argument("--width", type=int, default=500)
     ap.add_argument("--height", type=int, default=400)
@@ -213,7 +213,7 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--eprop-epochs", type=int, default=2)
-    ap.add_argument("--plain-batches", type=int, default=1,
+    ap.add_argument("--plain-batches", type=int, default=8,
                     help="minibatches the plain per-sample path is timed on")
     ap.add_argument("--no-eprop", action="store_true")
     ap.add_argument("--eprop-only", action="store_true")
@@ -253,11 +253,22 @@ def main():
     def make_plan(name):
         return ForwardPlan(hg, w.samples[0], w.num_steps, w.lif, parse_backend(name))
 
+    t_plan = time.time()
     plan = make_plan(args.variant)
+    plan_s = time.time() - t_plan
     ref = plan.run()
     events, spikes = ref.stats.synaptic_events, ref.stats.spikes
-    log(f"variant {args.variant} -> {plan.variant.name}: spikes={spikes} events={events} "
-        f"run={ref.stats.ms:.2f} ms")
+    log(f"variant {args.variant} -> {plan.variant.name} {plan.config}: spikes={spikes} "
+        f"events={events} run={ref.stats.ms:.2f} ms (plan {plan_s:.1f}s)")
+    autotune = None
+    if args.variant in ("hip-auto", "hip-auto-exact"):
+        rep = hg.tune_report(exact_only=args.variant == "hip-auto-exact")
+        autotune = {"search_ms": rep["search_ms"], "trial_steps": rep["trial_steps"],
+                    "chosen": rep["chosen"],
+                    "candidates": [{"variant": c.variant.name, "block_neurons": c.block_neurons,
+                                    "heavy_threshold": c.heavy_threshold,
+                                    "persistent": c.persistent, "ms_per_step": m}
+                                   for c, m in rep["candidates"]]}
 
     for _ in range(args.warmup):
         plan.launch()
@@ -308,7 +319,8 @@ def main():
         synchronize()
         wv = (time.perf_counter() - tt) / max(1, args.steps)
         variants[name] = {"GSynOps/s": r.stats.synaptic_events / wv / 1e9, "ms_per_step": wv * 1e3,
-                          "spikes": r.stats.spikes, "events": r.stats.synaptic_events}
+                          "spikes": r.stats.spikes, "events": r.stats.synaptic_events,
+                          "resolved": p.variant.name}
         log(f"variant {name}: {variants[name]}")
         p.close()
 
@@ -318,9 +330,12 @@ def main():
            "data": "synthetic",
            "config": {"workload": w.name, "neurons": g.num_neurons, "synapses": g.num_synapses,
                       "timesteps": w.num_steps, "batch": 1, "variant": plan.variant.name,
+                      "plan": {"block_neurons": plan.config.block_neurons,
+                               "heavy_threshold": plan.config.heavy_threshold,
+                               "persistent": plan.config.persistent},
                       "spikes_per_step": spikes, "events_per_step": events,
                       "parallelism": "single-gpu"},
-           "roofline": roofline, "variants": variants}
+           "roofline": roofline, "variants": variants, "autotune": autotune}
     if not args.no_eprop:
         out["eprop"] = eprop_all()
     if not args.no_cpu:

@@ -46,6 +46,8 @@ extern "C" {
 #define SNC_VARIANT_BUCKET 6 /* hip-bucket: per-target ordered fold, bit-exact */
 #define SNC_VARIANT_SORT 7   /* hip-sort: canonical expand + stable radix sort */
 #define SNC_VARIANT_AUTO 8   /* hip-auto: autotuner picks per graph           */
+#define SNC_VARIANT_AUTO_EXACT 9 /* hip-auto-exact: autotuner over the bit-exact
+                                    strategies (bucket, sort) only            */
 
 /* ---- graph roles (GraphRole, snn_graph.hpp:29) ---------------------------- */
 #define SNC_ROLE_INTERNAL 0
@@ -135,10 +137,44 @@ int snc_hip_graph_profile_get(const snc_hip_graph* g, snc_graph_profile* out);
 
 /* Build a forward plan: the sample's events are resolved to per-step
  * injections (build_schedule, cpu_backend.cpp:47-65) and uploaded, and the
- * device state is allocated. `variant` is SNC_VARIANT_*. */
+ * device state is allocated. `variant` is SNC_VARIANT_*: an explicit
+ * strategy runs with its default launch configuration; SNC_VARIANT_AUTO /
+ * SNC_VARIANT_AUTO_EXACT take the autotuner's configuration for this graph
+ * (searched on the first such call, then cached with the graph). */
 int snc_hip_plan_create(snc_hip_graph* g, const snc_event* events, int64_t n_events,
                         int32_t num_steps, const snc_lif* lif, int32_t variant,
                         snc_hip_plan** out);
+
+/* Launch configuration of a forward plan: the autotuner's search space. */
+typedef struct snc_plan_config {
+  int32_t variant;         /* SNC_VARIANT_ATOMIC / _BUCKET / _SORT                 */
+  int32_t block_neurons;   /* neurons per block of the step kernel: 128, 256, 512  */
+  int32_t heavy_threshold; /* hip-bucket: targets with in-degree above this are
+                              hubs stepped by their own block (bucket capacity of
+                              the per-lane fold); at most 512                       */
+  int32_t persistent;      /* 1: the whole run as one cooperative launch
+                              (hip-atomic / hip-bucket)                            */
+} snc_plan_config;
+int snc_hip_plan_create_config(snc_hip_graph* g, const snc_event* events, int64_t n_events,
+                               int32_t num_steps, const snc_lif* lif,
+                               const snc_plan_config* cfg, snc_hip_plan** out);
+int snc_hip_plan_get_config(const snc_hip_plan* p, snc_plan_config* out);
+
+/* The autotuner's search for a graph: the candidates it derived from the
+ * graph's in-degree statistics, the measured device time per step of each on
+ * a trial window of the first trial_steps steps of the sample that triggered
+ * the search, and the one chosen (the fastest). exact_only selects the
+ * SNC_VARIANT_AUTO_EXACT search. Fails when no search has run yet. */
+#define SNC_TUNE_MAX_CANDIDATES 24
+typedef struct snc_tune_report {
+  int32_t n_candidates;
+  int32_t chosen;
+  int32_t trial_steps;
+  double search_ms; /* wall time of the whole search */
+  snc_plan_config cand[SNC_TUNE_MAX_CANDIDATES];
+  double ms_per_step[SNC_TUNE_MAX_CANDIDATES];
+} snc_tune_report;
+int snc_hip_graph_tune_report(const snc_hip_graph* g, int32_t exact_only, snc_tune_report* out);
 /* Run the plan from a zero state. Inputs are already resident; stats->ms is
  * the device time of the step loop. */
 int snc_hip_plan_run(snc_hip_plan* p, snc_forward_capture* cap, snc_forward_stats* stats);

@@ -23,6 +23,7 @@ VARIANT_ATOMIC = 5
 VARIANT_BUCKET = 6
 VARIANT_SORT = 7
 VARIANT_AUTO = 8
+VARIANT_AUTO_EXACT = 9
 
 ROLE_INTERNAL, ROLE_INPUT, ROLE_OUTPUT = 0, 1, 2
 KERNEL_NAMES = ["k_hub_step", "k_step", "sort_step", "k_persist"]
@@ -140,6 +141,20 @@ class HubParams(C.Structure):
     ]
 
 
+class PlanConfig(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("block_neurons", C.c_int32),
+                ("heavy_threshold", C.c_int32), ("persistent", C.c_int32)]
+
+
+TUNE_MAX_CANDIDATES = 24
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n_candidates", C.c_int32), ("chosen", C.c_int32), ("trial_steps", C.c_int32),
+                ("search_ms", C.c_double), ("cand", PlanConfig * TUNE_MAX_CANDIDATES),
+                ("ms_per_step", C.c_double * TUNE_MAX_CANDIDATES)]
+
+
 _P = C.c_void_p
 _SIGS = {
     "snc_hip_last_error": (C.c_char_p, []),
@@ -149,6 +164,11 @@ _SIGS = {
     "snc_hip_graph_profile_get": (C.c_int, [_P, C.POINTER(GraphProfile)]),
     "snc_hip_plan_create": (C.c_int, [_P, C.POINTER(Event), C.c_int64, C.c_int32, C.POINTER(Lif),
                                       C.c_int32, C.POINTER(_P)]),
+    "snc_hip_plan_create_config": (C.c_int, [_P, C.POINTER(Event), C.c_int64, C.c_int32,
+                                             C.POINTER(Lif), C.POINTER(PlanConfig),
+                                             C.POINTER(_P)]),
+    "snc_hip_plan_get_config": (C.c_int, [_P, C.POINTER(PlanConfig)]),
+    "snc_hip_graph_tune_report": (C.c_int, [_P, C.c_int32, C.POINTER(TuneReport)]),
     "snc_hip_plan_run": (C.c_int, [_P, C.POINTER(ForwardCapture), C.POINTER(ForwardStats)]),
     "snc_hip_plan_launch": (C.c_int, [_P]),
     "snc_hip_plan_variant": (C.c_int32, [_P]),

@@ -1,10 +1,74 @@
-// Per-graph delivery strategy choice (first version: fixed preference).
+// Autotuner candidate generation from in-degree statistics.
+//
+// What the statistics decide:
+//   * hub thresholds of hip-bucket: the per-lane fold of the step kernel
+//     handles columns of up to 512 positions; lower thresholds move more
+//     targets to the block-per-hub kernel. A threshold is tried only when it
+//     changes the hub set (the number of targets above it differs from the
+//     next larger threshold) and the hub count stays within kMaxHubs (one
+//     block per hub per step).
+//   * hip-atomic (unless only bit-exact strategies are wanted) at every block
+//     size: its cost is the expansion's atomics, whose contention on hubs is
+//     measured by the trial rather than modelled.
+//   * hip-sort is tried once (its cost follows the events, not the shape).
+//   * persistent variants only when cooperative launch is available.
 #include "autotune.hpp"
 
-#include "snc_hip.h"
+#include <algorithm>
 
 namespace snc_amd {
 
-int autotune_variant(const DevGraph&, int64_t, int) { return SNC_VARIANT_BUCKET; }
+namespace {
+constexpr int kLightLimit = 512;
+constexpr int kMaxHubs = 16384;
+constexpr int kThresholds[] = {64, 128, 256, 512};
+}  // namespace
+
+snc_plan_config default_plan_config(int variant) {
+  snc_plan_config c{};
+  c.variant = variant;
+  c.block_neurons = 256;
+  c.heavy_threshold = kLightLimit;
+  c.persistent = 0;
+  return c;
+}
+
+std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop) {
+  std::vector<snc_plan_config> out;
+  auto add = [&](int variant, int block, int thr, int persistent) {
+    if ((int)out.size() >= SNC_TUNE_MAX_CANDIDATES) return;
+    snc_plan_config c{variant, block, thr, persistent};
+    out.push_back(c);
+  };
+  auto hubs_above = [&](int thr) {
+    int64_t k = 0;
+    for (int d : g.in_degree) k += d > thr;
+    return k;
+  };
+
+  // hip-bucket: hub thresholds that change the hub set, largest first.
+  std::vector<int> thr;
+  int64_t prev = -1;
+  for (int q = (int)(sizeof(kThresholds) / sizeof(int)) - 1; q >= 0; --q) {
+    const int64_t h = hubs_above(kThresholds[q]);
+    if (h > kMaxHubs) break;
+    if (h != prev) thr.push_back(kThresholds[q]);
+    prev = h;
+  }
+  if (thr.empty() || thr.front() != kLightLimit) thr.insert(thr.begin(), kLightLimit);
+  for (int t : thr)
+    for (int blk : {128, 256, 512}) add(SNC_VARIANT_BUCKET, blk, t, 0);
+
+  if (!exact_only)
+    for (int blk : {128, 256, 512}) add(SNC_VARIANT_ATOMIC, blk, kLightLimit, 0);
+  if ((uint64_t)g.n * (uint64_t)g.L < (1ull << 32)) add(SNC_VARIANT_SORT, 256, kLightLimit, 0);
+  if (coop) {
+    add(SNC_VARIANT_BUCKET, 256, kLightLimit, 1);
+    if (!exact_only) add(SNC_VARIANT_ATOMIC, 256, kLightLimit, 1);
+  }
+  return out;
+}
+
+int autotune_trial_steps(int num_steps) { return std::max(1, std::min(num_steps, 128)); }
 
 }  // namespace snc_amd

@@ -1,12 +1,27 @@
 // Per-graph choice of the delivery strategy and its launch shape.
+//
+// The search space is {strategy} x {step-kernel block} x {hub threshold} x
+// {persistent}; the candidates are derived from the graph's in-degree
+// statistics, timed on a trial window of the real input, and the fastest is
+// cached with the graph.
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "device_graph.hpp"
+#include "snc_hip.h"
 
 namespace snc_amd {
 
-int autotune_variant(const DevGraph& g, int64_t n_events, int num_steps);
+// Default configuration of an explicitly named strategy.
+snc_plan_config default_plan_config(int variant);
+
+// Candidates for the search. exact_only drops hip-atomic; coop says whether
+// the device can run the persistent (cooperative) kernel.
+std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop);
+
+// Steps of the trial window for a sample of num_steps steps.
+int autotune_trial_steps(int num_steps);
 
 }  // namespace snc_amd

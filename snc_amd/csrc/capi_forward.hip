@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -82,6 +83,9 @@ using namespace snc_amd;
 
 struct snc_hip_plan;
 static int sort_alloc(snc_hip_plan* P);
+static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_events,
+                           int num_steps, const snc_lif* lif, bool exact_only,
+                           snc_plan_config* out);
 
 struct snc_hip_plan {
   snc_hip_graph* G = nullptr;
@@ -90,8 +94,13 @@ struct snc_hip_plan {
   int T = 0;
   snc_lif lif{};
   int variant = SNC_VARIANT_ATOMIC;
+  snc_plan_config cfg{};
   bool persistent = false;
   PersistConfig pcfg{};
+  // hub set of this plan's threshold (hip-bucket and the persistent kernel)
+  std::vector<int32_t> heavy_host;
+  int32_t* heavy_ids = nullptr;
+  uint8_t* is_heavy = nullptr;
   std::vector<int32_t> h_inj_ptr;
   // owned device buffers
   int32_t *inj_ptr = nullptr, *inj_nid = nullptr;
@@ -149,6 +158,8 @@ struct snc_hip_plan {
     dfree(blk_nid);
     dfree(blk_amp);
     dfree(inj_heavy);
+    dfree(heavy_ids);
+    dfree(is_heavy);
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
     if (side) (void)hipStreamDestroy(side);
@@ -237,13 +248,12 @@ int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
     cval[p] = wsig[e];
     cdel[p] = del8[e];
   }
+  // Hubs at the default threshold, for the profile (plans build their own).
   std::vector<int32_t> heavy;
-  std::vector<uint8_t> is_heavy(n, 0);
   int64_t heavy_syn = 0;
   for (int j = 0; j < n; ++j)
     if (g.in_degree[j] > g.heavy_threshold) {
       heavy.push_back(j);
-      is_heavy[j] = 1;
       heavy_syn += g.in_degree[j];
     }
   g.n_heavy = (int)heavy.size();
@@ -258,8 +268,6 @@ int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
   if (!rc) rc = dupload(&g.ccol, ccol.data(), (size_t)n + 1);
   if (!rc) rc = dupload(&g.cval, cval.data(), (size_t)P);
   if (!rc) rc = dupload(&g.cdel, cdel.data(), (size_t)P);
-  if (!rc) rc = dupload(&g.heavy_ids, heavy.data(), heavy.size());
-  if (!rc) rc = dupload(&g.is_heavy, is_heavy.data(), (size_t)n);
   if (rc) return rc;
 
   g.role.assign(gv->role, gv->role + n);
@@ -296,25 +304,39 @@ int snc_hip_graph_profile_get(const snc_hip_graph* g, snc_graph_profile* out) {
   return SNC_OK;
 }
 
-int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_events,
-                        int32_t num_steps, const snc_lif* lif, int32_t variant,
-                        snc_hip_plan** out) {
-  if (!G || !lif || !out || (n_events > 0 && !events) || num_steps < 0)
+int snc_hip_plan_create_config(snc_hip_graph* G, const snc_event* events, int64_t n_events,
+                               int32_t num_steps, const snc_lif* lif,
+                               const snc_plan_config* cfg, snc_hip_plan** out) {
+  if (!G || !lif || !out || !cfg || (n_events > 0 && !events) || num_steps < 0)
     return fail(SNC_E_INVALID, "bad plan arguments");
   *out = nullptr;
   if (lif->reset_mode != SNC_RESET_TO_VALUE && lif->reset_mode != SNC_RESET_SUBTRACT)
     return fail(SNC_E_INVALID, "unknown reset mode");
+  const int variant = cfg->variant;
+  if (variant != SNC_VARIANT_ATOMIC && variant != SNC_VARIANT_BUCKET && variant != SNC_VARIANT_SORT)
+    return fail(SNC_E_INVALID, "unknown variant");
+  if (!fused_block_ok(cfg->block_neurons))
+    return fail(SNC_E_INVALID, "block_neurons must be 128, 256 or 512");
+  if (cfg->heavy_threshold < 1 || cfg->heavy_threshold > fused_light_max_in_degree())
+    return fail(SNC_E_INVALID, "heavy_threshold must be in [1, 512]");
   const DevGraph& g = G->g;
   std::unique_ptr<snc_hip_plan> P(new snc_hip_plan());
   P->G = G;
   P->T = num_steps;
   P->lif = *lif;
-  if (variant == SNC_VARIANT_AUTO) variant = autotune_variant(g, n_events, num_steps);
-  if (variant != SNC_VARIANT_ATOMIC && variant != SNC_VARIANT_BUCKET && variant != SNC_VARIANT_SORT)
-    return fail(SNC_E_INVALID, "unknown variant");
+  P->cfg = *cfg;
   P->variant = variant;
   P->core.variant = variant;
   const int n = g.n;
+
+  // Hub set of this plan's threshold.
+  std::vector<uint8_t> is_heavy_h((size_t)n, 0);
+  for (int j = 0; j < n; ++j)
+    if (g.in_degree[j] > cfg->heavy_threshold) {
+      P->heavy_host.push_back(j);
+      is_heavy_h[j] = 1;
+    }
+  const int n_heavy = (int)P->heavy_host.size();
 
   // Per-step injections, resolved channel -> INPUT neurons in the reference's
   // order (cpu_backend.cpp:50-63) and folded per neuron in that order, so the
@@ -349,13 +371,13 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   // Fused per-step kernels: injections regrouped per (step, block of
   // fused_block_neurons() neurons); hip-bucket steps heavy targets in their
   // own blocks, so their injections go to a dense [step][hub] table.
-  const int nb = fused_block_neurons();
+  const int nb = cfg->block_neurons;
   const int R = std::max(1, (n + nb - 1) / nb);
-  const bool hubs = variant == SNC_VARIANT_BUCKET && g.n_heavy > 0;
+  const bool hubs = variant == SNC_VARIANT_BUCKET && n_heavy > 0;
   std::vector<int32_t> hub_index(hubs ? n : 0, -1);
-  for (int k = 0; hubs && k < g.n_heavy; ++k) hub_index[g.heavy_host[k]] = k;
+  for (int k = 0; hubs && k < n_heavy; ++k) hub_index[P->heavy_host[k]] = k;
   std::vector<int32_t> blk_ptr((size_t)num_steps * (R + 1), 0), bnid;
-  std::vector<float> bamp, hinj(hubs ? (size_t)num_steps * g.n_heavy : 1, 0.0f);
+  std::vector<float> bamp, hinj(hubs ? (size_t)num_steps * n_heavy : 1, 0.0f);
   for (int t = 0; t < num_steps; ++t) {
     int32_t* bp = &blk_ptr[(size_t)t * (R + 1)];
     int blk = 0;
@@ -363,7 +385,7 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
     for (int k = ip[t]; k < ip[t + 1]; ++k) {  // entries sorted by neuron
       const int j = inid[k];
       if (hubs && hub_index[j] >= 0) {
-        hinj[(size_t)t * g.n_heavy + hub_index[j]] = iamp[k];
+        hinj[(size_t)t * n_heavy + hub_index[j]] = iamp[k];
         continue;
       }
       while (blk < j / nb) bp[++blk] = (int32_t)bnid.size();
@@ -376,11 +398,14 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   // One persistent launch per run when the device and graph allow it (and it
   // is asked for); its ext is double-buffered by step parity. The bucket
   // bitmap and dirty flags are double-buffered in every mode.
-  P->persistent = persist_choose(g, variant, P->pcfg);
+  P->persistent = persist_choose(g, variant, cfg->persistent != 0, P->pcfg);
+  P->cfg.persistent = P->persistent ? 1 : 0;
   const int halves = P->persistent ? 2 : 1;
   const int bucket_halves = 2;
   const size_t dirty_stride = ((size_t)n + 3) / 4 * 4;
   int rc = SNC_OK;
+  if (!rc) rc = dupload(&P->heavy_ids, P->heavy_host.data(), P->heavy_host.size());
+  if (!rc) rc = dupload(&P->is_heavy, is_heavy_h.data(), is_heavy_h.size());
   if (!rc) rc = dupload(&P->inj_blk_ptr, blk_ptr.data(), blk_ptr.size());
   if (!rc) rc = dupload(&P->blk_nid, bnid.data(), bnid.size());
   if (!rc) rc = dupload(&P->blk_amp, bamp.data(), bamp.size());
@@ -400,15 +425,15 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   if (!rc && variant == SNC_VARIANT_BUCKET) rc = dalloc(&P->bits, P->n_bits_words * bucket_halves);
   P->core.lif_grid = std::max(1, (n + 255) / 256);
   P->core.expand_grid = std::max(1, std::min(2048, (n + 255) / 256));
-  P->core.n_heavy = variant == SNC_VARIANT_BUCKET ? g.n_heavy : 0;
-  P->n_blk_slots = (size_t)std::max(4096, R + g.n_heavy + 1);
+  P->core.n_heavy = variant == SNC_VARIANT_BUCKET ? n_heavy : 0;
+  P->n_blk_slots = (size_t)std::max(4096, R + n_heavy + 1);
   if (!rc) rc = dalloc(&P->blk_spikes, P->n_blk_slots);
   if (!rc) rc = dalloc(&P->blk_events, P->n_blk_slots);
   if (variant == SNC_VARIANT_SORT && (uint64_t)n * (uint64_t)g.L >= (1ull << 32))
     return fail(SNC_E_UNSUPPORTED, "hip-sort keys need n * ring_len < 2^32");
   if (!rc && variant == SNC_VARIANT_SORT) rc = sort_alloc(P.get());
   if (!rc && P->persistent) {
-    const std::vector<int32_t> hp = persist_heavy_ptr(g, g.heavy_host, P->pcfg);
+    const std::vector<int32_t> hp = persist_heavy_ptr(g, P->heavy_host, P->pcfg);
     rc = dupload(&P->heavy_blk_ptr, hp.data(), hp.size());
     P->pcfg.heavy_blk_ptr = P->heavy_blk_ptr;
     P->pcfg.bits_words = (int64_t)P->n_bits_words;
@@ -440,7 +465,7 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   a.ccol = g.ccol;
   a.cval = g.cval;
   a.cdel = g.cdel;
-  a.heavy_ids = g.heavy_ids;
+  a.heavy_ids = P->heavy_ids;
   a.inj_ptr = P->inj_ptr;
   a.inj_nid = P->inj_nid;
   a.inj_amp = P->inj_amp;
@@ -467,14 +492,48 @@ int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_eve
   a.blk_nid = P->blk_nid;
   a.blk_amp = P->blk_amp;
   a.inj_heavy = P->inj_heavy;
-  a.is_heavy = g.is_heavy;
+  a.is_heavy = P->is_heavy;
   a.n_reg_blocks = R;
-  a.n_heavy = hubs ? g.n_heavy : 0;
+  a.blk = nb;
+  a.n_heavy = hubs ? n_heavy : 0;
   a.bits_half = (int64_t)P->n_bits_words;
   a.dirty_half = (int64_t)dirty_stride;
 
   *out = P.release();
   return SNC_OK;
+}
+
+int snc_hip_plan_get_config(const snc_hip_plan* p, snc_plan_config* out) {
+  if (!p || !out) return fail(SNC_E_INVALID, "null argument");
+  *out = p->cfg;
+  return SNC_OK;
+}
+
+int snc_hip_graph_tune_report(const snc_hip_graph* g, int32_t exact_only, snc_tune_report* out) {
+  if (!g || !out) return fail(SNC_E_INVALID, "null argument");
+  const int k = exact_only ? 1 : 0;
+  if (!g->tuned[k]) return fail(SNC_E_INVALID, "no autotuner search has run for this graph");
+  *out = g->tune[k];
+  return SNC_OK;
+}
+
+int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_events,
+                        int32_t num_steps, const snc_lif* lif, int32_t variant,
+                        snc_hip_plan** out) {
+  if (!G || !lif || !out || (n_events > 0 && !events) || num_steps < 0)
+    return fail(SNC_E_INVALID, "bad plan arguments");
+  snc_plan_config cfg;
+  if (variant == SNC_VARIANT_AUTO || variant == SNC_VARIANT_AUTO_EXACT) {
+    const int rc = autotune_search(G, events, n_events, num_steps, lif,
+                                   variant == SNC_VARIANT_AUTO_EXACT, &cfg);
+    if (rc) return rc;
+  } else if (variant == SNC_VARIANT_ATOMIC || variant == SNC_VARIANT_BUCKET ||
+             variant == SNC_VARIANT_SORT) {
+    cfg = default_plan_config(variant);
+  } else {
+    return fail(SNC_E_INVALID, "unknown variant");
+  }
+  return snc_hip_plan_create_config(G, events, n_events, num_steps, lif, &cfg, out);
 }
 
 int32_t snc_hip_plan_variant(const snc_hip_plan* p) { return p ? p->variant : -1; }
@@ -728,3 +787,48 @@ int snc_hip_forward(const snc_graph_view* gv, const snc_event* events, int64_t n
 }
 
 }  // extern "C"
+
+// The autotuner's search (cached per graph): every candidate runs the first
+// trial_steps steps of this sample twice (the first run captures and
+// instantiates its graph); the device time of the second run decides.
+static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_events,
+                           int num_steps, const snc_lif* lif, bool exact_only,
+                           snc_plan_config* out) {
+  const int k = exact_only ? 1 : 0;
+  if (G->tuned[k]) {
+    *out = G->tune[k].cand[G->tune[k].chosen];
+    return SNC_OK;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  const std::vector<snc_plan_config> cands =
+      autotune_candidates(G->g, exact_only, persist_supported());
+  snc_tune_report rep{};
+  rep.trial_steps = autotune_trial_steps(num_steps);
+  int best = -1;
+  double best_ms = 0.0;
+  for (const snc_plan_config& c : cands) {
+    snc_hip_plan* P = nullptr;
+    int rc = snc_hip_plan_create_config(G, events, n_events, rep.trial_steps, lif, &c, &P);
+    snc_forward_stats st{};
+    if (!rc) rc = snc_hip_plan_run(P, nullptr, &st);
+    if (!rc) rc = snc_hip_plan_run(P, nullptr, &st);
+    snc_hip_plan_free(P);
+    const double ms = rc ? -1.0 : st.ms / rep.trial_steps;  // -1: could not run here
+    if (rep.n_candidates >= SNC_TUNE_MAX_CANDIDATES) break;
+    rep.cand[rep.n_candidates] = c;
+    rep.ms_per_step[rep.n_candidates] = ms;
+    if (ms >= 0.0 && (best < 0 || ms < best_ms)) {
+      best = rep.n_candidates;
+      best_ms = ms;
+    }
+    ++rep.n_candidates;
+  }
+  if (best < 0) return fail(SNC_E_HIP, "autotuner: no candidate configuration ran");
+  rep.chosen = best;
+  rep.search_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  G->tune[k] = rep;
+  G->tuned[k] = true;
+  *out = rep.cand[best];
+  return SNC_OK;
+}

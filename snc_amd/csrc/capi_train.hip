@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <random>
@@ -55,6 +57,7 @@ struct snc_hip_train {
   int n = 0, S = 0, L = 2, classes = 1, n_internal = 0, batch = 1, max_steps = 1;
   int64_t P = 0;
   bool f64 = true, ring_lds = true;
+  size_t lds_max = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   TrainArgs a{};
@@ -74,7 +77,7 @@ struct snc_hip_train {
   int64_t max_inj = 0;
   // batch buffers
   int32_t* idx = nullptr;
-  void* E = nullptr;
+  float* hist = nullptr;  // [batch][max_steps][2][hn] psi / trace histories
   float *C = nullptr, *ring_g = nullptr, *logits = nullptr;
   double* loss = nullptr;
   int32_t* correct = nullptr;
@@ -89,8 +92,17 @@ struct snc_hip_train {
   uint32_t *p_fw = nullptr, *p_bits = nullptr;
   void* p_E = nullptr;
   unsigned long long *p_spikes = nullptr, *p_events = nullptr;
+  int32_t* p_cur = nullptr;
+  // captured plain-path chains, keyed by (sample length, update on/off)
+  std::map<std::pair<int, int>, hipGraphExec_t> plain_graphs;
+
+  void drop_graphs() {
+    for (auto& kv : plain_graphs) (void)hipGraphExecDestroy(kv.second);
+    plain_graphs.clear();
+  }
 
   ~snc_hip_train() {
+    drop_graphs();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -98,12 +110,12 @@ struct snc_hip_train {
                     (void*)csc_e, (void*)channel, (void*)internal_row, (void*)inputs, (void*)cdel,
                     (void*)role, (void*)w, (void*)wc, (void*)Bfb, (void*)X, (void*)labels,
                     (void*)lengths, (void*)ev_base, (void*)ev_step_ptr, (void*)ev_nid,
-                    (void*)ev_amp, (void*)idx, E, (void*)C, (void*)ring_g, (void*)logits,
+                    (void*)ev_amp, (void*)idx, (void*)hist, (void*)C, (void*)ring_g, (void*)logits,
                     (void*)loss, (void*)correct, (void*)blk_spikes, (void*)blk_events,
                     (void*)neur_fire, (void*)raster, (void*)p_v, (void*)p_tr, (void*)p_psi,
                     (void*)p_ext, (void*)p_ring, (void*)p_logit, (void*)p_L, (void*)p_refr,
                     (void*)p_fired, (void*)p_nf, (void*)p_fw, (void*)p_bits, p_E,
-                    (void*)p_spikes, (void*)p_events})
+                    (void*)p_spikes, (void*)p_events, (void*)p_cur})
       if (p) (void)hipFree(p);
   }
 };
@@ -217,8 +229,8 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   if (!rc) rc = talloc(&s->wc, (size_t)P);
   if (!rc) rc = tupload(&s->Bfb, Bfb.data(), Bfb.size());
   const size_t esz = s->f64 ? sizeof(double) : sizeof(float);
-  if (!rc && hipMalloc(&s->E, esz * (size_t)batch * std::max(1, S)) != hipSuccess)
-    rc = fail(SNC_E_NOMEM, "eligibility allocation failed");
+  const int hn = (n + 3) / 4 * 4;
+  if (!rc) rc = talloc(&s->hist, (size_t)batch * max_steps * 2 * hn);
   if (!rc) rc = talloc(&s->C, (size_t)batch * S);
   if (!rc) rc = talloc(&s->idx, (size_t)batch);
   if (!rc) rc = talloc(&s->logits, (size_t)batch * s->classes);
@@ -244,6 +256,7 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
     rc = fail(SNC_E_NOMEM, "eligibility allocation failed");
   if (!rc) rc = talloc(&s->p_spikes, 1);
   if (!rc) rc = talloc(&s->p_events, 1);
+  if (!rc) rc = talloc(&s->p_cur, 4);
   if (rc) return rc;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
@@ -288,7 +301,8 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   a.Bfb = s->Bfb;
   a.inputs = s->inputs;
   a.idx = s->idx;
-  a.E = s->E;
+  a.hist = s->hist;
+  a.hn = hn;
   a.logits = s->logits;
   a.loss = s->loss;
   a.correct = s->correct;
@@ -299,9 +313,12 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   hipDeviceProp_t prop;
   (void)hipGetDeviceProperties(&prop, 0);
   const size_t lds_max = prop.sharedMemPerBlock;
-  if (train_lds_bytes(a, false) > lds_max)
+  s->lds_max = lds_max;
+  // Leave room for the kernel's few bytes of static LDS.
+  const size_t lds_budget = lds_max > 256 ? lds_max - 256 : 0;
+  if (train_lds_bytes(a, false) + sizeof(float) * 2 * (size_t)hn > lds_budget)
     return fail(SNC_E_UNSUPPORTED, "per-sample state does not fit in LDS");
-  s->ring_lds = train_lds_bytes(a, true) <= lds_max;
+  s->ring_lds = train_lds_bytes(a, true) <= lds_budget;
   if (!s->ring_lds) {
     rc = talloc(&s->ring_g, (size_t)batch * s->L * n);
     if (rc) return rc;
@@ -309,7 +326,8 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   a.ring_g = s->ring_g;
 
   s->ps = {s->p_v, s->p_tr, s->p_psi, s->p_ext, s->p_ring, s->p_logit, s->p_L, s->p_refr,
-           s->p_fired, s->p_nf, s->p_fw, s->p_bits, s->p_E, s->p_spikes, s->p_events};
+           s->p_fired, s->p_nf, s->p_cur, s->p_fw, s->p_bits, s->p_E, s->p_spikes,
+           s->p_events};
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(SNC_E_HIP, "init failed");
   *out = s.release();
   return SNC_OK;
@@ -356,6 +374,7 @@ int snc_hip_train_upload_features(snc_hip_train* s, const float* X, int32_t n, i
   if (rc) return rc;
   s->n_data = n;
   s->dim = dim;
+  s->drop_graphs();  // captured chains hold the old dataset pointers
   s->h_labels.assign(labels, labels + n);
   s->h_lengths = len;
   s->a.X = s->X;
@@ -414,6 +433,7 @@ int snc_hip_train_upload_events(snc_hip_train* s, const snc_event* ev, int64_t n
     step_ptr[q + 1] = (int32_t)nid.size();
     mx = std::max<int64_t>(mx, step_ptr[q + 1] - step_ptr[q]);
   }
+  s->drop_graphs();  // captured chains hold the old dataset pointers
   tfree(s->X);
   tfree(s->labels);
   tfree(s->lengths);
@@ -474,24 +494,58 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
   }
   (void)hipEventRecord(s->ev0, sm);
   if (mode == SNC_TRAIN_BATCHED) {
-    rc = train_launch_batched(a, B, s->f64, s->ring_lds, sm);
+    rc = train_launch_batched(a, B, s->f64, s->ring_lds, s->lds_max, sm);
   } else if (mode == SNC_TRAIN_PLAIN) {
     const size_t esz = s->f64 ? sizeof(double) : sizeof(float);
     rc = fill0(sm, s->p_spikes, 8);
     if (!rc) rc = fill0(sm, s->p_events, 8);
+    // One sample's chain: clear its state (fill kernels), then T steps of
+    // six kernels and the end-of-sample kernels.
+    auto chain = [&](int T) {
+      int r = fill0(sm, s->p_v, 4 * (size_t)s->n);
+      if (!r) r = fill0(sm, s->p_tr, 4 * (size_t)s->n);
+      if (!r) r = fill0(sm, s->p_ext, 4 * (size_t)s->n);
+      if (!r) r = fill0(sm, s->p_refr, 4 * (size_t)s->n);
+      if (!r) r = fill0(sm, s->p_ring, 4 * (size_t)s->L * s->n);
+      if (!r) r = fill0(sm, s->p_logit, 4 * (size_t)s->classes);
+      if (!r) r = fill0(sm, s->p_bits, 4 * (size_t)(s->P / 32));
+      if (!r) r = fill0(sm, s->p_nf, 4);
+      if (!r) r = fill0(sm, s->p_E, esz * (size_t)s->S);
+      if (!r) r = train_launch_plain_sample(a, s->ps, T, s->f64, sm);
+      return r;
+    };
+    // The chain is captured once per (length, update) into a hipGraph and
+    // replayed; rasters (a per-call buffer) and SNC_HIP_GRAPH=0 run eagerly.
+    const char* ge = std::getenv("SNC_HIP_GRAPH");
+    const bool use_graph = !a.raster && !(ge && ge[0] == '0');
     for (int b = 0; b < B && !rc; ++b) {
       const int sample = idx[b];
       const int T = s->h_lengths[sample];
-      rc = fill0(sm, s->p_v, 4 * (size_t)s->n);
-      if (!rc) rc = fill0(sm, s->p_tr, 4 * (size_t)s->n);
-      if (!rc) rc = fill0(sm, s->p_ext, 4 * (size_t)s->n);
-      if (!rc) rc = fill0(sm, s->p_refr, 4 * (size_t)s->n);
-      if (!rc) rc = fill0(sm, s->p_ring, 4 * (size_t)s->L * s->n);
-      if (!rc) rc = fill0(sm, s->p_logit, 4 * (size_t)s->classes);
-      if (!rc) rc = fill0(sm, s->p_bits, 4 * (size_t)(s->P / 32));
-      if (!rc) rc = fill0(sm, s->p_nf, 4);
-      if (!rc) rc = fill0(sm, s->p_E, esz * (size_t)s->S);
-      if (!rc) rc = train_launch_plain_sample(a, s->ps, b, sample, T, s->f64, sm);
+      rc = train_launch_plain_begin(s->ps, b, sample, epoch, sm);
+      if (rc) break;
+      if (!use_graph) {
+        rc = chain(T);
+        continue;
+      }
+      const auto key = std::make_pair(T, apply ? 1 : 0);
+      auto it = s->plain_graphs.find(key);
+      if (it == s->plain_graphs.end()) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal) != hipSuccess)
+          return fail(SNC_E_HIP, "hipStreamBeginCapture failed");
+        const int crc = chain(T);
+        const hipError_t e = hipStreamEndCapture(sm, &graph);
+        if (crc || e != hipSuccess || !graph) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return crc ? crc : fail(SNC_E_HIP, "hipStreamEndCapture failed");
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail(SNC_E_HIP, "hipGraphInstantiate failed");
+        it = s->plain_graphs.emplace(key, exec).first;
+      }
+      if (hipGraphLaunch(it->second, sm) != hipSuccess) rc = fail(SNC_E_HIP, "hipGraphLaunch failed");
     }
   } else {
     return fail(SNC_E_INVALID, "unknown training mode");

@@ -64,6 +64,7 @@ struct StepArgs {
   const float* inj_heavy;      // [T][n_heavy] injections of heavy targets (hip-bucket)
   const uint8_t* is_heavy;     // [n]
   int n_reg_blocks;
+  int blk;                     // neurons per block of the fused step kernel
   int n_heavy;
   int64_t bits_half;           // words of one bitmap half (step parity)
   int64_t dirty_half;          // bytes of one dirty-flag half
@@ -89,7 +90,8 @@ int fwd_launch_first_injection(const FwdPlanCore& pc, const StepArgs& a);
 int fused_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_t hub_stream);
 int fused_launch_hub(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_t st);
 int fused_launch_regular(const FwdPlanCore& pc, const StepArgs& a, int t);
-int fused_block_neurons();
+bool fused_block_ok(int block);
+int fused_light_max_in_degree();  // largest in-degree the step kernel folds per lane
 
 // hip-sort (sort.hip)
 size_t sort_hist_words(int n);

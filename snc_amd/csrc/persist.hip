@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "common.hpp"
@@ -172,12 +173,21 @@ __global__ __launch_bounds__(kPersistThreads) void k_persist(StepArgs a, Persist
   if (tid == 0) a.blk_events[blockIdx.x] += s_ev;
 }
 
-bool persist_choose(const DevGraph& g, int variant, PersistConfig& cfg) {
+bool persist_supported() {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return false;
+  return prop.cooperativeLaunch != 0;
+}
+
+bool persist_choose(const DevGraph& g, int variant, bool want, PersistConfig& cfg) {
   if (variant != SNC_VARIANT_ATOMIC && variant != SNC_VARIANT_BUCKET) return false;
-  // Opt-in: on MI355X the graph-launched fused step kernels measured faster
-  // than one cooperative launch with a grid barrier per step.
+  // Opt-in (the autotuner's candidate, or SNC_HIP_PERSIST=1): on MI355X the
+  // graph-launched fused step kernels measured faster than one cooperative
+  // launch with a grid barrier per step.
   const char* e = std::getenv("SNC_HIP_PERSIST");
-  if (!(e && e[0] == '1')) return false;
+  if (!want && !(e && e[0] == '1')) return false;
   int dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)

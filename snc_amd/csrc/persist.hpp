@@ -25,8 +25,11 @@ struct PersistArgs {
   int64_t bits_words, dirty_stride;
 };
 
-// Decide whether a plan runs as one persistent launch; fills the launch shape.
-bool persist_choose(const DevGraph& g, int variant, PersistConfig& cfg);
+// Whether the device can run the persistent kernel at all.
+bool persist_supported();
+// Decide whether a plan runs as one persistent launch (when `want`, or when
+// SNC_HIP_PERSIST=1 asks for it); fills the launch shape.
+bool persist_choose(const DevGraph& g, int variant, bool want, PersistConfig& cfg);
 std::vector<int32_t> persist_heavy_ptr(const DevGraph& g, const std::vector<int32_t>& heavy_sorted,
                                        const PersistConfig& cfg);
 int persist_launch(const DevGraph& g, const FwdPlanCore& pc, const StepArgs& a,

@@ -1,7 +1,8 @@
 // Fused per-step kernels of hip-atomic and hip-bucket: one launch per
 // timestep does fold + integrate/fire + expansion for every neuron.
 //
-// A block owns 256 consecutive neurons. Within step t it
+// A block owns BLOCK consecutive neurons (128, 256 or 512: a launch-shape
+// choice of the autotuner), one per thread. Within step t it
 //   1. stages its own injections of step t in LDS (per-(step, block) ranges),
 //   2. [hip-bucket] folds the deposits made at step t-1 into its targets:
 //      the target's column words are loaded together, set bits are walked in
@@ -59,15 +60,15 @@ __device__ __forceinline__ int expand_row(const StepArgs& a, int i, int tm, uint
   return re - rb;
 }
 
-template <bool BUCKET>
-__global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a, int t, int tm) {
-  __shared__ float s_ext[kStepThreads];
-  __shared__ int32_t s_fired[kStepThreads];
+template <bool BUCKET, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_step(StepArgs a, int t, int tm) {
+  __shared__ float s_ext[BLOCK];
+  __shared__ int32_t s_fired[BLOCK];
   __shared__ int s_nf;
   __shared__ unsigned long long s_ev;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const int b = blockIdx.x;
-  const int base = b * kStepThreads;
+  const int base = b * BLOCK;
   const int j = base + tid;
   const int n = a.n, L = a.L;
   s_ext[tid] = 0.0f;
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a, int t, int tm
   }
   __syncthreads();
   const int32_t* bp = a.inj_blk_ptr + (size_t)t * (a.n_reg_blocks + 1) + b;
-  for (int k = bp[0] + tid; k < bp[1]; k += kStepThreads) s_ext[a.blk_nid[k] - base] = a.blk_amp[k];
+  for (int k = bp[0] + tid; k < bp[1]; k += BLOCK) s_ext[a.blk_nid[k] - base] = a.blk_amp[k];
   __syncthreads();
 
   bool f = false;
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a, int t, int tm
   uint32_t* bits_now = BUCKET ? a.bits + (size_t)(t & 1) * a.bits_half : nullptr;
   uint8_t* dirty_now = BUCKET ? a.dirty + (size_t)(t & 1) * a.dirty_half : nullptr;
   unsigned long long ev = 0;
-  for (int k = wave; k < nf; k += kStepThreads / kWave)
+  for (int k = wave; k < nf; k += BLOCK / kWave)
     ev += (unsigned long long)expand_row<BUCKET>(a, s_fired[k], tm, bits_now, dirty_now, lane);
   if (lane == 0 && ev) atomicAdd(&s_ev, ev);
   __syncthreads();
@@ -262,14 +263,24 @@ int fused_launch_hub(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_
   return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "hub step launch failed");
 }
 
-int fused_launch_regular(const FwdPlanCore& pc, const StepArgs& a, int t) {
-  const int tm = t % a.L;
+template <int BLOCK>
+static void launch_step_block(const FwdPlanCore& pc, const StepArgs& a, int t, int tm) {
   if (pc.variant == SNC_VARIANT_BUCKET)
-    hipLaunchKernelGGL(k_step<true>, dim3(a.n_reg_blocks), dim3(kStepThreads), 0, pc.stream, a,
+    hipLaunchKernelGGL((k_step<true, BLOCK>), dim3(a.n_reg_blocks), dim3(BLOCK), 0, pc.stream, a,
                        t, tm);
   else
-    hipLaunchKernelGGL(k_step<false>, dim3(a.n_reg_blocks), dim3(kStepThreads), 0, pc.stream, a,
-                       t, tm);
+    hipLaunchKernelGGL((k_step<false, BLOCK>), dim3(a.n_reg_blocks), dim3(BLOCK), 0, pc.stream,
+                       a, t, tm);
+}
+
+int fused_launch_regular(const FwdPlanCore& pc, const StepArgs& a, int t) {
+  const int tm = t % a.L;
+  switch (a.blk) {
+    case 128: launch_step_block<128>(pc, a, t, tm); break;
+    case 256: launch_step_block<256>(pc, a, t, tm); break;
+    case 512: launch_step_block<512>(pc, a, t, tm); break;
+    default: return fail(SNC_E_INVALID, "step block must be 128, 256 or 512 neurons");
+  }
   return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "step launch failed");
 }
 
@@ -279,6 +290,8 @@ int fused_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream
   return rc;
 }
 
-int fused_block_neurons() { return kStepThreads; }
+bool fused_block_ok(int block) { return block == 128 || block == 256 || block == 512; }
+
+int fused_light_max_in_degree() { return kLightMaxWords * 32; }
 
 }  // namespace snc_amd

@@ -50,14 +50,15 @@ constexpr int kTrainThreads = 1024;
 // ---------------------------------------------------------------------------
 
 // Drive of input neuron j at step t of dataset sample `sample`.
-__device__ __forceinline__ float input_drive(const TrainArgs& a, int j, int sample, int t) {
+__device__ __forceinline__ float input_drive(const TrainArgs& a, int j, int sample, int t,
+                                             int epoch) {
   const int c = a.channel[j];
   if (c < 0 || c >= a.dim) return 0.0f;
   const float x = a.X[(size_t)sample * a.dim + c];
   if (!(x > a.enc_thr)) return 0.0f;
   if (a.encoder == SNC_ENC_DIRECT) return x * a.gain;
   const Philox4 r = philox4x32_10((uint32_t)(c >> 2), (uint32_t)t, (uint32_t)sample,
-                                  (uint32_t)a.epoch, a.seed_lo, a.seed_hi);
+                                  (uint32_t)epoch, a.seed_lo, a.seed_hi);
   const float p = fminf(1.0f, x * a.max_rate);
   return r.v[c & 3] < prob_to_u32((double)p) ? a.gain : 0.0f;
 }
@@ -103,11 +104,11 @@ __device__ __forceinline__ void elig_add(ET* E, size_t k, float psi, float tr);
 template <>
 __device__ __forceinline__ void elig_add<double>(double* E, size_t k, float psi, float tr) {
   // psi * tr is exact in double, so one rounding: the CPU's double sum.
-  E[k] = fma((double)psi, (double)tr, E[k]);
+  E[k] = __fma_rn((double)psi, (double)tr, E[k]);
 }
 template <>
 __device__ __forceinline__ void elig_add<float>(float* E, size_t k, float psi, float tr) {
-  E[k] = fmaf(psi, tr, E[k]);
+  E[k] = __fmaf_rn(psi, tr, E[k]);
 }
 
 // End of sample: delta, loss, prediction, learning signal into Ls[n].
@@ -149,19 +150,57 @@ __device__ void sample_epilogue(const TrainArgs& a, int b, const float* logits_i
   }
 }
 
-template <typename ET>
-__device__ __forceinline__ void write_contrib(const TrainArgs& a, int b, const ET* E,
-                                              const float* Ls, float scale, int tid, int nthreads,
-                                              size_t stride_base) {
-  for (int s = tid; s < a.S; s += nthreads) {
-    const float e = (float)E[stride_base + s];
-    a.C[(size_t)b * a.S + s] = __fmul_rn(__fmul_rn(scale, Ls[a.post[s]]), e);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // batched persistent kernel: blockIdx.x = position in the minibatch
 // ---------------------------------------------------------------------------
+
+constexpr int kEligPerThread = 16;  // synapses whose eligibility a lane keeps in registers
+
+// Eligibility and contributions of sample b after its T steps, from the psi
+// and trace histories the step loop wrote:
+//   E_s = sum_t psi_t[post_s] * tr_t[pre_s],   t ascending, one fma per step,
+// exactly the per-step accumulation of trainer.cpp:132-139 (the product of
+// two floats is exact in double, so fma == add of the exact product; steps
+// where tr_pre = 0 add +0 and change nothing). Lanes keep kEligPerThread
+// sums in registers; the histories stream through LDS in chunks of steps.
+template <typename ET>
+__device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, const float* Ls,
+                             float* s_chunk, int chunk_steps, int tid) {
+  const int hn = a.hn;
+  const float* hb = a.hist + (size_t)b * a.T_max * 2 * hn;
+  for (int s0 = 0; s0 < a.S; s0 += kEligPerThread * kTrainThreads) {
+    int pre[kEligPerThread], post[kEligPerThread];
+    ET E[kEligPerThread];
+#pragma unroll
+    for (int k = 0; k < kEligPerThread; ++k) {
+      const int sy = s0 + k * kTrainThreads + tid;
+      pre[k] = sy < a.S ? a.src[sy] : 0;
+      post[k] = sy < a.S ? a.post[sy] : 0;
+      E[k] = (ET)0;
+    }
+    for (int t0 = 0; t0 < T; t0 += chunk_steps) {
+      const int tc = min(chunk_steps, T - t0);
+      __syncthreads();  // previous chunk consumed
+      const float4* src4 = (const float4*)(hb + (size_t)t0 * 2 * hn);
+      float4* dst4 = (float4*)s_chunk;
+      const int quads = tc * 2 * hn / 4;  // hn is a multiple of 4
+      for (int q = tid; q < quads; q += kTrainThreads) dst4[q] = src4[q];
+      __syncthreads();
+      for (int t = 0; t < tc; ++t) {
+        const float* ps = s_chunk + (size_t)t * 2 * hn;
+        const float* tr = ps + hn;
+#pragma unroll
+        for (int k = 0; k < kEligPerThread; ++k) elig_add<ET>(E, k, ps[post[k]], tr[pre[k]]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kEligPerThread; ++k) {
+      const int sy = s0 + k * kTrainThreads + tid;
+      if (sy < a.S && a.C)
+        a.C[(size_t)b * a.S + sy] = __fmul_rn(__fmul_rn(scale, Ls[post[k]]), (float)E[k]);
+    }
+  }
+}
 
 template <typename ET, bool RING_LDS>
 __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
@@ -169,15 +208,15 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   const int n = a.n, L = a.L, tid = threadIdx.x;
   float* s_v = (float*)smem;
   float* s_tr = s_v + n;
-  float* s_psi = s_tr + n;
-  float* s_ext = s_psi + n;
+  float* s_ext = s_tr + n;
   int* s_refr = (int*)(s_ext + n);
   int* s_fired = s_refr + n;                      // fired list
   uint32_t* s_fw = (uint32_t*)(s_fired + n);      // fired bitmap [words]
   uint32_t* s_bits = s_fw + a.words;              // delivery buckets [bits_words]
   float* s_logit = (float*)(s_bits + a.bits_words);  // [classes]
   float* s_delta = s_logit + a.classes;              // [classes]
-  float* s_ring = s_delta + a.classes;               // [L][n] when RING_LDS
+  const size_t state_bytes = train_state_bytes(n, a.words, a.bits_words, a.classes);
+  float* s_ring = (float*)(smem + state_bytes);      // [L][n] when RING_LDS
   __shared__ int s_nf;
   __shared__ unsigned long long s_ev, s_sp;
 
@@ -186,7 +225,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   const int T = a.lengths[sample];
   const int label = a.labels[sample];
   float* ring = RING_LDS ? s_ring : a.ring_g + (size_t)b * L * n;
-  ET* E = (ET*)a.E + (size_t)b * a.S;
+  float* hist = a.hist + (size_t)b * a.T_max * 2 * a.hn;
+  const int ev0 = a.ev_base ? a.ev_base[sample] : 0;
 
   for (int j = tid; j < n; j += kTrainThreads) {
     s_v[j] = 0.0f;
@@ -197,7 +237,6 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   for (int k = tid; k < a.bits_words; k += kTrainThreads) s_bits[k] = 0u;
   for (int k = tid; k < a.classes; k += kTrainThreads) s_logit[k] = 0.0f;
   for (size_t k = tid; k < (size_t)L * n; k += kTrainThreads) ring[k] = 0.0f;
-  for (int s = tid; s < a.S; s += kTrainThreads) E[s] = (ET)0;
   if (tid == 0) {
     s_nf = 0;
     s_ev = 0;
@@ -205,10 +244,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   }
   __syncthreads();
 
-  const int ev0 = a.ev_base ? a.ev_base[sample] : 0;
-  int tm = 0;
-  for (int t = 0; t < T; ++t) {
-    // inject
+  // injections of step t into s_ext (entries are pre-folded per neuron)
+  auto inject = [&](int t) {
     if (a.ev_base) {
       for (int k = a.ev_step_ptr[ev0 + t] + tid; k < a.ev_step_ptr[ev0 + t + 1];
            k += kTrainThreads)
@@ -216,13 +253,19 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     } else {
       for (int q = tid; q < a.n_inputs; q += kTrainThreads) {
         const int j = a.inputs[q];
-        const float x = input_drive(a, j, sample, t);
+        const float x = input_drive(a, j, sample, t, a.epoch);
         if (x != 0.0f) s_ext[j] = x;
       }
     }
-    __syncthreads();
-    // LIF + psi, fired bitmap and list
-    const float* slot = ring + (size_t)tm * n;
+  };
+  if (T > 0) inject(0);
+  __syncthreads();
+
+  int tm = 0;
+  for (int t = 0; t < T; ++t) {
+    // (A) LIF + psi, trace, histories, fired bitmap and list
+    float* slot = ring + (size_t)tm * n;
+    float* hrow = hist + (size_t)t * 2 * a.hn;
     for (int j0 = 0; j0 < n; j0 += kTrainThreads) {
       const int j = j0 + tid;
       bool f = false;
@@ -233,9 +276,14 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
         f = train_lif(a, j, v, r, s_ext[j], slot[j], psi);
         s_v[j] = v;
         s_refr[j] = r;
-        s_psi[j] = psi;
         s_ext[j] = 0.0f;
-        ((float*)slot)[j] = 0.0f;
+        slot[j] = 0.0f;
+        const float tr = trace_step(a.lif.decay, s_tr[j], f);
+        s_tr[j] = tr;
+        if (a.C) {  // histories feed the update only
+          hrow[j] = psi;
+          hrow[a.hn + j] = tr;
+        }
         if (f) {
           if (a.neur_fire) atomicAdd(a.neur_fire + j, 1ull);
           if (!a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT && a.channel[j] >= 0)
@@ -258,20 +306,10 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
       }
     }
     __syncthreads();
+    // (B) raster row; fired rows set bucket bits
     if (a.raster)
       for (int w = tid; w < a.words; w += kTrainThreads)
         a.raster[((size_t)b * a.T_max + t) * a.words + w] = s_fw[w];
-    // trace
-    for (int j = tid; j < n; j += kTrainThreads)
-      s_tr[j] = trace_step(a.lif.decay, s_tr[j], (s_fw[j >> 5] >> (j & 31)) & 1u);
-    __syncthreads();
-    // eligibility over all synapses; skipped where the pre trace is zero,
-    // as the CPU skips those rows (trainer.cpp:135-136).
-    for (int s = tid; s < a.S; s += kTrainThreads) {
-      const float tp = s_tr[a.src[s]];
-      if (tp != 0.0f) elig_add<ET>(E, s, s_psi[a.post[s]], tp);
-    }
-    // delivery: fired rows set bucket bits
     const int nf = s_nf;
     const int wave = tid >> 6, lane = lane_id();
     unsigned long long ev = 0;
@@ -286,7 +324,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     }
     if (lane == 0 && ev) atomicAdd(&s_ev, ev);
     __syncthreads();
-    // ordered fold into the ring
+    // (C) ordered fold into the ring; next step's injections
     for (int j = tid; j < n; j += kTrainThreads) {
       const int w0 = a.ccol[j] >> 5, w1 = a.ccol[j + 1] >> 5;
       int cur = -1;
@@ -310,6 +348,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
       }
       if (cur >= 0) ring[(size_t)cur * n + j] = acc;
     }
+    if (t + 1 < T) inject(t + 1);
     if (tid == 0) {
       s_sp += (unsigned long long)nf;
       s_nf = 0;
@@ -321,11 +360,15 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   if (a.readout_leaky && tid < a.classes)
     s_logit[tid] = __fdiv_rn(s_logit[tid], (float)max(1, T));
   __syncthreads();
-  sample_epilogue(a, b, s_logit, s_psi, s_delta, label, n, tid, kTrainThreads);
-  __syncthreads();
+  // The state arrays are dead from here: the learning signal takes s_v's
+  // place and the rest of LDS holds history chunks.
+  float* Ls = s_v;
+  sample_epilogue(a, b, s_logit, Ls, s_delta, label, n, tid, kTrainThreads);
   if (a.C) {
     const float scale = __fdiv_rn(a.lr, (float)max(1, T));
-    write_contrib<ET>(a, b, E, s_psi, scale, tid, kTrainThreads, 0);
+    const int chunk_steps =
+        (int)(((size_t)a.lds_bytes - state_bytes) / (sizeof(float) * 2 * (size_t)a.hn));
+    elig_contrib<ET>(a, b, T, scale, Ls, s_ring, max(1, chunk_steps), tid);
   }
   if (tid == 0) {
     a.blk_spikes[b] = s_sp;
@@ -337,15 +380,25 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
 // plain path: grid-wide kernels, one sample at a time
 // ---------------------------------------------------------------------------
 
-__global__ void k_p_inject(TrainArgs a, PlainState ps, int sample, int t) {
+// The plain kernels read the sample being run from ps.cur = {position b in
+// the minibatch, dataset index, epoch}, so one captured chain per sample
+// length serves every sample.
+__global__ void k_p_begin(PlainState ps, int b, int sample, int epoch) {
+  ps.cur[0] = b;
+  ps.cur[1] = sample;
+  ps.cur[2] = epoch;
+}
+
+__global__ void k_p_inject(TrainArgs a, PlainState ps, int t) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sample = ps.cur[1];
   if (a.ev_base) {
     const int e0 = a.ev_base[sample];
     const int k = a.ev_step_ptr[e0 + t] + q;
     if (k < a.ev_step_ptr[e0 + t + 1]) ps.ext[a.ev_nid[k]] = a.ev_amp[k];
   } else if (q < a.n_inputs) {
     const int j = a.inputs[q];
-    const float x = input_drive(a, j, sample, t);
+    const float x = input_drive(a, j, sample, t, ps.cur[2]);
     if (x != 0.0f) ps.ext[j] = x;
   }
 }
@@ -386,10 +439,11 @@ __global__ void k_p_lif(TrainArgs a, PlainState ps, int t, int tm) {
   }
 }
 
-__global__ void k_p_trace(TrainArgs a, PlainState ps, int b, int t) {
+__global__ void k_p_trace(TrainArgs a, PlainState ps, int t) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < a.n) ps.tr[j] = trace_step(a.lif.decay, ps.tr[j], (ps.fw[j >> 5] >> (j & 31)) & 1u);
-  if (a.raster && j < a.words) a.raster[((size_t)b * a.T_max + t) * a.words + j] = ps.fw[j];
+  if (a.raster && j < a.words)
+    a.raster[((size_t)ps.cur[0] * a.T_max + t) * a.words + j] = ps.fw[j];
 }
 
 template <typename ET>
@@ -427,24 +481,24 @@ __global__ void k_p_fold(TrainArgs a, PlainState ps, int tm) {
   fold_light(j, a.n, a.L, tm, a.ccol, a.wc, a.cdel, ps.bits, ps.ring);
 }
 
-__global__ __launch_bounds__(kTrainThreads) void k_p_finish(TrainArgs a, PlainState ps, int b,
-                                                            int T) {
+__global__ __launch_bounds__(kTrainThreads) void k_p_finish(TrainArgs a, PlainState ps, int T) {
   __shared__ float s_delta[64];
   __shared__ float s_logit[64];
   const int tid = threadIdx.x;
+  const int b = ps.cur[0];
   if (tid < a.classes)
     s_logit[tid] = a.readout_leaky ? __fdiv_rn(ps.logit[tid], (float)max(1, T)) : ps.logit[tid];
   __syncthreads();
-  sample_epilogue(a, b, s_logit, ps.L, s_delta, a.labels[a.idx[b]], a.n, tid, kTrainThreads);
+  sample_epilogue(a, b, s_logit, ps.L, s_delta, a.labels[ps.cur[1]], a.n, tid, kTrainThreads);
 }
 
 template <typename ET>
-__global__ void k_p_contrib(TrainArgs a, PlainState ps, int b, int T) {
+__global__ void k_p_contrib(TrainArgs a, PlainState ps, int T) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.S) return;
   const float scale = __fdiv_rn(a.lr, (float)max(1, T));
   const float e = (float)((const ET*)ps.E)[s];
-  a.C[(size_t)b * a.S + s] = __fmul_rn(__fmul_rn(scale, ps.L[a.post[s]]), e);
+  a.C[(size_t)ps.cur[0] * a.S + s] = __fmul_rn(__fmul_rn(scale, ps.L[a.post[s]]), e);
 }
 
 // ---------------------------------------------------------------------------
@@ -477,38 +531,57 @@ __global__ void k_gather_wc(const float* __restrict__ w, const int32_t* __restri
 static inline int blocks_for(int64_t n, int t) { return (int)std::max<int64_t>(1, (n + t - 1) / t); }
 
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds) {
-  size_t b = sizeof(float) * 4 * (size_t)a.n + sizeof(int) * 2 * (size_t)a.n +
-             sizeof(uint32_t) * ((size_t)a.words + a.bits_words) + sizeof(float) * 2 * a.classes;
+  size_t b = train_state_bytes(a.n, a.words, a.bits_words, a.classes);
   if (ring_lds) b += sizeof(float) * (size_t)a.L * a.n;
-  return (b + 15) / 16 * 16;
+  return b;
 }
 
-int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, hipStream_t st) {
-  const size_t lds = train_lds_bytes(a, ring_lds);
+int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
+                         hipStream_t st) {
+  // State (+ ring), then room for history chunks of up to kChunkTarget steps
+  // in the update phase; never below one step.
+  constexpr size_t kChunkTarget = 32;
   const void* fn = f64 ? (ring_lds ? (const void*)k_train_batched<double, true>
                                    : (const void*)k_train_batched<double, false>)
                        : (ring_lds ? (const void*)k_train_batched<float, true>
                                    : (const void*)k_train_batched<float, false>);
+  // The kernel's own static LDS counts against the same per-block limit.
+  hipFuncAttributes attr{};
+  if (hipFuncGetAttributes(&attr, fn) != hipSuccess)
+    return fail(SNC_E_HIP, "hipFuncGetAttributes of the batched trainer failed");
+  if (attr.sharedSizeBytes >= lds_max) return fail(SNC_E_UNSUPPORTED, "no dynamic LDS left");
+  const size_t dyn_max = lds_max - attr.sharedSizeBytes;
+  const size_t state = train_state_bytes(a.n, a.words, a.bits_words, a.classes);
+  const size_t step_bytes = sizeof(float) * 2 * (size_t)a.hn;
+  size_t lds = std::max(train_lds_bytes(a, ring_lds), state + step_bytes);
+  lds = std::max(lds, std::min(dyn_max, state + kChunkTarget * step_bytes));
+  if (lds > dyn_max) return fail(SNC_E_UNSUPPORTED, "batched trainer state exceeds LDS");
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
     return fail(SNC_E_UNSUPPORTED, "LDS request of the batched trainer rejected");
   TrainArgs aa = a;
+  aa.lds_bytes = (int)lds;
   void* args[] = {(void*)&aa};
   if (hipLaunchKernel(fn, dim3(B), dim3(kTrainThreads), args, lds, st) != hipSuccess)
     return fail(SNC_E_HIP, "batched trainer launch failed");
   return SNC_OK;
 }
 
-int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int b, int sample, int T,
-                              bool f64, hipStream_t st) {
+int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, hipStream_t st) {
+  hipLaunchKernelGGL(k_p_begin, dim3(1), dim3(1), 0, st, ps, b, sample, epoch);
+  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "plain begin launch failed");
+}
+
+int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int T, bool f64,
+                              hipStream_t st) {
   const int n = a.n, L = a.L;
   const int gn = blocks_for(n, 256), gs = blocks_for(a.S, 256);
   const int gi = blocks_for(std::max<int64_t>(a.n_inputs, a.max_inj_per_step), 256);
   int tm = 0;
   for (int t = 0; t < T; ++t) {
-    hipLaunchKernelGGL(k_p_inject, dim3(gi), dim3(256), 0, st, a, ps, sample, t);
+    hipLaunchKernelGGL(k_p_inject, dim3(gi), dim3(256), 0, st, a, ps, t);
     hipLaunchKernelGGL(k_p_lif, dim3(gn), dim3(256), 0, st, a, ps, t, tm);
     hipLaunchKernelGGL(k_p_trace, dim3(std::max(gn, blocks_for(a.words, 256))), dim3(256), 0, st,
-                       a, ps, b, t);
+                       a, ps, t);
     if (f64)
       hipLaunchKernelGGL(k_p_elig<double>, dim3(gs), dim3(256), 0, st, a, ps);
     else
@@ -517,12 +590,12 @@ int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int b, i
     hipLaunchKernelGGL(k_p_fold, dim3(gn), dim3(256), 0, st, a, ps, tm);
     tm = tm + 1 == L ? 0 : tm + 1;
   }
-  hipLaunchKernelGGL(k_p_finish, dim3(1), dim3(kTrainThreads), 0, st, a, ps, b, T);
+  hipLaunchKernelGGL(k_p_finish, dim3(1), dim3(kTrainThreads), 0, st, a, ps, T);
   if (a.C) {
     if (f64)
-      hipLaunchKernelGGL(k_p_contrib<double>, dim3(gs), dim3(256), 0, st, a, ps, b, T);
+      hipLaunchKernelGGL(k_p_contrib<double>, dim3(gs), dim3(256), 0, st, a, ps, T);
     else
-      hipLaunchKernelGGL(k_p_contrib<float>, dim3(gs), dim3(256), 0, st, a, ps, b, T);
+      hipLaunchKernelGGL(k_p_contrib<float>, dim3(gs), dim3(256), 0, st, a, ps, T);
   }
   return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "plain trainer launch failed");
 }

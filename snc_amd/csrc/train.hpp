@@ -45,7 +45,9 @@ struct TrainArgs {
   const float* ev_amp;
   // batch
   const int32_t* idx;  // [B] dataset indices
-  void* E;             // [B][S] float or double
+  float* hist;         // [B][T_max][2][hn]: psi then trace of every neuron, per step
+  int hn;              // history row length: n rounded up to a multiple of 4
+  int lds_bytes;       // dynamic LDS of the batched launch
   float* C;            // [B][S] contributions, or null (no update)
   float* ring_g;       // [B][L][n] when the ring is not in LDS
   float* logits;       // [B][classes]
@@ -61,15 +63,30 @@ struct TrainArgs {
 struct PlainState {
   float *v, *tr, *psi, *ext, *ring, *logit, *L;
   int32_t *refr, *fired, *nf;
+  int32_t* cur;  // [3]: position in the minibatch, dataset index, epoch
   uint32_t *fw, *bits;
   void* E;  // [S]
   unsigned long long *spikes, *events;
 };
 
+// LDS bytes of the batched kernel's per-sample state (membrane, trace,
+// injection, refractory, fired list, fired bitmap, buckets, logits, delta),
+// rounded up to 16; the ring (when in LDS) and later the history chunks
+// start there.
+__host__ __device__ inline size_t train_state_bytes(int n, int words, int bits_words,
+                                                    int classes) {
+  const size_t b = sizeof(float) * 3 * (size_t)n + sizeof(int) * 2 * (size_t)n +
+                   sizeof(uint32_t) * ((size_t)words + bits_words) +
+                   sizeof(float) * 2 * (size_t)classes;
+  return (b + 15) / 16 * 16;
+}
+
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds);
-int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, hipStream_t st);
-int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int b, int sample, int T,
-                              bool f64, hipStream_t st);
+int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
+                         hipStream_t st);
+int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, hipStream_t st);
+int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int T, bool f64,
+                              hipStream_t st);
 int train_launch_apply(const TrainArgs& a, float* w, int B, const int32_t* csc_e, float* wc,
                        int64_t P, hipStream_t st);
 int train_launch_gather(const float* w, const int32_t* csc_e, float* wc, int64_t P,

@@ -8,6 +8,9 @@ falls back: a missing library or device raises.
 """
 from __future__ import annotations
 
+import hashlib
+from collections import OrderedDict
+
 import ctypes as C
 from dataclasses import dataclass, field
 from enum import IntEnum
@@ -30,6 +33,7 @@ class Backend(IntEnum):
     HipBucket = L.VARIANT_BUCKET
     HipSort = L.VARIANT_SORT
     HipAuto = L.VARIANT_AUTO
+    HipAutoExact = L.VARIANT_AUTO_EXACT
 
 
 _NAMES = {
@@ -37,6 +41,7 @@ _NAMES = {
     Backend.CudaBucket: "cuda-bucket", Backend.CudaSort: "cuda-sort",
     Backend.HipAtomic: "hip-atomic", Backend.HipBucket: "hip-bucket",
     Backend.HipSort: "hip-sort", Backend.HipAuto: "hip-auto",
+    Backend.HipAutoExact: "hip-auto-exact",
 }
 
 
@@ -52,7 +57,8 @@ def backend_name(b: Backend) -> str:
 
 
 def backend_is_hip(b: Backend) -> bool:
-    return Backend(b) in (Backend.HipAtomic, Backend.HipBucket, Backend.HipSort, Backend.HipAuto)
+    return Backend(b) in (Backend.HipAtomic, Backend.HipBucket, Backend.HipSort, Backend.HipAuto,
+                          Backend.HipAutoExact)
 
 
 @dataclass
@@ -100,6 +106,24 @@ def available() -> bool:
         return False
 
 
+@dataclass
+class PlanConfig:
+    """snc_plan_config: strategy and launch shape of a forward plan."""
+    variant: Backend = Backend.HipBucket
+    block_neurons: int = 256
+    heavy_threshold: int = 512
+    persistent: bool = False
+
+    def c(self):
+        return L.PlanConfig(int(self.variant), self.block_neurons, self.heavy_threshold,
+                            1 if self.persistent else 0)
+
+    @staticmethod
+    def from_c(c) -> "PlanConfig":
+        return PlanConfig(Backend(c.variant), c.block_neurons, c.heavy_threshold,
+                          bool(c.persistent))
+
+
 class HipGraph:
     """A graph uploaded once to HBM (snc_hip_graph_upload)."""
 
@@ -112,6 +136,15 @@ class HipGraph:
         p = L.GraphProfile()
         L.check(L.lib().snc_hip_graph_profile_get(self._h, C.byref(p)))
         return {k: getattr(p, k) for k, _ in p._fields_}
+
+    def tune_report(self, exact_only=False) -> dict:
+        """The autotuner's search for this graph (after a hip-auto plan):
+        candidates with their measured ms per step, and the chosen one."""
+        r = L.TuneReport()
+        L.check(L.lib().snc_hip_graph_tune_report(self._h, 1 if exact_only else 0, C.byref(r)))
+        cands = [(PlanConfig.from_c(r.cand[i]), r.ms_per_step[i]) for i in range(r.n_candidates)]
+        return {"candidates": cands, "chosen": r.chosen, "trial_steps": r.trial_steps,
+                "search_ms": r.search_ms}
 
     def close(self):
         if self._h:
@@ -128,8 +161,11 @@ class HipGraph:
 class ForwardPlan:
     """Graph + one sample's events + device state, reusable across runs."""
 
-    def __init__(self, hg: HipGraph, events, num_steps: int, lif: LIFParams, backend: Backend):
-        if not backend_is_hip(backend):
+    def __init__(self, hg: HipGraph, events, num_steps: int, lif: LIFParams,
+                 backend: Backend = Backend.HipAuto, config: PlanConfig | None = None):
+        """backend: a hip-* strategy with its default launch shape, or hip-auto /
+        hip-auto-exact (autotuned); config: an explicit PlanConfig instead."""
+        if config is None and not backend_is_hip(backend):
             raise ValueError(f"{backend_name(backend)} is not a hip backend")
         self.hg = hg
         self.num_steps = int(num_steps)
@@ -137,12 +173,24 @@ class ForwardPlan:
         self._events = ev
         self._h = C.c_void_p()
         lc = lif.c()
-        L.check(L.lib().snc_hip_plan_create(hg._h, p, ev.size, self.num_steps, C.byref(lc),
-                                            int(backend), C.byref(self._h)))
+        if config is not None:
+            cc = config.c()
+            L.check(L.lib().snc_hip_plan_create_config(hg._h, p, ev.size, self.num_steps,
+                                                       C.byref(lc), C.byref(cc),
+                                                       C.byref(self._h)))
+        else:
+            L.check(L.lib().snc_hip_plan_create(hg._h, p, ev.size, self.num_steps, C.byref(lc),
+                                                int(backend), C.byref(self._h)))
 
     @property
     def variant(self) -> Backend:
         return Backend(L.lib().snc_hip_plan_variant(self._h))
+
+    @property
+    def config(self) -> PlanConfig:
+        c = L.PlanConfig()
+        L.check(L.lib().snc_hip_plan_get_config(self._h, C.byref(c)))
+        return PlanConfig.from_c(c)
 
     def run(self, capture_raster=False, capture_v=False) -> ForwardResult:
         g = self.hg.graph
@@ -192,17 +240,42 @@ class ForwardPlan:
             pass
 
 
+_GRAPH_CACHE: "OrderedDict[bytes, HipGraph]" = OrderedDict()
+_GRAPH_CACHE_SIZE = 4
+
+
+def _graph_key(g: SNNGraph) -> bytes:
+    h = hashlib.blake2b(digest_size=20)
+    for a in (g.row_ptr, g.post_ids, g.weights, g.delays, g.sign, g.role, g.channel):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.digest()
+
+
+def resident_graph(g: SNNGraph) -> HipGraph:
+    """The device copy of g, uploaded on first use and kept (with its
+    autotuner results) for the next calls on a graph with the same contents."""
+    key = _graph_key(g)
+    hg = _GRAPH_CACHE.get(key)
+    if hg is not None:
+        _GRAPH_CACHE.move_to_end(key)
+        return hg
+    hg = HipGraph(g)
+    _GRAPH_CACHE[key] = hg
+    while len(_GRAPH_CACHE) > _GRAPH_CACHE_SIZE:
+        _GRAPH_CACHE.popitem(last=False)[1].close()
+    return hg
+
+
 def forward(g: SNNGraph, events, num_steps: int, lif: LIFParams, backend: Backend,
             capture_raster=False, capture_v=False) -> ForwardResult:
-    """hip::forward: upload, run one sample, read back. Raises when unavailable."""
+    """hip::forward: run one sample, read back. The graph stays resident
+    between calls on the same contents (content hash), so the autotuner's
+    search for hip-auto runs once per graph. Raises when unavailable."""
     if not backend_is_hip(backend):
         raise ValueError(f"{backend_name(backend)}: the hip runtime serves hip-* backends only")
-    hg = HipGraph(g)
+    hg = resident_graph(g)
+    plan = ForwardPlan(hg, events, num_steps, lif, backend)
     try:
-        plan = ForwardPlan(hg, events, num_steps, lif, backend)
-        try:
-            return plan.run(capture_raster=capture_raster, capture_v=capture_v)
-        finally:
-            plan.close()
+        return plan.run(capture_raster=capture_raster, capture_v=capture_v)
     finally:
-        hg.close()
+        plan.close()

@@ -16,11 +16,12 @@ from conftest import GOLDEN, load_graph_fixture, unpack_raster
 from helpers import events, make_graph, random_events, random_graph
 from snc_amd import configs as K
 from snc_amd import graph as G
-from snc_amd.runtime import Backend, ForwardPlan, HipGraph, LIFParams, available, forward
+from snc_amd.runtime import (Backend, ForwardPlan, HipGraph, LIFParams, PlanConfig, available,
+                             forward)
 
 pytestmark = pytest.mark.gpu
 
-EXACT = [Backend.HipBucket]
+EXACT = [Backend.HipBucket, Backend.HipSort]
 V_RTOL, V_ATOL = 1e-5, 1e-6
 
 
@@ -194,3 +195,92 @@ def test_c1_shaped_reduced(backend):
     assert r2.stats.spikes == r.stats.spikes
     if backend != Backend.HipAtomic:
         assert np.array_equal(r2.raster, r.raster)
+
+
+def _heavy_graph(seed=7):
+    """3000 inputs onto two heavy targets (in-degree 3000 and 700) with mixed
+    delays, plus 50 light targets."""
+    rng = np.random.default_rng(seed)
+    n_in = 3000
+    edges = []
+    for a in range(n_in):
+        edges.append((a, n_in, float(rng.normal(0, 0.05)), int(rng.integers(1, 40))))
+        if a < 700:
+            edges.append((a, n_in + 1, float(rng.normal(0, 0.08)), int(rng.integers(1, 5))))
+        if a % 3 == 0:
+            edges.append((a, n_in + 2 + a % 50, float(rng.normal(0, 0.3)), int(rng.integers(1, 3))))
+    n = n_in + 52
+    g = make_graph(n, edges, [1] * n_in + [0] * 52, list(range(n_in)) + [-1] * 52)
+    return g, random_events(rng, g, 80, rate=0.05, amp=(0.9, 1.2))
+
+
+@pytest.mark.parametrize("block", [128, 256, 512])
+@pytest.mark.parametrize("threshold", [64, 200, 512])
+def test_bucket_launch_configs_are_bit_exact(block, threshold):
+    """Every block shape and hub threshold the autotuner may pick keeps the
+    bucket strategy bit-exact (thresholds below the light limit move the
+    70-180 in-degree targets of the reduced delivery-stress graph and the
+    heavy targets onto the hub kernel)."""
+    g = G.hub_sheet_graph(60, 50, 40, hub_fraction=0.02, hub_min=40, hub_max=1500,
+                          w_std=0.06, max_delay=20, seed=9)
+    from snc_amd import inputs as I
+    ev = I.background_drive(g.num_neurons, 120, 0.02, 1.0, seed=3)
+    lif = K.lif_tau20()
+    cfg = PlanConfig(Backend.HipBucket, block, threshold, False)
+    r = ForwardPlan(HipGraph(g), ev, 120, lif, config=cfg).run(capture_raster=True, capture_v=True)
+    _check_exact(r, O.forward(g, ev, 120, lif), g.num_neurons)
+    g2, ev2 = _heavy_graph()
+    lif2 = LIFParams(1.0, 0.95, 0.0, 1)
+    r2 = ForwardPlan(HipGraph(g2), ev2, 80, lif2, config=cfg).run(capture_raster=True,
+                                                                  capture_v=True)
+    _check_exact(r2, O.forward(g2, ev2, 80, lif2), g2.num_neurons)
+
+
+@pytest.mark.parametrize("block", [128, 512])
+def test_atomic_block_shapes(block):
+    g2, ev2 = _heavy_graph(8)
+    lif2 = LIFParams(1.0, 0.95, 0.0, 1)
+    cfg = PlanConfig(Backend.HipAtomic, block, 512, False)
+    r = ForwardPlan(HipGraph(g2), ev2, 80, lif2, config=cfg).run(capture_v=True)
+    _check_atomic(r, O.forward(g2, ev2, 80, lif2))
+
+
+def test_invalid_plan_config_is_rejected():
+    from snc_amd._lib import HipError
+    g, ev = _heavy_graph()
+    hg = HipGraph(g)
+    for cfg in (PlanConfig(Backend.HipBucket, 192, 512), PlanConfig(Backend.HipBucket, 256, 600),
+                PlanConfig(Backend.HipBucket, 256, 0), PlanConfig(Backend.HipAuto, 256, 512)):
+        with pytest.raises(HipError):
+            ForwardPlan(hg, ev, 10, LIFParams(), config=cfg)
+
+
+def test_autotuner_search_picks_fastest_and_stays_exact():
+    g = G.hub_sheet_graph(120, 100, 60, hub_fraction=0.01, hub_min=60, hub_max=3000,
+                          w_std=0.05, max_delay=32, seed=5)
+    from snc_amd import inputs as I
+    ev = I.background_drive(g.num_neurons, 200, 0.01, 1.0, seed=6)
+    lif = K.lif_tau20()
+    hg = HipGraph(g)
+    ref = O.forward(g, ev, 200, lif)
+    plan = ForwardPlan(hg, ev, 200, lif, Backend.HipAutoExact)
+    assert plan.variant in (Backend.HipBucket, Backend.HipSort)
+    _check_exact(plan.run(capture_raster=True, capture_v=True), ref, g.num_neurons)
+    rep = hg.tune_report(exact_only=True)
+    ms = [m for _, m in rep["candidates"]]
+    assert len(ms) >= 4 and rep["trial_steps"] == 128
+    assert all(c.variant != Backend.HipAtomic for c, _ in rep["candidates"])
+    ran = [m for m in ms if m >= 0]
+    assert ms[rep["chosen"]] == min(ran)
+    assert plan.config == rep["candidates"][rep["chosen"]][0]
+    # cached: a second plan reuses the search
+    plan2 = ForwardPlan(hg, ev, 200, lif, Backend.HipAutoExact)
+    assert plan2.config == plan.config
+    assert hg.tune_report(exact_only=True)["search_ms"] == rep["search_ms"]
+    # the unrestricted search may pick hip-atomic; its result meets that bar
+    p3 = ForwardPlan(hg, ev, 200, lif, Backend.HipAuto)
+    r3 = p3.run(capture_raster=True, capture_v=True)
+    if p3.variant == Backend.HipAtomic:
+        _check_atomic(r3, ref)
+    else:
+        _check_exact(r3, ref, g.num_neurons)

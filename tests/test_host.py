@@ -17,7 +17,7 @@ from helpers import make_graph
 
 def _declared_functions():
     names = set()
-    for h in ("snc_hip.h", "snc_gen.h"):
+    for h in ("snc_hip.h", "snc_gen.h", "snc_struct.h"):
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         for m in re.finditer(r"\b(snc_\w+)\s*\(", src):

@@ -101,6 +101,29 @@ def test_plain_and_batched_are_bit_identical():
         assert np.array_equal(a, b)
 
 
+def test_plain_graph_replay_equals_batched_over_minibatches():
+    """Without rasters the plain path replays one captured hipGraph per sample
+    length; over three minibatches (lengths repeat, so graphs are reused with
+    new samples and epochs) its weights stay bit-identical to the batched
+    path's, and its evaluation matches."""
+    g = G.sheet_graph([100, 64, 10], 16, recurrent_fan_in=8, radius=3.0, sheet=10.0, w_std=0.3,
+                      max_delay=5, seed=3)
+    n = 48
+    X, y = I.prototype_images(n, 100, 10, 0.6, 0.2, seed=5)
+    Xf = X.astype(np.float32) / np.float32(255.0)
+    lengths = np.random.default_rng(2).choice([12, 20, 27], n).astype(np.int32)
+    gcfg, _ = _cfgs(30, readout=READOUT_LEAKY, reset_mode=1, encoder=1, max_rate=0.4)
+    res = []
+    for mode in (PLAIN, BATCHED):
+        s = HipTrainSession(g, gcfg, batch=16, max_steps=30)
+        s.upload_features(Xf, y, lengths)
+        for k, ep in enumerate((1, 1, 2)):
+            s.step(np.arange(16 * k, 16 * k + 16), epoch=ep, mode=mode)
+        res.append((s.get_weights(), s.evaluate(mode=mode)))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1]
+
+
 @pytest.mark.parametrize("readout,reset_mode", [(READOUT_LEAKY, 1), (READOUT_SPIKING, 1),
                                                  (READOUT_LEAKY, 0)])
 def test_device_poisson_and_extensions_match_oracle(readout, reset_mode):
