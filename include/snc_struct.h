@@ -43,6 +43,11 @@ typedef struct snc_struct_config {
   float reward_chance;
   float reward_floor;
   uint64_t seed;
+  /* Heuristic mode with grow logging only: probability that a freed post is
+   * regrown from a uniformly drawn unconnected window candidate instead of the
+   * most-firing one, so the log also holds outcomes of non-greedy choices.
+   * Drawn from a generator of its own; 0 leaves the heuristic unchanged. */
+  float explore;
 } snc_struct_config;
 
 /* StructReport (connectome.hpp:50-54) plus the candidates scored. */

@@ -45,6 +45,7 @@ struct snc_connectome {
   std::vector<int> channel;
   std::vector<std::vector<Edge>> adj;
   std::mt19937_64 rng;
+  std::mt19937_64 explore_rng;  // exploration draws of logging runs only
   std::vector<float> log_feat;
   std::vector<int32_t> log_label;
 
@@ -156,6 +157,7 @@ int snc_conn_layered_local(const int32_t* layers, int32_t n_layers, int32_t budg
   c->layer_begin.resize(n_layers);
   c->layer_count.resize(n_layers);
   c->rng.seed(seed);
+  c->explore_rng.seed(seed ^ 0xA5A5A5A5DEADBEEFULL);
   int idx = 0;
   for (int l = 0; l < n_layers; ++l) {
     c->layer_begin[l] = idx;
@@ -393,9 +395,20 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
   } else {
     std::vector<int> qs;
     std::vector<float> qf;
+    const bool explore = log_grown && cfg->explore > 0.0f;
+    std::uniform_real_distribution<float> u01(0.0f, 1.0f);
     for (int p : freed) {
       if (grown >= pruned) break;
-      const int q = grow_local(p);
+      int q = grow_local(p);
+      if (q >= 0 && explore && u01(c->explore_rng) < cfg->explore) {
+        int b0, klo, khi;
+        std::vector<int> open;
+        if (c->window(p, cfg->locality_window, b0, klo, khi))
+          for (int kk = klo; kk <= khi; ++kk)
+            if (!c->connected(b0 + kk, p)) open.push_back(b0 + kk);
+        if (!open.empty())
+          q = open[std::uniform_int_distribution<size_t>(0, open.size() - 1)(c->explore_rng)];
+      }
       if (q >= 0) {
         int64_t lid = -1;
         if (log_grown) {

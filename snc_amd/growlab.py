@@ -1,63 +1,66 @@
-"""Learned grow proposal, end to end: log grow outcomes from heuristic
-co-training runs, train the MLP on the GPU, and compare learned against
-heuristic growth at an equal synapse budget over several seeds.
+"""Learned grow proposal, end to end: log grow outcomes from the substrate's
+own heuristic co-training runs, train the MLP on the GPU with its own
+forward/backward kernels, and compare learned against heuristic growth at an
+equal synapse budget over several seeds.
 
-Workload: the batched e-prop network 784 -> 512 -> 10 (leaky readout,
-prototype images, Poisson input) as a layered local connectome under a
-3000-synapse budget (Connectome::layered_local semantics, 1-D positions),
-grow = prune = 200 per structural epoch, 12 structural epochs.
+Workload (structural co-training with a learned grow proposal): the batched
+e-prop network's data and training setup (784 inputs, 512 hidden LIF, 10
+leaky readout; prototype images with 20 % pixel noise, Poisson p = pixel/255
+* 0.2, T_b ~ U[50, 200], lr 1.0, minibatch 256) as a layered local connectome
+(Connectome::layered_local semantics) under a 3000-synapse budget; grow =
+prune = 200 per structural epoch, 12 structural epochs of one training epoch
+each. Logging runs explore: a share of freed posts regrows from a uniformly
+drawn window candidate, so the log holds outcomes of non-greedy choices too.
 """
 from __future__ import annotations
 
+import dataclasses
 import time
 
 import numpy as np
 
 from . import configs as K
-from . import inputs as I
 from .grow import GrowMLP
 from .structure import GROW_HEURISTIC, GROW_LEARNED, Connectome, StructConfig, cotrain
-from .train import HipTrainSession, READOUT_LEAKY, TrainConfig
+from .train import HipTrainSession
+
+LAYERS = [784, 512, 10]
 
 
-def dataset(n_train=1024, n_test=512, steps=60, seed=7):
-    Xu, y = I.prototype_images(n_train + n_test, 784, 10, 0.7, 0.2, seed=seed)
-    X = Xu.astype(np.float32) / np.float32(255.0)
-    return X, y, np.full(n_train + n_test, steps, np.int32), n_train
+def workload(n_train=2048, n_test=512, seed=1):
+    """Data and trainer configuration of the co-training runs."""
+    w = K.c2(n_train=n_train, n_test=n_test, seed=seed)
+    cfg = dataclasses.replace(w.cfg, w_init=0.3)
+    return w, cfg
 
 
-def make_session_factory(X, y, lengths, batch=256, steps=60, seed=1, lr=1.0):
-    cfg = TrainConfig(num_steps=steps, lif=K.lif_tau20(), lr=lr, w_init=0.3, w_max=4.0,
-                      surrogate_scale=0.3, feedback_scale=1.0, readout_mode=READOUT_LEAKY,
-                      readout_psi=1.0, encoder=1, enc_gain=1.0, enc_max_rate=0.2, seed=seed)
+def run_cotrain(seed, mode, w, cfg, model=None, log=False, explore=0.0, outer=12, budget=3000,
+                grow=200, on_round=None):
+    """One co-training run: returns (connectome, per-round records)."""
     def make(g):
-        s = HipTrainSession(g, cfg, batch=batch, max_steps=steps)
-        s.upload_features(X, y, lengths)
-        s.n_data = X.shape[0]
+        s = HipTrainSession(g, dataclasses.replace(cfg, seed=seed), batch=w.batch,
+                            max_steps=w.max_steps)
+        s.upload_features(w.X, w.labels, w.lengths)
         return s
 
-    return make, cfg
-
-
-def run_cotrain(seed, mode, model=None, log=False, data=None, outer=12, inner=1, budget=3000,
-                grow=200, steps=60, batch=256):
-    X, y, lengths, n_train = data or dataset(steps=steps)
-    make, _ = make_session_factory(X, y, lengths, batch=batch, steps=steps, seed=seed)
-    con = Connectome([784, 512, 10], budget, 1, 0.3, seed)
+    con = Connectome(LAYERS, budget, 1, 0.02, seed)
     scfg = StructConfig(grow_per_epoch=grow, delay=1, w_grow_init=0.02, locality_window=0.2,
-                        protect_rounds=2, seed=seed)
-    train_idx = np.arange(n_train, dtype=np.int32)
-    test_idx = np.arange(n_train, X.shape[0], dtype=np.int32)
-    rounds = cotrain(con, make, outer=outer, inner=inner, scfg=scfg, mode=mode, model=model,
-                     log=log, train_idx=train_idx, test_idx=test_idx,
-                     activity_norm=float(n_train * steps))
+                        protect_rounds=2, seed=seed, explore=explore)
+    n = w.labels.size
+    train_idx = np.arange(w.n_train, dtype=np.int32)
+    test_idx = np.arange(w.n_train, n, dtype=np.int32)
+    steps = float(w.lengths[: w.n_train].sum())
+    rounds = cotrain(con, make, outer=outer, inner=1, scfg=scfg, mode=mode, model=model, log=log,
+                     train_idx=train_idx, test_idx=test_idx, activity_norm=steps,
+                     on_round=on_round)
     return con, rounds
 
 
-def collect_log(seeds=(101, 102, 103), data=None, **kw):
+def collect_log(seeds, w, cfg, explore=0.5, outer=12):
+    """Labelled grow events (features, survived) from heuristic runs."""
     Xs, ys = [], []
     for s in seeds:
-        con, _ = run_cotrain(s, GROW_HEURISTIC, log=True, data=data, **kw)
+        con, _ = run_cotrain(s, GROW_HEURISTIC, w, cfg, log=True, explore=explore, outer=outer)
         X, y = con.grow_log()
         keep = y >= 0
         Xs.append(X[keep])
@@ -65,33 +68,59 @@ def collect_log(seeds=(101, 102, 103), data=None, **kw):
     return np.concatenate(Xs), np.concatenate(ys).astype(np.float32)
 
 
-def ab_compare(seeds=(1, 2, 3, 4, 5), train_seeds=(101, 102, 103), outer=12, steps=60,
-               mlp_epochs=60):
-    """Returns a dict with per-seed final test accuracies for both growth
-    rules at the same budget, the MLP's training loss and scoring rate."""
+def ab_compare(seeds=(1, 2, 3, 4, 5), train_seeds=(101, 102, 103), outer=12, mlp_epochs=40,
+               explore=0.5, log=print):
+    """Learned vs heuristic grow at equal budget. Returns a dict with the
+    per-seed final test accuracies and synapse counts, the MLP's loss before
+    and after training, and its scoring rate on one million candidates."""
     t0 = time.time()
-    data = dataset(steps=steps)
-    Xl, yl = collect_log(train_seeds, data=data, outer=outer, steps=steps)
+    w, cfg = workload()
+    Xl, yl = collect_log(train_seeds, w, cfg, explore=explore, outer=outer)
+    log(f"grow log: {Xl.shape[0]} labelled events, survival rate {yl.mean():.3f} "
+        f"({time.time() - t0:.1f}s)")
     mlp = GrowMLP(seed=11)
     loss0, _ = mlp.grad(Xl, yl)
     loss = mlp.train(Xl, yl, epochs=mlp_epochs, batch=256, lr=2e-3, seed=5)
-    res = {"log_examples": int(Xl.shape[0]), "log_positive_rate": float(yl.mean()),
-           "mlp_loss_before": loss0, "mlp_loss_after": loss, "heuristic": [], "learned": [],
-           "synapses": []}
+    log(f"MLP BCE {loss0:.4f} -> {loss:.4f}")
+    res = {"log_examples": int(Xl.shape[0]), "log_survival_rate": float(yl.mean()),
+           "mlp_loss_before": float(loss0), "mlp_loss_after": float(loss),
+           "heuristic": [], "learned": [], "synapses": [], "candidates_scored_per_step": []}
     for s in seeds:
-        _, rh = run_cotrain(s, GROW_HEURISTIC, data=data, outer=outer, steps=steps)
-        _, rl = run_cotrain(s, GROW_LEARNED, model=mlp, data=data, outer=outer, steps=steps)
+        _, rh = run_cotrain(s, GROW_HEURISTIC, w, cfg, outer=outer)
+        _, rl = run_cotrain(s, GROW_LEARNED, w, cfg, model=mlp, outer=outer)
         res["heuristic"].append(rh[-1]["test_acc"])
         res["learned"].append(rl[-1]["test_acc"])
-        res["synapses"].append((rh[-1]["synapses"], rl[-1]["synapses"]))
-    # scoring throughput on one million synthetic candidates
+        res["synapses"].append([rh[-1]["synapses"], rl[-1]["synapses"]])
+        cs = [r["candidates_scored"] for r in rl if "candidates_scored" in r]
+        res["candidates_scored_per_step"].append(float(np.mean(cs)) if cs else 0.0)
+        log(f"seed {s}: heuristic {rh[-1]['test_acc']:.4f} learned {rl[-1]['test_acc']:.4f} "
+            f"synapses {rh[-1]['synapses']}/{rl[-1]['synapses']}")
     F = np.random.default_rng(3).standard_normal((1 << 20, 16)).astype(np.float32)
     mlp.score(F)
     mlp.score(F)
     res["score_1M_kernel_ms"] = mlp.last_kernel_ms
     res["score_candidates_per_s"] = (1 << 20) / (mlp.last_kernel_ms / 1e3)
-    res["wall_s"] = time.time() - t0
     h, l = np.array(res["heuristic"]), np.array(res["learned"])
     res["heuristic_mean"], res["learned_mean"] = float(h.mean()), float(l.mean())
-    res["gain_mean"] = float((l - h).mean())
+    res["gain_mean_pt"] = float(100.0 * (l - h).mean())
+    res["wins"] = int((l > h).sum())
+    res["wall_s"] = time.time() - t0
     return res
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description="learned vs heuristic grow at equal budget")
+    ap.add_argument("--seeds", default="1,2,3,4,5")
+    ap.add_argument("--train-seeds", default="101,102,103")
+    ap.add_argument("--outer", type=int, default=12)
+    ap.add_argument("--mlp-epochs", type=int, default=40)
+    ap.add_argument("--explore", type=float, default=0.5)
+    a = ap.parse_args()
+    out = ab_compare(tuple(int(x) for x in a.seeds.split(",")),
+                     tuple(int(x) for x in a.train_seeds.split(",")), a.outer, a.mlp_epochs,
+                     a.explore, log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(out))

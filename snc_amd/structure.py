@@ -19,7 +19,8 @@ class StructConfigC(C.Structure):
     _fields_ = [("grow_per_epoch", C.c_int32), ("delay", C.c_int32), ("w_grow_init", C.c_float),
                 ("locality_window", C.c_float), ("protect_rounds", C.c_int32),
                 ("max_attempts_mult", C.c_int32), ("reward_modulated", C.c_int32),
-                ("reward_chance", C.c_float), ("reward_floor", C.c_float), ("seed", C.c_uint64)]
+                ("reward_chance", C.c_float), ("reward_floor", C.c_float), ("seed", C.c_uint64),
+                ("explore", C.c_float)]
 
 
 class StructReportC(C.Structure):
@@ -40,12 +41,13 @@ class StructConfig:
     reward_chance: float = 0.1
     reward_floor: float = 0.1
     seed: int = 1
+    explore: float = 0.0   # logging runs: share of non-greedy (uniform) grow choices
 
     def c(self):
         return StructConfigC(self.grow_per_epoch, self.delay, self.w_grow_init,
                              self.locality_window, self.protect_rounds, self.max_attempts_mult,
                              1 if self.reward_modulated else 0, self.reward_chance,
-                             self.reward_floor, self.seed)
+                             self.reward_floor, self.seed, self.explore)
 
 
 _SIGS = {
