@@ -39,7 +39,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_units(g, prof, neuron_spikes, steps):
+def kernel_units(g, heavy_threshold, neuron_spikes, steps):
     """Work per kernel class over one run: (neurons stepped, events expanded,
     rows fired, launches). The hub kernel steps the heavy targets and expands
     their own rows; the step kernel does the rest."""
@@ -49,7 +49,7 @@ def kernel_units(g, prof, neuron_spikes, steps):
     indeg = np.bincount(g.post_ids, minlength=n) if g.num_synapses else np.zeros(n, np.int64)
     outdeg = np.diff(g.row_ptr).astype(np.int64)
     spk = neuron_spikes.astype(np.int64)
-    hub = indeg > prof["heavy_threshold"]
+    hub = indeg > heavy_threshold
     ev_hub, f_hub = int((spk * outdeg)[hub].sum()), int(spk[hub].sum())
     ev_all, f_all = int((spk * outdeg).sum()), int(spk.sum())
     n_hub = int(hub.sum())
@@ -139,39 +139,51 @@ def eprop_bench(w, epochs, plain_batches, elig):
 
     def run(sess, mode, bl, epoch):
         t0 = time.perf_counter()
-        dev, samples, spikes, events = 0.0, 0, 0, 0
+        dev, samples, spikes, events, steps = 0.0, 0, 0, 0, 0
         for b in bl:
             st, _, _ = sess.step(b, epoch=epoch, mode=mode, apply=True)
             dev += st.ms
             samples += st.samples
             spikes += st.spikes
             events += st.synaptic_events
-        return time.perf_counter() - t0, dev, samples, spikes, events
+            steps += int(w.lengths[b].sum())
+        return time.perf_counter() - t0, dev, samples, spikes, events, steps
 
-    # plain path over the first nb minibatches
-    sp = _train_session(w, elig)
-    wall_p, dev_p, n_p, spk_p, ev_p = run(sp, PLAIN, batches[:nb], 1)
-    w_plain = sp.get_weights()
-    acc_plain = sp.evaluate(te)
-    sp.close()
+    # plain path over the first nb minibatches (nb = 0: batched only)
+    if nb > 0:
+        sp = _train_session(w, elig)
+        wall_p, dev_p, n_p, spk_p, ev_p, _ = run(sp, PLAIN, batches[:nb], 1)
+        w_plain = sp.get_weights()
+        acc_plain = sp.evaluate(te)
+        sp.close()
     # batched path over the same minibatches, then the rest of the epochs
-    wall_b, dev_b, n_b, spk_b, ev_b = run(sb, BATCHED, batches[:nb], 1)
+    wall_b, dev_b, n_b, spk_b, ev_b, st_b = run(sb, BATCHED, batches[:nb], 1)
+    nmb = len(batches[:nb])
     w_batch = sb.get_weights()
     acc_same = sb.evaluate(te)
-    identical = bool(np.array_equal(w_plain, w_batch))
+    identical = bool(np.array_equal(w_plain, w_batch)) if nb > 0 else None
     if nb < len(batches):
         r = run(sb, BATCHED, batches[nb:], 1)
         wall_b, dev_b, n_b = wall_b + r[0], dev_b + r[1], n_b + r[2]
-        spk_b, ev_b = spk_b + r[3], ev_b + r[4]
+        spk_b, ev_b, st_b = spk_b + r[3], ev_b + r[4], st_b + r[5]
+        nmb += len(batches) - nb
     curve = [sb.evaluate(te)]
     for ep in range(2, epochs + 1):
         o = np.random.default_rng(12345 + ep).permutation(tr).astype(np.int32)
-        r = run(sb, BATCHED, [o[i:i + w.batch] for i in range(0, o.size, w.batch)], ep)
+        bl = [o[i:i + w.batch] for i in range(0, o.size, w.batch)]
+        r = run(sb, BATCHED, bl, ep)
         wall_b, dev_b, n_b = wall_b + r[0], dev_b + r[1], n_b + r[2]
-        spk_b, ev_b = spk_b + r[3], ev_b + r[4]
+        spk_b, ev_b, st_b = spk_b + r[3], ev_b + r[4], st_b + r[5]
+        nmb += len(bl)
         curve.append(sb.evaluate(te))
+    # Algorithmic bytes of the batched steps (per sample-step 28 N + 8 S, per
+    # event 9, per spike 8, per minibatch update 4 S B + 8 S) over their
+    # device time, against the 8 TB/s HBM peak.
+    N, S = w.graph.num_neurons, w.graph.num_synapses
+    model = (28.0 * N + 8.0 * S) * st_b + 9.0 * ev_b + 8.0 * spk_b + (4.0 * S * w.batch + 8.0 * S) * nmb
+    gbs = model / (dev_b / 1e3) / 1e9
     sb.close()
-    bsps, psps = n_b / wall_b, n_p / wall_p
+    bsps = n_b / wall_b
     out = {
         "workload": w.name, "neurons": w.graph.num_neurons, "synapses": w.graph.num_synapses,
         "batch": w.batch, "train_samples": int(tr.size), "test_samples": int(te.size),
@@ -180,16 +192,80 @@ def eprop_bench(w, epochs, plain_batches, elig):
         "batched": {"samples_per_s": bsps, "device_samples_per_s": n_b / (dev_b / 1e3),
                     "samples": n_b, "epochs": epochs, "GSynOps/s": ev_b / wall_b / 1e9,
                     "test_acc_per_epoch": curve},
-        "plain": {"samples_per_s": psps, "device_samples_per_s": n_p / (dev_p / 1e3),
-                  "samples": n_p, "GSynOps/s": ev_p / wall_p / 1e9},
-        "speedup_batched_vs_plain": bsps / psps,
         "test_acc_initial": acc0,
-        "equal_accuracy_check": {"minibatches": nb, "weights_bit_identical": identical,
-                                 "test_acc_plain": acc_plain, "test_acc_batched": acc_same,
-                                 "acc_diff_pt": 100.0 * (acc_same - acc_plain)},
+        "roofline": {"bound": "hbm", "kernel": "k_train_batched + k_apply",
+                     "bytes_model": "(28N + 8S) per sample-step + 9 per event + 8 per spike "
+                                    "+ (4SB + 8S) per minibatch",
+                     "bytes": model, "device_ms": dev_b, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "note": "eligibility stays in registers: the 8S per sample-step of the "
+                             "model is never moved, so frac can exceed what HBM carries"},
     }
-    log(f"e-prop {w.name}: batched {bsps:.1f} samples/s, plain {psps:.1f} samples/s "
-        f"(x{bsps / psps:.1f}), identical={identical}, acc {acc0:.3f} -> {curve}")
+    if nb > 0:
+        psps = n_p / wall_p
+        out["plain"] = {"samples_per_s": psps, "device_samples_per_s": n_p / (dev_p / 1e3),
+                        "samples": n_p, "GSynOps/s": ev_p / wall_p / 1e9}
+        out["speedup_batched_vs_plain"] = bsps / psps
+        out["equal_accuracy_check"] = {"minibatches": nb, "weights_bit_identical": identical,
+                                       "test_acc_plain": acc_plain, "test_acc_batched": acc_same,
+                                       "acc_diff_pt": 100.0 * (acc_same - acc_plain)}
+        log(f"e-prop {w.name}: batched {bsps:.1f} samples/s, plain {psps:.1f} samples/s "
+            f"(x{bsps / psps:.1f}), identical={identical}, acc {acc0:.3f} -> {curve}")
+    else:
+        log(f"e-prop {w.name}: batched {bsps:.1f} samples/s, acc {acc0:.3f} -> {curve}")
+    return out
+
+
+def batched_forward(w, hg, B, steps, warmup, variant):
+    """The delivery-stress graph with B independent drives in one batched plan
+    (one launch per step covers every sample): events/s and the step
+    kernel's algorithmic bandwidth."""
+    import numpy as np
+    from snc_amd import inputs as I
+    from snc_amd.runtime import ForwardPlan, parse_backend, synchronize
+
+    g = w.graph
+    drives = [w.samples[0]] + [I.background_drive(g.num_neurons, w.num_steps, 0.01, 1.0,
+                                                  seed=100 + b) for b in range(1, B)]
+    allev = np.concatenate([d.copy() for d in drives])
+    off = 0
+    for b, d in enumerate(drives):
+        allev["sample_id"][off:off + d.size] = b
+        off += d.size
+    t0 = time.time()
+    plan = ForwardPlan(hg, allev, w.num_steps, w.lif, parse_backend(variant), num_samples=B)
+    plan_s = time.time() - t0
+    r = plan.run()
+    events, spikes = r.stats.synaptic_events, r.stats.spikes
+    for _ in range(warmup):
+        plan.launch()
+    synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        plan.launch()
+    synchronize()
+    wall = (time.perf_counter() - t1) / max(1, steps)
+    prof = plan.profile()
+    units = kernel_units(g, plan.config.heavy_threshold, r.neuron_spikes.sum(axis=0), w.num_steps)
+    kernels = {}
+    for name, (tot, k) in prof.items():
+        key = "k_step_with_hubs" if name == "k_step" and "k_hub_step" in prof else name
+        nu, ev_u, f_u = units[key]
+        by = model_bytes(B * nu, ev_u, f_u) / k  # neuron terms scale with B
+        kernels[name] = {"avg_ms": tot / k, "launches": k, "bytes_per_launch": by,
+                         "gbs": by / (tot / k / 1e3) / 1e9}
+    step_bytes = model_bytes(B * g.num_neurons * w.num_steps, events, spikes)
+    cfgp = plan.config
+    out = {"samples": B, "variant": plan.variant.name,
+           "plan": {"block_neurons": cfgp.block_neurons, "heavy_threshold": cfgp.heavy_threshold},
+           "plan_build_s": plan_s, "ms_per_run": wall * 1e3,
+           "GSynOps/s": events / wall / 1e9, "events_per_run": events,
+           "whole_run": {"gbs": step_bytes / wall / 1e9,
+                         "frac": step_bytes / wall / 1e9 / HBM_PEAK_GBS},
+           "kernels": kernels}
+    log(f"batched forward x{B} ({out['variant']}): {wall * 1e3:.2f} ms/run, "
+        f"{out['GSynOps/s']:.2f} GSynOps/s, whole-run {out['whole_run']['frac'] * 100:.1f}% of HBM")
+    plan.close()
     return out
 
 
@@ -214,9 +290,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--eprop-epochs", type=int, default=2)
     ap.add_argument("--plain-batches", type=int, default=8,
-                    help="minibatches the plain per-sample path is timed on")
+                    help="minibatches the plain per-sample path is timed on (0: skip it)")
+    ap.add_argument("--batch-samples", default="8,32",
+                    help="sample counts of the batched-forward lines (empty: none)")
     ap.add_argument("--no-eprop", action="store_true")
+    ap.add_argument("--no-grow", action="store_true",
+                    help="skip the learned-vs-heuristic grow comparison")
     ap.add_argument("--eprop-only", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for counter runs: a 256 MiB device copy (the calibration of the "
+                         "traffic counters), then one run of --variant on the workload")
     ap.add_argument("--profile-only", action="store_true",
                     help="build the workload and run K steps without extra measurements")
     args = ap.parse_args()
@@ -243,6 +326,15 @@ def main():
     if args.eprop_only:
         print(json.dumps({"eprop": eprop_all()}))
         return
+    if args.pmc_probe:
+        import torch
+
+        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)  # 256 MiB read + 256 MiB written per dispatch
+        torch.cuda.synchronize()
+        del x, y
     t0 = time.time()
     w = K.c1(args.width, args.height, args.timesteps)
     g = w.graph
@@ -287,7 +379,7 @@ def main():
         return
 
     prof = plan.profile()
-    units = kernel_units(g, hg.profile(), ref.neuron_spikes, w.num_steps)
+    units = kernel_units(g, plan.config.heavy_threshold, ref.neuron_spikes, w.num_steps)
     kernels = {}
     for name, (tot, k) in prof.items():
         key = "k_step_with_hubs" if name == "k_step" and "k_hub_step" in prof else name
@@ -336,8 +428,17 @@ def main():
                       "spikes_per_step": spikes, "events_per_step": events,
                       "parallelism": "single-gpu"},
            "roofline": roofline, "variants": variants, "autotune": autotune}
+    out["batched_forward"] = {}
+    for bs in [int(x) for x in args.batch_samples.split(",") if x]:
+        for vn in ("hip-auto", "hip-auto-exact"):
+            out["batched_forward"][f"{vn}_x{bs}"] = batched_forward(w, hg, bs, max(1, args.steps // 2),
+                                                                    1, vn)
     if not args.no_eprop:
         out["eprop"] = eprop_all()
+    if not args.no_grow:
+        from snc_amd.growlab import ab_compare
+
+        out["grow_ab"] = ab_compare(log=log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_repeats)
     print(json.dumps(out))

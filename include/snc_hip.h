@@ -162,9 +162,10 @@ int snc_hip_plan_get_config(const snc_hip_plan* p, snc_plan_config* out);
 
 /* The autotuner's search for a graph: the candidates it derived from the
  * graph's in-degree statistics, the measured device time per step of each on
- * a trial window of the first trial_steps steps of the sample that triggered
+ * a trial window of the first trial_steps steps of the input that triggered
  * the search, and the one chosen (the fastest). exact_only selects the
- * SNC_VARIANT_AUTO_EXACT search. Fails when no search has run yet. */
+ * SNC_VARIANT_AUTO_EXACT search, batched the search of batched plans (kept
+ * apart from single-sample plans). Fails when no such search has run yet. */
 #define SNC_TUNE_MAX_CANDIDATES 24
 typedef struct snc_tune_report {
   int32_t n_candidates;
@@ -174,7 +175,25 @@ typedef struct snc_tune_report {
   snc_plan_config cand[SNC_TUNE_MAX_CANDIDATES];
   double ms_per_step[SNC_TUNE_MAX_CANDIDATES];
 } snc_tune_report;
-int snc_hip_graph_tune_report(const snc_hip_graph* g, int32_t exact_only, snc_tune_report* out);
+int snc_hip_graph_tune_report(const snc_hip_graph* g, int32_t exact_only, int32_t batched,
+                              snc_tune_report* out);
+
+/* Batched forward: num_samples independent samples of the same graph in one
+ * plan, stepped together (one launch per step covers every sample). An event
+ * belongs to sample event.sample_id in [0, num_samples); others are ignored.
+ * No reference counterpart: snc_bench evaluates samples one forward() at a
+ * time (src/bench/snc_bench.cpp:185-211); each sample's results equal that
+ * forward's. cfg (may be NULL) fixes the launch configuration, else variant
+ * selects it as in snc_hip_plan_create (hip-sort and persistent launches are
+ * single-sample only). Capture arrays of a batched run carry a leading sample
+ * dimension: logits [B][channels], neuron_spikes and final_v [B][N], raster
+ * [B][T][ceil(N/32)], step_spikes [B][T]; stats are totals over the samples. */
+int snc_hip_plan_create_batch(snc_hip_graph* g, const snc_event* events, int64_t n_events,
+                              int32_t num_samples, int32_t num_steps, const snc_lif* lif,
+                              int32_t variant, const snc_plan_config* cfg, snc_hip_plan** out);
+int32_t snc_hip_plan_num_samples(const snc_hip_plan* p);
+/* Spikes and synaptic events of each sample ([num_samples] each) in the last run. */
+int snc_hip_plan_sample_stats(const snc_hip_plan* p, int64_t* spikes, int64_t* events);
 /* Run the plan from a zero state. Inputs are already resident; stats->ms is
  * the device time of the step loop. */
 int snc_hip_plan_run(snc_hip_plan* p, snc_forward_capture* cap, snc_forward_stats* stats);

@@ -168,7 +168,12 @@ _SIGS = {
                                              C.POINTER(Lif), C.POINTER(PlanConfig),
                                              C.POINTER(_P)]),
     "snc_hip_plan_get_config": (C.c_int, [_P, C.POINTER(PlanConfig)]),
-    "snc_hip_graph_tune_report": (C.c_int, [_P, C.c_int32, C.POINTER(TuneReport)]),
+    "snc_hip_graph_tune_report": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(TuneReport)]),
+    "snc_hip_plan_create_batch": (C.c_int, [_P, C.POINTER(Event), C.c_int64, C.c_int32, C.c_int32,
+                                            C.POINTER(Lif), C.c_int32, C.POINTER(PlanConfig),
+                                            C.POINTER(_P)]),
+    "snc_hip_plan_num_samples": (C.c_int32, [_P]),
+    "snc_hip_plan_sample_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "snc_hip_plan_run": (C.c_int, [_P, C.POINTER(ForwardCapture), C.POINTER(ForwardStats)]),
     "snc_hip_plan_launch": (C.c_int, [_P]),
     "snc_hip_plan_variant": (C.c_int32, [_P]),

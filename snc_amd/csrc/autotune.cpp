@@ -10,8 +10,10 @@
 //   * hip-atomic (unless only bit-exact strategies are wanted) at every block
 //     size: its cost is the expansion's atomics, whose contention on hubs is
 //     measured by the trial rather than modelled.
-//   * hip-sort is tried once (its cost follows the events, not the shape).
-//   * persistent variants only when cooperative launch is available.
+//   * hip-sort is tried once (its cost follows the events, not the shape),
+//     for single-sample plans.
+//   * persistent variants only when cooperative launch is available, for
+//     single-sample plans.
 #include "autotune.hpp"
 
 #include <algorithm>
@@ -33,7 +35,8 @@ snc_plan_config default_plan_config(int variant) {
   return c;
 }
 
-std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop) {
+std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop,
+                                                 bool allow_sort) {
   std::vector<snc_plan_config> out;
   auto add = [&](int variant, int block, int thr, int persistent) {
     if ((int)out.size() >= SNC_TUNE_MAX_CANDIDATES) return;
@@ -61,7 +64,8 @@ std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_o
 
   if (!exact_only)
     for (int blk : {128, 256, 512}) add(SNC_VARIANT_ATOMIC, blk, kLightLimit, 0);
-  if ((uint64_t)g.n * (uint64_t)g.L < (1ull << 32)) add(SNC_VARIANT_SORT, 256, kLightLimit, 0);
+  if (allow_sort && (uint64_t)g.n * (uint64_t)g.L < (1ull << 32))
+    add(SNC_VARIANT_SORT, 256, kLightLimit, 0);
   if (coop) {
     add(SNC_VARIANT_BUCKET, 256, kLightLimit, 1);
     if (!exact_only) add(SNC_VARIANT_ATOMIC, 256, kLightLimit, 1);

@@ -18,8 +18,10 @@ namespace snc_amd {
 snc_plan_config default_plan_config(int variant);
 
 // Candidates for the search. exact_only drops hip-atomic; coop says whether
-// the device can run the persistent (cooperative) kernel.
-std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop);
+// the persistent (cooperative) kernel may run; allow_sort whether hip-sort
+// may (single-sample plans only).
+std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop,
+                                                 bool allow_sort);
 
 // Steps of the trial window for a sample of num_steps steps.
 int autotune_trial_steps(int num_steps);

@@ -84,7 +84,7 @@ using namespace snc_amd;
 struct snc_hip_plan;
 static int sort_alloc(snc_hip_plan* P);
 static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_events,
-                           int num_steps, const snc_lif* lif, bool exact_only,
+                           int num_samples, int num_steps, const snc_lif* lif, bool exact_only,
                            snc_plan_config* out);
 
 struct snc_hip_plan {
@@ -92,6 +92,7 @@ struct snc_hip_plan {
   FwdPlanCore core;
   StepArgs a{};
   int T = 0;
+  int B = 1;  // samples (grid.y of the step kernels)
   snc_lif lif{};
   int variant = SNC_VARIANT_ATOMIC;
   snc_plan_config cfg{};
@@ -119,9 +120,10 @@ struct snc_hip_plan {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   size_t n_blk_slots = 0;
-  int bucket_halves = 2;
   int halves = 1;
   size_t dirty_stride = 0;
+  size_t bits_total = 0, dirty_total = 0, ring_total = 0;  // elements allocated
+  std::vector<int64_t> last_spikes, last_events;           // per sample, last run
   uint32_t *keys = nullptr, *keys_alt = nullptr;
   int32_t *vals = nullptr, *vals_alt = nullptr, *hist = nullptr, *ev_ptr = nullptr;
   size_t sort_capacity = 0;
@@ -227,8 +229,13 @@ int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
       g.in_degree[gv->post_ids[e]]++;
     }
   }
-  // Padded incoming columns: the CSR walk (pre ascending, e ascending)
-  // appends to each target's column, so columns come out in CPU fold order.
+  // Padded incoming columns, ordered (delay descending, pre ascending, CSR
+  // position ascending). For one arrival step the CPU adds a target's
+  // deposits in order of their origin step (earlier first, i.e. longer delay
+  // first) and, within an origin step, in CSR order; so walking the column
+  // in this order over the deposits that arrive together reproduces the
+  // CPU's sum. Per origin step (one delay) it is the plain CSR order, which
+  // the ring-based folds rely on.
   std::vector<int32_t> ccol((size_t)n + 1, 0);
   int64_t P = 0;
   for (int j = 0; j < n; ++j) {
@@ -238,15 +245,34 @@ int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
   }
   ccol[n] = (int32_t)P;
   g.P = P;
-  std::vector<int32_t> cpos(S), fillc(n, 0);
+  std::vector<int32_t> cpos(S), fillc(n, 0), csr_order(S);
+  std::vector<int64_t> cstart((size_t)n + 1, 0);
+  for (int j = 0; j < n; ++j) cstart[j + 1] = cstart[j] + g.in_degree[j];
+  for (int64_t e = 0; e < S; ++e) {  // each column's synapses in CSR order
+    const int j = gv->post_ids[e];
+    csr_order[cstart[j] + fillc[j]++] = (int32_t)e;
+  }
   std::vector<float> cval((size_t)P, 0.0f);
   std::vector<uint8_t> cdel((size_t)P, 0);
-  for (int64_t e = 0; e < S; ++e) {
-    const int j = gv->post_ids[e];
-    const int32_t p = ccol[j] + fillc[j]++;
-    cpos[e] = p;
-    cval[p] = wsig[e];
-    cdel[p] = del8[e];
+  {
+    int32_t cnt[257];
+    for (int j = 0; j < n; ++j) {  // stable counting sort by delay, descending
+      std::fill(cnt, cnt + 257, 0);
+      for (int64_t k = cstart[j]; k < cstart[j + 1]; ++k) cnt[255 - del8[csr_order[k]]]++;
+      int32_t run = ccol[j];
+      for (int b = 0; b < 256; ++b) {
+        const int32_t c = cnt[b];
+        cnt[b] = run;
+        run += c;
+      }
+      for (int64_t k = cstart[j]; k < cstart[j + 1]; ++k) {
+        const int32_t e = csr_order[k];
+        const int32_t p = cnt[255 - del8[e]]++;
+        cpos[e] = p;
+        cval[p] = wsig[e];
+        cdel[p] = del8[e];
+      }
+    }
   }
   // Hubs at the default threshold, for the profile (plans build their own).
   std::vector<int32_t> heavy;
@@ -304,10 +330,14 @@ int snc_hip_graph_profile_get(const snc_hip_graph* g, snc_graph_profile* out) {
   return SNC_OK;
 }
 
-int snc_hip_plan_create_config(snc_hip_graph* G, const snc_event* events, int64_t n_events,
-                               int32_t num_steps, const snc_lif* lif,
-                               const snc_plan_config* cfg, snc_hip_plan** out) {
-  if (!G || !lif || !out || !cfg || (n_events > 0 && !events) || num_steps < 0)
+// Builds a plan for B samples. With B = 1 every event belongs to the one
+// sample whatever its sample_id (the reference's forward reads one sample's
+// events); with B > 1 an event belongs to sample sample_id and events of
+// other ids are ignored. Batched plans run the fused step kernels only.
+static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_events, int B,
+                      int32_t num_steps, const snc_lif* lif, const snc_plan_config* cfg,
+                      snc_hip_plan** out) {
+  if (!G || !lif || !out || !cfg || (n_events > 0 && !events) || num_steps < 0 || B < 1)
     return fail(SNC_E_INVALID, "bad plan arguments");
   *out = nullptr;
   if (lif->reset_mode != SNC_RESET_TO_VALUE && lif->reset_mode != SNC_RESET_SUBTRACT)
@@ -319,10 +349,13 @@ int snc_hip_plan_create_config(snc_hip_graph* G, const snc_event* events, int64_
     return fail(SNC_E_INVALID, "block_neurons must be 128, 256 or 512");
   if (cfg->heavy_threshold < 1 || cfg->heavy_threshold > fused_light_max_in_degree())
     return fail(SNC_E_INVALID, "heavy_threshold must be in [1, 512]");
+  if (B > 1 && (variant == SNC_VARIANT_SORT || cfg->persistent))
+    return fail(SNC_E_UNSUPPORTED, "batched plans run hip-atomic or hip-bucket step kernels");
   const DevGraph& g = G->g;
   std::unique_ptr<snc_hip_plan> P(new snc_hip_plan());
   P->G = G;
   P->T = num_steps;
+  P->B = B;
   P->lif = *lif;
   P->cfg = *cfg;
   P->variant = variant;
@@ -338,71 +371,94 @@ int snc_hip_plan_create_config(snc_hip_graph* G, const snc_event* events, int64_
     }
   const int n_heavy = (int)P->heavy_host.size();
 
-  // Per-step injections, resolved channel -> INPUT neurons in the reference's
-  // order (cpu_backend.cpp:50-63) and folded per neuron in that order, so the
-  // device writes one pre-summed ext value per (step, neuron).
-  std::vector<std::vector<std::pair<int32_t, float>>> by_step(num_steps);
-  const int n_in_ch = (int)g.chan_to_inputs.size();
-  for (int64_t k = 0; k < n_events; ++k) {
-    const snc_event& e = events[k];
-    if (e.time < 0 || e.time >= num_steps) continue;
-    if (e.channel_id < 0 || e.channel_id >= n_in_ch) continue;
-    for (int nid : g.chan_to_inputs[e.channel_id]) by_step[e.time].push_back({nid, e.amplitude});
+  // Events of each sample, in input order.
+  std::vector<int64_t> ev_ptr((size_t)B + 1, 0), ev_idx;
+  if (B == 1) {
+    ev_ptr[1] = n_events;
+  } else {
+    for (int64_t k = 0; k < n_events; ++k)
+      if (events[k].sample_id >= 0 && events[k].sample_id < B) ev_ptr[events[k].sample_id + 1]++;
+    for (int b = 0; b < B; ++b) ev_ptr[b + 1] += ev_ptr[b];
+    ev_idx.resize((size_t)ev_ptr[B]);
+    std::vector<int64_t> fill(ev_ptr.begin(), ev_ptr.end() - 1);
+    for (int64_t k = 0; k < n_events; ++k)
+      if (events[k].sample_id >= 0 && events[k].sample_id < B) ev_idx[fill[events[k].sample_id]++] = k;
   }
-  std::vector<int32_t> ip((size_t)num_steps + 1, 0), inid;
-  std::vector<float> iamp;
-  for (int t = 0; t < num_steps; ++t) {
-    auto& lst = by_step[t];
-    std::stable_sort(lst.begin(), lst.end(),
-                     [](const auto& x, const auto& y) { return x.first < y.first; });
-    for (size_t k = 0; k < lst.size();) {
-      float acc = 0.0f;
-      size_t q = k;
-      for (; q < lst.size() && lst[q].first == lst[k].first; ++q) acc += lst[q].second;
-      inid.push_back(lst[k].first);
-      iamp.push_back(acc);
-      k = q;
-    }
-    ip[t + 1] = (int32_t)inid.size();
-    P->core.max_inj_per_step = std::max<int64_t>(P->core.max_inj_per_step, ip[t + 1] - ip[t]);
-  }
-  P->h_inj_ptr = ip;
 
-  // Fused per-step kernels: injections regrouped per (step, block of
-  // fused_block_neurons() neurons); hip-bucket steps heavy targets in their
-  // own blocks, so their injections go to a dense [step][hub] table.
+  // Per sample: injections resolved channel -> INPUT neurons in the
+  // reference's order (cpu_backend.cpp:50-63) and folded per (step, neuron)
+  // in that order; then regrouped per (step, block of block_neurons neurons)
+  // for the fused kernels, hub targets (hip-bucket) into a dense
+  // [step][hub] table.
+  const int n_in_ch = (int)g.chan_to_inputs.size();
   const int nb = cfg->block_neurons;
   const int R = std::max(1, (n + nb - 1) / nb);
   const bool hubs = variant == SNC_VARIANT_BUCKET && n_heavy > 0;
   std::vector<int32_t> hub_index(hubs ? n : 0, -1);
   for (int k = 0; hubs && k < n_heavy; ++k) hub_index[P->heavy_host[k]] = k;
-  std::vector<int32_t> blk_ptr((size_t)num_steps * (R + 1), 0), bnid;
-  std::vector<float> bamp, hinj(hubs ? (size_t)num_steps * n_heavy : 1, 0.0f);
-  for (int t = 0; t < num_steps; ++t) {
-    int32_t* bp = &blk_ptr[(size_t)t * (R + 1)];
-    int blk = 0;
-    bp[0] = (int32_t)bnid.size();
-    for (int k = ip[t]; k < ip[t + 1]; ++k) {  // entries sorted by neuron
-      const int j = inid[k];
-      if (hubs && hub_index[j] >= 0) {
-        hinj[(size_t)t * n_heavy + hub_index[j]] = iamp[k];
-        continue;
-      }
-      while (blk < j / nb) bp[++blk] = (int32_t)bnid.size();
-      bnid.push_back(j);
-      bamp.push_back(iamp[k]);
+  std::vector<int32_t> blk_ptr((size_t)B * num_steps * (R + 1), 0), bnid;
+  std::vector<float> bamp, hinj(hubs ? (size_t)B * num_steps * n_heavy : 1, 0.0f);
+  std::vector<int32_t> ip((size_t)num_steps + 1, 0), inid;  // sample 0 (hip-sort, persistent)
+  std::vector<float> iamp;
+  for (int smp = 0; smp < B; ++smp) {
+    std::vector<std::vector<std::pair<int32_t, float>>> by_step(num_steps);
+    for (int64_t q = ev_ptr[smp]; q < ev_ptr[smp + 1]; ++q) {
+      const snc_event& e = events[B == 1 ? q : ev_idx[q]];
+      if (e.time < 0 || e.time >= num_steps) continue;
+      if (e.channel_id < 0 || e.channel_id >= n_in_ch) continue;
+      for (int nid : g.chan_to_inputs[e.channel_id]) by_step[e.time].push_back({nid, e.amplitude});
     }
-    while (blk < R) bp[++blk] = (int32_t)bnid.size();
+    for (int t = 0; t < num_steps; ++t) {
+      auto& lst = by_step[t];
+      std::stable_sort(lst.begin(), lst.end(),
+                       [](const auto& x, const auto& y) { return x.first < y.first; });
+      int32_t* bp = &blk_ptr[((size_t)smp * num_steps + t) * (R + 1)];
+      int blk = 0;
+      bp[0] = (int32_t)bnid.size();
+      int64_t cnt = 0;
+      for (size_t k = 0; k < lst.size();) {
+        float acc = 0.0f;
+        size_t q = k;
+        for (; q < lst.size() && lst[q].first == lst[k].first; ++q) acc += lst[q].second;
+        const int j = lst[k].first;
+        k = q;
+        ++cnt;
+        if (smp == 0) {
+          inid.push_back(j);
+          iamp.push_back(acc);
+        }
+        if (hubs && hub_index[j] >= 0) {
+          hinj[((size_t)smp * num_steps + t) * n_heavy + hub_index[j]] = acc;
+          continue;
+        }
+        while (blk < j / nb) bp[++blk] = (int32_t)bnid.size();
+        bnid.push_back(j);
+        bamp.push_back(acc);
+      }
+      while (blk < R) bp[++blk] = (int32_t)bnid.size();
+      if (smp == 0) ip[t + 1] = (int32_t)inid.size();
+      P->core.max_inj_per_step = std::max<int64_t>(P->core.max_inj_per_step, cnt);
+    }
+    if (bnid.size() > (size_t)0x7fffffff) return fail(SNC_E_UNSUPPORTED, "too many injections");
   }
+  P->h_inj_ptr = ip;
 
-  // One persistent launch per run when the device and graph allow it (and it
-  // is asked for); its ext is double-buffered by step parity. The bucket
-  // bitmap and dirty flags are double-buffered in every mode.
-  P->persistent = persist_choose(g, variant, cfg->persistent != 0, P->pcfg);
+  // State layout. hip-atomic and hip-sort: ring [B][L][n]. hip-bucket with
+  // the fused kernels: one arrival bitmap and one arrival-flag array per
+  // ring slot, [B][L][...], and no ring. Persistent: ring plus bitmap and
+  // flags double-buffered by step parity.
+  P->persistent = B == 1 && persist_choose(g, variant, cfg->persistent != 0, P->pcfg);
   P->cfg.persistent = P->persistent ? 1 : 0;
+  const bool bucket = variant == SNC_VARIANT_BUCKET;
+  const bool slots = bucket && !P->persistent;
   const int halves = P->persistent ? 2 : 1;
-  const int bucket_halves = 2;
   const size_t dirty_stride = ((size_t)n + 3) / 4 * 4;
+  P->n_bits_words = (size_t)(g.P / 32);
+  P->dirty_stride = dirty_stride;
+  const size_t planes = slots ? (size_t)g.L : (bucket ? 2 : 0);
+  P->bits_total = (size_t)B * planes * P->n_bits_words;
+  P->dirty_total = (size_t)B * planes * dirty_stride;
+  P->ring_total = slots ? 0 : (size_t)B * g.L * n;
   int rc = SNC_OK;
   if (!rc) rc = dupload(&P->heavy_ids, P->heavy_host.data(), P->heavy_host.size());
   if (!rc) rc = dupload(&P->is_heavy, is_heavy_h.data(), is_heavy_h.size());
@@ -413,22 +469,21 @@ int snc_hip_plan_create_config(snc_hip_graph* G, const snc_event* events, int64_
   if (!rc) rc = dupload(&P->inj_ptr, ip.data(), ip.size());
   if (!rc) rc = dupload(&P->inj_nid, inid.data(), inid.size());
   if (!rc) rc = dupload(&P->inj_amp, iamp.data(), iamp.size());
-  if (!rc) rc = dalloc(&P->v, (size_t)n);
+  if (!rc) rc = dalloc(&P->v, (size_t)B * n);
   if (!rc) rc = dalloc(&P->ext, (size_t)n * halves);
-  if (!rc) rc = dalloc(&P->ring, (size_t)g.L * n);
-  if (!rc) rc = dalloc(&P->refr, (size_t)n);
+  if (!rc && P->ring_total) rc = dalloc(&P->ring, P->ring_total);
+  if (!rc) rc = dalloc(&P->refr, (size_t)B * n);
   if (!rc) rc = dalloc(&P->fired, (size_t)n);
   if (!rc) rc = dalloc(&P->fcnt, 2);
-  if (!rc) rc = dalloc(&P->nspk, (size_t)n);
-  if (!rc) rc = dalloc(&P->dirty, dirty_stride * bucket_halves);  // cleared in 4-byte words
-  P->n_bits_words = (size_t)(g.P / 32);
-  if (!rc && variant == SNC_VARIANT_BUCKET) rc = dalloc(&P->bits, P->n_bits_words * bucket_halves);
+  if (!rc) rc = dalloc(&P->nspk, (size_t)B * n);
+  if (!rc && P->dirty_total) rc = dalloc(&P->dirty, P->dirty_total);  // cleared in 4-byte words
+  if (!rc && P->bits_total) rc = dalloc(&P->bits, P->bits_total);
   P->core.lif_grid = std::max(1, (n + 255) / 256);
   P->core.expand_grid = std::max(1, std::min(2048, (n + 255) / 256));
-  P->core.n_heavy = variant == SNC_VARIANT_BUCKET ? n_heavy : 0;
+  P->core.n_heavy = bucket ? n_heavy : 0;
   P->n_blk_slots = (size_t)std::max(4096, R + n_heavy + 1);
-  if (!rc) rc = dalloc(&P->blk_spikes, P->n_blk_slots);
-  if (!rc) rc = dalloc(&P->blk_events, P->n_blk_slots);
+  if (!rc) rc = dalloc(&P->blk_spikes, (size_t)B * P->n_blk_slots);
+  if (!rc) rc = dalloc(&P->blk_events, (size_t)B * P->n_blk_slots);
   if (variant == SNC_VARIANT_SORT && (uint64_t)n * (uint64_t)g.L >= (1ull << 32))
     return fail(SNC_E_UNSUPPORTED, "hip-sort keys need n * ring_len < 2^32");
   if (!rc && variant == SNC_VARIANT_SORT) rc = sort_alloc(P.get());
@@ -440,8 +495,6 @@ int snc_hip_plan_create_config(snc_hip_graph* G, const snc_event* events, int64_
     P->pcfg.dirty_stride = (int64_t)dirty_stride;
   }
   P->halves = halves;
-  P->bucket_halves = bucket_halves;
-  P->dirty_stride = dirty_stride;
   if (rc) return rc;
   if (hipStreamCreateWithFlags(&P->core.stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) != hipSuccess ||
@@ -498,9 +551,25 @@ int snc_hip_plan_create_config(snc_hip_graph* G, const snc_event* events, int64_
   a.n_heavy = hubs ? n_heavy : 0;
   a.bits_half = (int64_t)P->n_bits_words;
   a.dirty_half = (int64_t)dirty_stride;
+  a.bits_slot = (int64_t)P->n_bits_words;
+  a.dirty_slot = (int64_t)dirty_stride;
+  a.B = B;
+  a.ring_stride = (int64_t)g.L * n;
+  a.bits_stride = (int64_t)(planes * P->n_bits_words);
+  a.dirty_stride = (int64_t)(planes * dirty_stride);
+  a.blk_stride = (int64_t)P->n_blk_slots;
+  a.inj_stride = (int64_t)num_steps * (R + 1);
+  a.hinj_stride = (int64_t)num_steps * n_heavy;
+  a.raster_stride = (int64_t)num_steps * a.words;
 
   *out = P.release();
   return SNC_OK;
+}
+
+int snc_hip_plan_create_config(snc_hip_graph* G, const snc_event* events, int64_t n_events,
+                               int32_t num_steps, const snc_lif* lif,
+                               const snc_plan_config* cfg, snc_hip_plan** out) {
+  return plan_build(G, events, n_events, 1, num_steps, lif, cfg, out);
 }
 
 int snc_hip_plan_get_config(const snc_hip_plan* p, snc_plan_config* out) {
@@ -509,31 +578,39 @@ int snc_hip_plan_get_config(const snc_hip_plan* p, snc_plan_config* out) {
   return SNC_OK;
 }
 
-int snc_hip_graph_tune_report(const snc_hip_graph* g, int32_t exact_only, snc_tune_report* out) {
+int snc_hip_graph_tune_report(const snc_hip_graph* g, int32_t exact_only, int32_t batched,
+                              snc_tune_report* out) {
   if (!g || !out) return fail(SNC_E_INVALID, "null argument");
-  const int k = exact_only ? 1 : 0;
+  const int k = (exact_only ? 1 : 0) + (batched ? 2 : 0);
   if (!g->tuned[k]) return fail(SNC_E_INVALID, "no autotuner search has run for this graph");
   *out = g->tune[k];
   return SNC_OK;
 }
 
+int snc_hip_plan_create_batch(snc_hip_graph* G, const snc_event* events, int64_t n_events,
+                              int32_t num_samples, int32_t num_steps, const snc_lif* lif,
+                              int32_t variant, const snc_plan_config* cfg, snc_hip_plan** out) {
+  if (!G || !lif || !out || (n_events > 0 && !events) || num_steps < 0 || num_samples < 1)
+    return fail(SNC_E_INVALID, "bad plan arguments");
+  if (cfg) return plan_build(G, events, n_events, num_samples, num_steps, lif, cfg, out);
+  snc_plan_config c;
+  if (variant == SNC_VARIANT_AUTO || variant == SNC_VARIANT_AUTO_EXACT) {
+    const int rc = autotune_search(G, events, n_events, num_samples, num_steps, lif,
+                                   variant == SNC_VARIANT_AUTO_EXACT, &c);
+    if (rc) return rc;
+  } else if (variant == SNC_VARIANT_ATOMIC || variant == SNC_VARIANT_BUCKET ||
+             (variant == SNC_VARIANT_SORT && num_samples == 1)) {
+    c = default_plan_config(variant);
+  } else {
+    return fail(SNC_E_INVALID, "unknown variant (batched plans: hip-atomic, hip-bucket, auto)");
+  }
+  return plan_build(G, events, n_events, num_samples, num_steps, lif, &c, out);
+}
+
 int snc_hip_plan_create(snc_hip_graph* G, const snc_event* events, int64_t n_events,
                         int32_t num_steps, const snc_lif* lif, int32_t variant,
                         snc_hip_plan** out) {
-  if (!G || !lif || !out || (n_events > 0 && !events) || num_steps < 0)
-    return fail(SNC_E_INVALID, "bad plan arguments");
-  snc_plan_config cfg;
-  if (variant == SNC_VARIANT_AUTO || variant == SNC_VARIANT_AUTO_EXACT) {
-    const int rc = autotune_search(G, events, n_events, num_steps, lif,
-                                   variant == SNC_VARIANT_AUTO_EXACT, &cfg);
-    if (rc) return rc;
-  } else if (variant == SNC_VARIANT_ATOMIC || variant == SNC_VARIANT_BUCKET ||
-             variant == SNC_VARIANT_SORT) {
-    cfg = default_plan_config(variant);
-  } else {
-    return fail(SNC_E_INVALID, "unknown variant");
-  }
-  return snc_hip_plan_create_config(G, events, n_events, num_steps, lif, &cfg, out);
+  return snc_hip_plan_create_batch(G, events, n_events, 1, num_steps, lif, variant, nullptr, out);
 }
 
 int32_t snc_hip_plan_variant(const snc_hip_plan* p) { return p ? p->variant : -1; }
@@ -549,29 +626,30 @@ namespace {
 int reset_state(snc_hip_plan* P, bool raster) {
   const DevGraph& g = P->G->g;
   hipStream_t st = P->core.stream;
-  const int n = g.n;
+  const size_t n = (size_t)g.n, B = (size_t)P->B;
   int rc = SNC_OK;
   auto fill = [&](void* ptr, size_t bytes) {
     if (!rc && ptr && bytes) rc = fwd_launch_fill(st, ptr, 0u, (bytes + 3) / 4);
   };
-  fill(P->v, sizeof(float) * n);
+  fill(P->v, sizeof(float) * B * n);
   fill(P->ext, sizeof(float) * n * P->halves);
-  fill(P->ring, sizeof(float) * (size_t)g.L * n);
-  fill(P->refr, sizeof(int32_t) * n);
+  fill(P->ring, sizeof(float) * P->ring_total);
+  fill(P->refr, sizeof(int32_t) * B * n);
   fill(P->fcnt, sizeof(int32_t) * 2);
-  fill(P->nspk, sizeof(int32_t) * n);
-  fill(P->dirty, P->dirty_stride * P->bucket_halves);
-  fill(P->bits, sizeof(uint32_t) * P->n_bits_words * P->bucket_halves);
-  fill(P->blk_spikes, sizeof(unsigned long long) * P->n_blk_slots);
-  fill(P->blk_events, sizeof(unsigned long long) * P->n_blk_slots);
-  if (raster) fill(P->raster, sizeof(uint32_t) * (size_t)P->T * P->a.words);
-  if (!rc) rc = fwd_launch_first_injection(P->core, P->a);
+  fill(P->nspk, sizeof(int32_t) * B * n);
+  fill(P->dirty, P->dirty_total);
+  fill(P->bits, sizeof(uint32_t) * P->bits_total);
+  fill(P->blk_spikes, sizeof(unsigned long long) * B * P->n_blk_slots);
+  fill(P->blk_events, sizeof(unsigned long long) * B * P->n_blk_slots);
+  if (raster) fill(P->raster, sizeof(uint32_t) * B * (size_t)P->T * P->a.words);
+  if (!rc && (P->persistent || P->variant == SNC_VARIANT_SORT))
+    rc = fwd_launch_first_injection(P->core, P->a);
   return rc;
 }
 
 int ensure_raster(snc_hip_plan* P) {
   if (P->raster) return SNC_OK;
-  return dalloc(&P->raster, (size_t)std::max(1, P->T) * P->a.words);
+  return dalloc(&P->raster, (size_t)P->B * std::max(1, P->T) * P->a.words);
 }
 
 // The step chain. While capturing, each step's hub kernel runs on a forked
@@ -661,16 +739,25 @@ int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stat
   (void)hipEventElapsedTime(&ms, P->ev0, P->ev1);
   P->last_ms = ms;
 
-  const int n = g.n;
-  std::vector<unsigned long long> bs(P->n_blk_slots), be(P->n_blk_slots);
-  std::vector<int32_t> nspk(n);
+  const size_t n = (size_t)g.n, B = (size_t)P->B, slots = P->n_blk_slots;
+  std::vector<unsigned long long> bs(B * slots), be(B * slots);
+  std::vector<int32_t> nspk(B * n);
   if (hipMemcpy(bs.data(), P->blk_spikes, bs.size() * sizeof(bs[0]), hipMemcpyDeviceToHost) != hipSuccess ||
       hipMemcpy(be.data(), P->blk_events, be.size() * sizeof(be[0]), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(nspk.data(), P->nspk, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      hipMemcpy(nspk.data(), P->nspk, sizeof(int32_t) * B * n, hipMemcpyDeviceToHost) != hipSuccess)
     return fail(SNC_E_HIP, "result readback failed");
   std::memset(stats, 0, sizeof(*stats));
-  for (auto x : bs) stats->spikes += (int64_t)x;
-  for (auto x : be) stats->synaptic_events += (int64_t)x;
+  P->last_spikes.assign(B, 0);
+  P->last_events.assign(B, 0);
+  for (size_t b = 0; b < B; ++b)
+    for (size_t k = 0; k < slots; ++k) {
+      P->last_spikes[b] += (int64_t)bs[b * slots + k];
+      P->last_events[b] += (int64_t)be[b * slots + k];
+    }
+  for (size_t b = 0; b < B; ++b) {
+    stats->spikes += P->last_spikes[b];
+    stats->synaptic_events += P->last_events[b];
+  }
   stats->used_variant = P->variant;
   stats->ms = ms;
   if (cap) {
@@ -678,27 +765,40 @@ int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stat
     if (cap->logits) {
       // Output spikes summed per channel in float, as the CPU decode does
       // (cpu_backend.cpp:149-151); exact while counts stay below 2^24.
-      std::fill(cap->logits, cap->logits + g.n_out_channels, 0.0f);
-      for (int i = 0; i < n; ++i)
-        if (g.role[i] == SNC_ROLE_OUTPUT && g.channel[i] >= 0)
-          cap->logits[g.channel[i]] += (float)nspk[i];
+      const size_t C = (size_t)g.n_out_channels;
+      std::fill(cap->logits, cap->logits + B * C, 0.0f);
+      for (size_t b = 0; b < B; ++b)
+        for (size_t i = 0; i < n; ++i)
+          if (g.role[i] == SNC_ROLE_OUTPUT && g.channel[i] >= 0)
+            cap->logits[b * C + g.channel[i]] += (float)nspk[b * n + i];
     }
     if (cap->final_v &&
-        hipMemcpy(cap->final_v, P->v, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        hipMemcpy(cap->final_v, P->v, sizeof(float) * B * n, hipMemcpyDeviceToHost) != hipSuccess)
       return fail(SNC_E_HIP, "voltage readback failed");
     if (want_raster) {
-      std::vector<uint32_t> r((size_t)P->T * P->a.words);
+      const size_t W = (size_t)P->a.words, T = (size_t)P->T;
+      std::vector<uint32_t> r(B * T * W);
       if (!r.empty() && hipMemcpy(r.data(), P->raster, r.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
         return fail(SNC_E_HIP, "raster readback failed");
       if (cap->raster) std::copy(r.begin(), r.end(), cap->raster);
       if (cap->step_spikes)
-        for (int t = 0; t < P->T; ++t) {
+        for (size_t bt = 0; bt < B * T; ++bt) {
           int64_t c = 0;
-          for (int w = 0; w < P->a.words; ++w) c += __builtin_popcount(r[(size_t)t * P->a.words + w]);
-          cap->step_spikes[t] = c;
+          for (size_t w = 0; w < W; ++w) c += __builtin_popcount(r[bt * W + w]);
+          cap->step_spikes[bt] = c;
         }
     }
   }
+  return SNC_OK;
+}
+
+int32_t snc_hip_plan_num_samples(const snc_hip_plan* p) { return p ? p->B : -1; }
+
+int snc_hip_plan_sample_stats(const snc_hip_plan* p, int64_t* spikes, int64_t* events) {
+  if (!p || !spikes || !events) return fail(SNC_E_INVALID, "null argument");
+  if (p->last_spikes.size() != (size_t)p->B) return fail(SNC_E_INVALID, "the plan has not run");
+  std::copy(p->last_spikes.begin(), p->last_spikes.end(), spikes);
+  std::copy(p->last_events.begin(), p->last_events.end(), events);
   return SNC_OK;
 }
 
@@ -788,27 +888,29 @@ int snc_hip_forward(const snc_graph_view* gv, const snc_event* events, int64_t n
 
 }  // extern "C"
 
-// The autotuner's search (cached per graph): every candidate runs the first
-// trial_steps steps of this sample twice (the first run captures and
-// instantiates its graph); the device time of the second run decides.
+// The autotuner's search (cached per graph, separately for single-sample
+// and batched plans): every candidate runs the first trial_steps steps of
+// this input twice (the first run captures and instantiates its graph); the
+// device time of the second run decides.
 static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_events,
-                           int num_steps, const snc_lif* lif, bool exact_only,
+                           int num_samples, int num_steps, const snc_lif* lif, bool exact_only,
                            snc_plan_config* out) {
-  const int k = exact_only ? 1 : 0;
+  const int k = (exact_only ? 1 : 0) + (num_samples > 1 ? 2 : 0);
   if (G->tuned[k]) {
     *out = G->tune[k].cand[G->tune[k].chosen];
     return SNC_OK;
   }
   const auto t0 = std::chrono::steady_clock::now();
   const std::vector<snc_plan_config> cands =
-      autotune_candidates(G->g, exact_only, persist_supported());
+      autotune_candidates(G->g, exact_only, num_samples == 1 && persist_supported(),
+                          num_samples == 1);
   snc_tune_report rep{};
   rep.trial_steps = autotune_trial_steps(num_steps);
   int best = -1;
   double best_ms = 0.0;
   for (const snc_plan_config& c : cands) {
     snc_hip_plan* P = nullptr;
-    int rc = snc_hip_plan_create_config(G, events, n_events, rep.trial_steps, lif, &c, &P);
+    int rc = plan_build(G, events, n_events, num_samples, rep.trial_steps, lif, &c, &P);
     snc_forward_stats st{};
     if (!rc) rc = snc_hip_plan_run(P, nullptr, &st);
     if (!rc) rc = snc_hip_plan_run(P, nullptr, &st);

@@ -56,7 +56,7 @@ struct DevGraph {
 
 struct snc_hip_graph {
   snc_amd::DevGraph g;
-  // Autotuner results, [0] over all strategies, [1] over the bit-exact ones.
-  bool tuned[2] = {false, false};
-  snc_tune_report tune[2] = {};
+  // Autotuner results: [exact_only + 2 * batched].
+  bool tuned[4] = {false, false, false, false};
+  snc_tune_report tune[4] = {};
 };

@@ -66,9 +66,39 @@ struct StepArgs {
   int n_reg_blocks;
   int blk;                     // neurons per block of the fused step kernel
   int n_heavy;
-  int64_t bits_half;           // words of one bitmap half (step parity)
-  int64_t dirty_half;          // bytes of one dirty-flag half
+  int64_t bits_half;           // persistent: words of one bitmap half (step parity)
+  int64_t dirty_half;          // persistent: bytes of one dirty-flag half
+  int64_t bits_slot;           // fused hip-bucket: words of one arrival bitmap (per ring slot)
+  int64_t dirty_slot;          // fused hip-bucket: bytes of one arrival-flag array
+  // batched plans: samples are grid.y; per-sample strides of the state
+  int B;
+  int64_t ring_stride;    // L * n
+  int64_t bits_stride;    // both bitmap halves of one sample
+  int64_t dirty_stride;   // both dirty halves of one sample
+  int64_t blk_stride;     // stats slots of one sample
+  int64_t inj_stride;     // T * (n_reg_blocks + 1)
+  int64_t hinj_stride;    // T * n_heavy
+  int64_t raster_stride;  // T * words
 };
+
+// The kernel arguments of sample `smp` of a batched plan (uniform offsets).
+__host__ __device__ inline StepArgs sample_args(const StepArgs& a0, int smp) {
+  StepArgs a = a0;
+  if (smp) {
+    a.v += (size_t)smp * a.n;
+    a.refr += (size_t)smp * a.n;
+    a.nspk += (size_t)smp * a.n;
+    if (a.ring) a.ring += (size_t)smp * a.ring_stride;
+    if (a.bits) a.bits += (size_t)smp * a.bits_stride;
+    if (a.dirty) a.dirty += (size_t)smp * a.dirty_stride;
+    a.blk_spikes += (size_t)smp * a.blk_stride;
+    a.blk_events += (size_t)smp * a.blk_stride;
+    a.inj_blk_ptr += (size_t)smp * a.inj_stride;
+    if (a.inj_heavy) a.inj_heavy += (size_t)smp * a.hinj_stride;
+    if (a.raster) a.raster += (size_t)smp * a.raster_stride;
+  }
+  return a;
+}
 
 struct FwdPlanCore {
   hipStream_t stream = nullptr;
@@ -80,10 +110,7 @@ struct FwdPlanCore {
 };
 
 int fwd_launch_fill(hipStream_t st, void* p, uint32_t value, size_t words);
-int fwd_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t);
-int fwd_launch_heavy(const FwdPlanCore& pc, const StepArgs& a, int t);
-int fwd_launch_lif(const FwdPlanCore& pc, const StepArgs& a, int t);
-int fwd_launch_expand(const FwdPlanCore& pc, const StepArgs& a, int t);
+int fwd_launch_lif(const FwdPlanCore& pc, const StepArgs& a, int t);  // hip-sort
 int fwd_launch_first_injection(const FwdPlanCore& pc, const StepArgs& a);
 
 // fused per-step kernels (step.hip); hub_stream may be null (same stream)

@@ -1,25 +1,32 @@
 // Fused per-step kernels of hip-atomic and hip-bucket: one launch per
-// timestep does fold + integrate/fire + expansion for every neuron.
+// timestep does delivery-in + integrate/fire + expansion for every neuron.
 //
 // A block owns BLOCK consecutive neurons (128, 256 or 512: a launch-shape
-// choice of the autotuner), one per thread. Within step t it
-//   1. stages its own injections of step t in LDS (per-(step, block) ranges),
-//   2. [hip-bucket] folds the deposits made at step t-1 into its targets:
-//      the target's column words are loaded together, set bits are walked in
-//      ascending column position; values bound for slot t % L accumulate in a
-//      register on top of the slot value, others go to their ring slot,
-//   3. integrates/fires (ext, slot t % L read and cleared),
-//   4. expands its fired rows, one wave per row: float atomics into slots
-//      (t + d) % L (hip-atomic) or presence bits of step parity t & 1
-//      (hip-bucket).
-// No two blocks of one launch touch the same ring slot, bitmap half or dirty
-// half: the ring has max_delay + 1 slots, so slot t % L never receives a
-// deposit of step t, and the bucket state is double-buffered by step parity.
+// choice of the autotuner), one per thread; grid.y is the sample of a batched
+// plan. Within step t a block
+//   1. issues every load that depends only on the neuron (state, row
+//      pointers, column bounds, arrival flag) before its first barrier,
+//   2. stages its own injections of step t in LDS (per-(step, block) ranges),
+//   3. gathers the input arriving at step t:
+//        hip-atomic  the ring slot t % L (read and cleared);
+//        hip-bucket  the target's column bits in the arrival bitmap of step t,
+//                    walked in column order (delay descending, pre, CSR
+//                    position), their values summed from 0 -- the CPU's
+//                    ring-slot sum, bit for bit -- then cleared,
+//   4. integrates/fires,
+//   5. expands its fired rows, one wave per row, lanes striding the row:
+//        hip-atomic  float atomics into ring slots (t + d) % L;
+//        hip-bucket  presence bits in the arrival bitmap of step t + d and the
+//                    target's arrival flag for that step.
+// Arrival state of step t is only read (and cleared) at step t and only
+// written at steps t - L + 1 .. t - 1 (delays are 1 .. L - 1), so within one
+// launch readers and writers never touch the same bitmap or ring slot.
 //
-// Heavy targets (in-degree above the light limit) are stepped by
-// k_hub_step, one block per hub, which may run concurrently with k_step: the
-// block folds the hub's column in ordered rounds (block scan, LDS list, one
-// lane per ring slot), integrates the hub and expands its row.
+// Heavy targets (in-degree above the hub threshold) are stepped by
+// k_hub_step, one block per hub, concurrently with k_step: the block loads
+// the hub's arrival words in one round (up to 2048 words), orders the set
+// entries by a block scan into an LDS list, gathers their values in one
+// round of independent loads, and one lane sums them in column order.
 
 #include <hip/hip_runtime.h>
 
@@ -31,9 +38,10 @@
 namespace snc_amd {
 
 constexpr int kStepThreads = 256;
-constexpr int kLightMaxWords = 16;  // light targets: in-degree <= 512
-constexpr int kHubRoundWords = 256; // column words per hub-fold round (one per thread)
-constexpr int kHubCap = kHubRoundWords * 32;
+constexpr int kLightMaxWords = 16;     // light targets: in-degree <= 512
+constexpr int kHubWordsPerThread = 8;  // column words a hub-block thread loads per round
+constexpr int kHubSuperWords = kStepThreads * kHubWordsPerThread;
+constexpr int kHubCap = 8192;          // LDS entries of one gather pass
 
 __device__ __forceinline__ void raster_or(uint32_t* row, int first, uint64_t m, int words) {
   const int w0 = first >> 5;
@@ -41,19 +49,17 @@ __device__ __forceinline__ void raster_or(uint32_t* row, int first, uint64_t m, 
   if ((uint32_t)(m >> 32) && w0 + 1 < words) atomicOr(row + w0 + 1, (uint32_t)(m >> 32));
 }
 
-// Expansion of one fired row by one wave.
+// Expansion of CSR row [rb, re) by one wave.
 template <bool BUCKET>
-__device__ __forceinline__ int expand_row(const StepArgs& a, int i, int tm, uint32_t* bits_now,
-                                          uint8_t* dirty_now, int lane) {
-  const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
+__device__ __forceinline__ int expand_row(const StepArgs& a, int rb, int re, int tm, int lane) {
   for (int e = rb + lane; e < re; e += kWave) {
+    int s = tm + (int)a.delay[e];
+    if (s >= a.L) s -= a.L;
     if (BUCKET) {
       const int p = a.cpos[e];
-      atomicOr(bits_now + (p >> 5), 1u << (p & 31));
-      dirty_now[a.post[e]] = 1;
+      atomicOr(a.bits + (size_t)s * a.bits_slot + (p >> 5), 1u << (p & 31));
+      a.dirty[(size_t)s * a.dirty_slot + a.post[e]] = 1;
     } else {
-      int s = tm + (int)a.delay[e];
-      if (s >= a.L) s -= a.L;
       atomicAdd(a.ring + (size_t)s * a.n + a.post[e], a.wsig[e]);
     }
   }
@@ -61,16 +67,38 @@ __device__ __forceinline__ int expand_row(const StepArgs& a, int i, int tm, uint
 }
 
 template <bool BUCKET, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_step(StepArgs a, int t, int tm) {
+__global__ __launch_bounds__(BLOCK) void k_step(StepArgs a0, int t, int tm) {
+  const StepArgs a = sample_args(a0, blockIdx.y);
   __shared__ float s_ext[BLOCK];
-  __shared__ int32_t s_fired[BLOCK];
+  __shared__ int2 s_rows[BLOCK];  // CSR bounds of the block's fired rows
   __shared__ int s_nf;
   __shared__ unsigned long long s_ev;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const int b = blockIdx.x;
   const int base = b * BLOCK;
   const int j = base + tid;
-  const int n = a.n, L = a.L;
+  const int n = a.n;
+
+  bool mine = j < n;
+  float v = 0.0f, slot = 0.0f;
+  int r = 0, rb = 0, re = 0, c0 = 0, c1 = 0;
+  uint8_t arrived = 0;
+  float* slotp = BUCKET ? nullptr : a.ring + (size_t)tm * n + j;
+  uint8_t* dnow = BUCKET ? a.dirty + (size_t)tm * a.dirty_slot : nullptr;
+  if (mine) {
+    if (BUCKET) mine = a.is_heavy[j] == 0;
+    v = a.v[j];
+    r = a.refr[j];
+    rb = a.row_ptr[j];
+    re = a.row_ptr[j + 1];
+    if (BUCKET) {
+      arrived = dnow[j];
+      c0 = a.ccol[j];
+      c1 = a.ccol[j + 1];
+    } else {
+      slot = *slotp;
+    }
+  }
   s_ext[tid] = 0.0f;
   if (tid == 0) {
     s_nf = 0;
@@ -82,54 +110,33 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a, int t, int tm) {
   __syncthreads();
 
   bool f = false;
-  if (j < n && !(BUCKET && a.is_heavy[j])) {
-    float v = a.v[j];
-    int r = a.refr[j];
-    float* slotp = a.ring + (size_t)tm * n + j;
-    float slot = *slotp;
-    if (BUCKET && t > 0) {
-      uint8_t* dprev = a.dirty + (size_t)((t - 1) & 1) * a.dirty_half;
-      if (dprev[j]) {
-        dprev[j] = 0;
-        uint32_t* bprev = a.bits + (size_t)((t - 1) & 1) * a.bits_half;
-        const int tprev = tm == 0 ? L - 1 : tm - 1;
-        const int w0 = a.ccol[j] >> 5, nw = (a.ccol[j + 1] >> 5) - w0;
-        uint32_t wv[kLightMaxWords];
+  if (mine) {
+    if (BUCKET && arrived) {
+      dnow[j] = 0;
+      uint32_t* bnow = a.bits + (size_t)tm * a.bits_slot;
+      const int w0 = c0 >> 5, nw = (c1 >> 5) - w0;
+      uint32_t wv[kLightMaxWords];
 #pragma unroll
-        for (int k = 0; k < kLightMaxWords; ++k) wv[k] = k < nw ? bprev[w0 + k] : 0u;
+      for (int k = 0; k < kLightMaxWords; ++k) wv[k] = k < nw ? bnow[w0 + k] : 0u;
 #pragma unroll
-        for (int k = 0; k < kLightMaxWords; ++k)
-          if (wv[k]) bprev[w0 + k] = 0u;
-        int cur = -1;  // run of a slot other than tm, kept in a register
-        float acc = 0.0f;
+      for (int k = 0; k < kLightMaxWords; ++k)
+        if (wv[k]) bnow[w0 + k] = 0u;
+      float acc = 0.0f;  // the cleared slot the CPU adds onto
 #pragma unroll
-        for (int k = 0; k < kLightMaxWords; ++k) {
-          uint32_t bw = wv[k];
-          while (bw) {
-            const int p = ((w0 + k) << 5) | __builtin_ctz(bw);
-            bw &= bw - 1u;
-            int s = tprev + (int)a.cdel[p];
-            if (s >= L) s -= L;
-            const float x = a.cval[p];
-            if (s == tm) {
-              slot = __fadd_rn(slot, x);
-            } else {
-              if (s != cur) {
-                if (cur >= 0) a.ring[(size_t)cur * n + j] = acc;
-                cur = s;
-                acc = a.ring[(size_t)s * n + j];
-              }
-              acc = __fadd_rn(acc, x);
-            }
-          }
+      for (int k = 0; k < kLightMaxWords; ++k) {
+        uint32_t bw = wv[k];
+        while (bw) {
+          const int p = ((w0 + k) << 5) | __builtin_ctz(bw);
+          bw &= bw - 1u;
+          acc = __fadd_rn(acc, a.cval[p]);
         }
-        if (cur >= 0) a.ring[(size_t)cur * n + j] = acc;
       }
+      slot = acc;
     }
     f = lif_step(v, r, s_ext[tid], slot, a.lif);
     a.v[j] = v;
     a.refr[j] = r;
-    if (__float_as_uint(slot) != 0u) *slotp = 0.0f;
+    if (!BUCKET && __float_as_uint(slot) != 0u) *slotp = 0.0f;
     if (f) a.nspk[j] += 1;
   }
   const uint64_t m = ballot(f);
@@ -137,16 +144,16 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a, int t, int tm) {
     int pos = 0;
     if (lane == 0) pos = atomicAdd(&s_nf, __popcll(m));
     pos = __shfl(pos, 0, kWave);
-    if (f) s_fired[pos + rank_below(m)] = j;
+    if (f) s_rows[pos + rank_below(m)] = make_int2(rb, re);
     if (a.raster && lane == 0) raster_or(a.raster + (size_t)t * a.words, j - lane, m, a.words);
   }
   __syncthreads();
   const int nf = s_nf;
-  uint32_t* bits_now = BUCKET ? a.bits + (size_t)(t & 1) * a.bits_half : nullptr;
-  uint8_t* dirty_now = BUCKET ? a.dirty + (size_t)(t & 1) * a.dirty_half : nullptr;
   unsigned long long ev = 0;
-  for (int k = wave; k < nf; k += BLOCK / kWave)
-    ev += (unsigned long long)expand_row<BUCKET>(a, s_fired[k], tm, bits_now, dirty_now, lane);
+  for (int k = wave; k < nf; k += BLOCK / kWave) {
+    const int2 row = s_rows[k];
+    ev += (unsigned long long)expand_row<BUCKET>(a, row.x, row.y, tm, lane);
+  }
   if (lane == 0 && ev) atomicAdd(&s_ev, ev);
   __syncthreads();
   if (tid == 0) {
@@ -156,92 +163,97 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a, int t, int tm) {
 }
 
 // One block per heavy target (hip-bucket).
-__global__ __launch_bounds__(kStepThreads) void k_hub_step(StepArgs a, int t, int tm) {
-  __shared__ float s_val[kHubCap];
-  __shared__ uint8_t s_slot[kHubCap];
+__global__ __launch_bounds__(kStepThreads) void k_hub_step(StepArgs a0, int t, int tm) {
+  const StepArgs a = sample_args(a0, blockIdx.y);
+  // s_val holds an entry's column position until the gather replaces it by
+  // the entry's value (same thread, same index).
+  __shared__ __align__(16) float s_val[kHubCap];
   __shared__ int s_wsum[kStepThreads / kWave + 1];
   __shared__ int s_fire;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const int hidx = blockIdx.x;
   const int h = a.heavy_ids[hidx];
-  const int n = a.n, L = a.L;
   const int slot_base = a.n_reg_blocks;  // stats slots after the regular blocks
-  if (t > 0) {
-    uint8_t* dprev = a.dirty + (size_t)((t - 1) & 1) * a.dirty_half;
-    if (dprev[h]) {  // same value for every thread of the block
-      uint32_t* bprev = a.bits + (size_t)((t - 1) & 1) * a.bits_half;
-      const int tprev = tm == 0 ? L - 1 : tm - 1;
-      float acc[kMaxSlotsPerLane];
-      if (wave == 0)
+  uint8_t* dnow = a.dirty + (size_t)tm * a.dirty_slot;
+  // neuron-only loads first (the same addresses in every lane)
+  const uint8_t arrived = dnow[h];
+  const int c0 = a.ccol[h], c1 = a.ccol[h + 1];
+  float v = a.v[h];
+  int r = a.refr[h];
+  const float x_in = a.inj_heavy[(size_t)t * a.n_heavy + hidx];
+  float in = 0.0f;  // sum of the arriving values, kept by thread 0
+  if (arrived) {    // same value for every thread of the block
+    uint32_t* bnow = a.bits + (size_t)tm * a.bits_slot;
+    const int w0 = c0 >> 5, w1 = c1 >> 5;
+    for (int sb = w0; sb < w1; sb += kHubSuperWords) {  // uniform trip count
+      const int nw = min(kHubSuperWords, w1 - sb);
+      const int per = (nw + kStepThreads - 1) / kStepThreads;
+      const int my0 = sb + tid * per;
+      const int my1 = min(sb + nw, my0 + per);
+      uint32_t wv[kHubWordsPerThread];
+      int c = 0;
 #pragma unroll
-        for (int k = 0; k < kMaxSlotsPerLane; ++k) {
-          const int s = lane + k * kWave;
-          acc[k] = s < L ? a.ring[(size_t)s * n + h] : 0.0f;
+      for (int k = 0; k < kHubWordsPerThread; ++k) {
+        wv[k] = my0 + k < my1 ? bnow[my0 + k] : 0u;
+        c += __popc(wv[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < kHubWordsPerThread; ++k)
+        if (wv[k]) bnow[my0 + k] = 0u;
+      const int incl = wave_inclusive_scan(c);
+      if (lane == kWave - 1) s_wsum[wave] = incl;
+      __syncthreads();
+      if (tid == 0) {
+        int run = 0;
+        for (int q = 0; q < kStepThreads / kWave; ++q) {
+          const int x = s_wsum[q];
+          s_wsum[q] = run;
+          run += x;
         }
-      const int w0 = a.ccol[h] >> 5, w1 = a.ccol[h + 1] >> 5;
-      for (int wb = w0; wb < w1; wb += kHubRoundWords) {  // uniform trip count
-        const int w = wb + tid;
-        uint32_t bw = 0;
-        if (w < w1) {
-          bw = bprev[w];
-          if (bw) bprev[w] = 0u;
+        s_wsum[kStepThreads / kWave] = run;
+      }
+      __syncthreads();
+      const int off = s_wsum[wave] + incl - c;
+      const int total = s_wsum[kStepThreads / kWave];
+      for (int p0 = 0; p0 < total; p0 += kHubCap) {  // uniform trip count
+        const int cnt = min(kHubCap, total - p0);
+        // (1) column positions of this pass's entries, in column order
+        if (c && off < p0 + kHubCap && off + c > p0) {
+          int o = off;
+#pragma unroll
+          for (int k = 0; k < kHubWordsPerThread; ++k) {
+            uint32_t bw = wv[k];
+            while (bw) {
+              if (o >= p0 && o < p0 + kHubCap)
+                s_val[o - p0] = __int_as_float(((my0 + k) << 5) | __builtin_ctz(bw));
+              bw &= bw - 1u;
+              ++o;
+            }
+          }
         }
-        // block-wide exclusive scan of the set-bit counts (word order = column order)
-        const int c = __popc(bw);
-        const int incl = wave_inclusive_scan(c);
-        if (lane == kWave - 1) s_wsum[wave] = incl;
         __syncthreads();
+        // (2) one round of independent value loads
+        for (int q = tid; q < cnt; q += kStepThreads) s_val[q] = a.cval[__float_as_int(s_val[q])];
+        __syncthreads();
+        // (3) the ordered sum, four entries per LDS read
         if (tid == 0) {
-          int run = 0;
-          for (int q = 0; q < kStepThreads / kWave; ++q) {
-            const int x = s_wsum[q];
-            s_wsum[q] = run;
-            run += x;
+          int q = 0;
+          for (; q + 4 <= cnt; q += 4) {
+            const float4 x = *(const float4*)&s_val[q];
+            in = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(in, x.x), x.y), x.z), x.w);
           }
-          s_wsum[kStepThreads / kWave] = run;
+          for (; q < cnt; ++q) in = __fadd_rn(in, s_val[q]);
         }
-        __syncthreads();
-        int o = s_wsum[wave] + incl - c;
-        const int total = s_wsum[kStepThreads / kWave];
-        while (bw) {
-          const int p = (w << 5) | __builtin_ctz(bw);
-          bw &= bw - 1u;
-          int s = tprev + (int)a.cdel[p];
-          if (s >= L) s -= L;
-          s_slot[o] = (uint8_t)s;
-          s_val[o] = a.cval[p];
-          ++o;
-        }
-        __syncthreads();
-        if (wave == 0)
-          for (int q = 0; q < total; ++q) {
-            const int s = s_slot[q];
-            const float x = s_val[q];
-#pragma unroll
-            for (int k = 0; k < kMaxSlotsPerLane; ++k)
-              if (s == lane + k * kWave) acc[k] = __fadd_rn(acc[k], x);
-          }
         __syncthreads();
       }
-      if (wave == 0)
-#pragma unroll
-        for (int k = 0; k < kMaxSlotsPerLane; ++k) {
-          const int s = lane + k * kWave;
-          if (s < L) a.ring[(size_t)s * n + h] = acc[k];
-        }
-      if (tid == 0) dprev[h] = 0;
-      __syncthreads();
+      __syncthreads();  // s_wsum is reused by the next super-round
     }
+    if (tid == 0) dnow[h] = 0;
   }
   if (tid == 0) {
-    float v = a.v[h];
-    int r = a.refr[h];
-    float* slotp = a.ring + (size_t)tm * n + h;
-    const float slot = *slotp;
-    const bool f = lif_step(v, r, a.inj_heavy[(size_t)t * a.n_heavy + hidx], slot, a.lif);
+    const bool f = lif_step(v, r, x_in, in, a.lif);
     a.v[h] = v;
     a.refr[h] = r;
-    if (__float_as_uint(slot) != 0u) *slotp = 0.0f;
     s_fire = f ? 1 : 0;
     if (f) {
       a.nspk[h] += 1;
@@ -251,26 +263,26 @@ __global__ __launch_bounds__(kStepThreads) void k_hub_step(StepArgs a, int t, in
   }
   __syncthreads();
   if (s_fire && wave == 0) {
-    const int deg = expand_row<true>(a, h, tm, a.bits + (size_t)(t & 1) * a.bits_half,
-                                     a.dirty + (size_t)(t & 1) * a.dirty_half, lane);
+    const int deg = expand_row<true>(a, a.row_ptr[h], a.row_ptr[h + 1], tm, lane);
     if (lane == 0) a.blk_events[slot_base + hidx] += (unsigned long long)deg;
   }
 }
 
 int fused_launch_hub(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_t st) {
   if (pc.variant == SNC_VARIANT_BUCKET && a.n_heavy > 0)
-    hipLaunchKernelGGL(k_hub_step, dim3(a.n_heavy), dim3(kStepThreads), 0, st, a, t, t % a.L);
+    hipLaunchKernelGGL(k_hub_step, dim3(a.n_heavy, a.B), dim3(kStepThreads), 0, st, a, t,
+                       t % a.L);
   return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "hub step launch failed");
 }
 
 template <int BLOCK>
 static void launch_step_block(const FwdPlanCore& pc, const StepArgs& a, int t, int tm) {
   if (pc.variant == SNC_VARIANT_BUCKET)
-    hipLaunchKernelGGL((k_step<true, BLOCK>), dim3(a.n_reg_blocks), dim3(BLOCK), 0, pc.stream, a,
-                       t, tm);
+    hipLaunchKernelGGL((k_step<true, BLOCK>), dim3(a.n_reg_blocks, a.B), dim3(BLOCK), 0,
+                       pc.stream, a, t, tm);
   else
-    hipLaunchKernelGGL((k_step<false, BLOCK>), dim3(a.n_reg_blocks), dim3(BLOCK), 0, pc.stream,
-                       a, t, tm);
+    hipLaunchKernelGGL((k_step<false, BLOCK>), dim3(a.n_reg_blocks, a.B), dim3(BLOCK), 0,
+                       pc.stream, a, t, tm);
 }
 
 int fused_launch_regular(const FwdPlanCore& pc, const StepArgs& a, int t) {

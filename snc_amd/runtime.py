@@ -137,11 +137,12 @@ class HipGraph:
         L.check(L.lib().snc_hip_graph_profile_get(self._h, C.byref(p)))
         return {k: getattr(p, k) for k, _ in p._fields_}
 
-    def tune_report(self, exact_only=False) -> dict:
+    def tune_report(self, exact_only=False, batched=False) -> dict:
         """The autotuner's search for this graph (after a hip-auto plan):
         candidates with their measured ms per step, and the chosen one."""
         r = L.TuneReport()
-        L.check(L.lib().snc_hip_graph_tune_report(self._h, 1 if exact_only else 0, C.byref(r)))
+        L.check(L.lib().snc_hip_graph_tune_report(self._h, 1 if exact_only else 0,
+                                                  1 if batched else 0, C.byref(r)))
         cands = [(PlanConfig.from_c(r.cand[i]), r.ms_per_step[i]) for i in range(r.n_candidates)]
         return {"candidates": cands, "chosen": r.chosen, "trial_steps": r.trial_steps,
                 "search_ms": r.search_ms}
@@ -159,21 +160,34 @@ class HipGraph:
 
 
 class ForwardPlan:
-    """Graph + one sample's events + device state, reusable across runs."""
+    """Graph + inputs + device state, reusable across runs.
+
+    num_samples=None: one sample (every event belongs to it). num_samples=B:
+    a batched plan of B independent samples stepped together; an event
+    belongs to sample ``sample_id``, and results carry a leading sample axis.
+    """
 
     def __init__(self, hg: HipGraph, events, num_steps: int, lif: LIFParams,
-                 backend: Backend = Backend.HipAuto, config: PlanConfig | None = None):
+                 backend: Backend = Backend.HipAuto, config: PlanConfig | None = None,
+                 num_samples: int | None = None):
         """backend: a hip-* strategy with its default launch shape, or hip-auto /
         hip-auto-exact (autotuned); config: an explicit PlanConfig instead."""
         if config is None and not backend_is_hip(backend):
             raise ValueError(f"{backend_name(backend)} is not a hip backend")
         self.hg = hg
         self.num_steps = int(num_steps)
+        self.batched = num_samples is not None
+        self.num_samples = int(num_samples) if self.batched else 1
         ev, p = event_ptr(events)
         self._events = ev
         self._h = C.c_void_p()
         lc = lif.c()
-        if config is not None:
+        if self.batched:
+            cc = config.c() if config is not None else None
+            L.check(L.lib().snc_hip_plan_create_batch(
+                hg._h, p, ev.size, self.num_samples, self.num_steps, C.byref(lc), int(backend),
+                C.byref(cc) if cc is not None else None, C.byref(self._h)))
+        elif config is not None:
             cc = config.c()
             L.check(L.lib().snc_hip_plan_create_config(hg._h, p, ev.size, self.num_steps,
                                                        C.byref(lc), C.byref(cc),
@@ -194,21 +208,35 @@ class ForwardPlan:
 
     def run(self, capture_raster=False, capture_v=False) -> ForwardResult:
         g = self.hg.graph
-        n = g.num_neurons
-        logits = np.zeros(max(g.num_output_channels, 1), dtype=np.float32)
-        nspk = np.zeros(n, dtype=np.int32)
+        n, B, T = g.num_neurons, self.num_samples, self.num_steps
+        C_out = max(g.num_output_channels, 1)
         words = (n + 31) // 32
-        raster = np.zeros((self.num_steps, words), dtype=np.uint32) if capture_raster else None
-        steps = np.zeros(self.num_steps, dtype=np.int64) if capture_raster else None
-        fv = np.zeros(n, dtype=np.float32) if capture_v else None
+        logits = np.zeros((B, C_out), dtype=np.float32)
+        nspk = np.zeros((B, n), dtype=np.int32)
+        raster = np.zeros((B, T, words), dtype=np.uint32) if capture_raster else None
+        steps = np.zeros((B, T), dtype=np.int64) if capture_raster else None
+        fv = np.zeros((B, n), dtype=np.float32) if capture_v else None
         cap = L.ForwardCapture(L.ptr(logits, C.c_float), L.ptr(nspk, C.c_int32),
                                L.ptr(raster, C.c_uint32), L.ptr(fv, C.c_float),
                                L.ptr(steps, C.c_int64))
         st = L.ForwardStats()
         L.check(L.lib().snc_hip_plan_run(self._h, C.byref(cap), C.byref(st)))
-        return ForwardResult(logits[: g.num_output_channels], RunStats(st.spikes, st.synaptic_events,
-                                                                       st.overflow, st.ms),
+        logits = logits[:, : g.num_output_channels]
+        if not self.batched:
+            logits, nspk = logits[0], nspk[0]
+            raster = raster[0] if raster is not None else None
+            steps = steps[0] if steps is not None else None
+            fv = fv[0] if fv is not None else None
+        return ForwardResult(logits, RunStats(st.spikes, st.synaptic_events, st.overflow, st.ms),
                              Backend(st.used_variant), nspk, raster, fv, steps)
+
+    def sample_stats(self):
+        """(spikes, synaptic_events) per sample of the last run."""
+        sp = np.zeros(self.num_samples, np.int64)
+        ev = np.zeros(self.num_samples, np.int64)
+        L.check(L.lib().snc_hip_plan_sample_stats(self._h, L.ptr(sp, C.c_int64),
+                                                  L.ptr(ev, C.c_int64)))
+        return sp, ev
 
     def launch(self):
         """Enqueue one run without synchronising (throughput timing)."""

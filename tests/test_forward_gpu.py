@@ -284,3 +284,72 @@ def test_autotuner_search_picks_fastest_and_stays_exact():
         _check_atomic(r3, ref)
     else:
         _check_exact(r3, ref, g.num_neurons)
+
+
+@pytest.mark.parametrize("backend", EXACT + [Backend.HipAtomic])
+def test_mega_hub_multi_round_fold(backend):
+    """A hub with in-degree 70,000 (2188 column words: two load rounds, eight
+    words per thread in the first) and a second with 20,000, where every
+    input fires in the same step (70,000 set bits: nine LDS passes of the
+    ordered fold), with delays 1..20 so the entries spread over ring slots."""
+    rng = np.random.default_rng(11)
+    n_in = 70000
+    edges = []
+    for a in range(n_in):
+        edges.append((a, n_in, float(rng.normal(0, 0.01)), int(rng.integers(1, 21))))
+        if a % 7 < 2:
+            edges.append((a, n_in + 1, float(rng.normal(0, 0.02)), int(rng.integers(1, 4))))
+    g = make_graph(n_in + 2, edges, [1] * n_in + [0, 0], list(range(n_in)) + [-1, -1])
+    ev = [(c, 0, 1.5) for c in range(n_in)]
+    ev += [(int(c), 3, 1.5) for c in rng.choice(n_in, 20000, replace=False)]
+    lif = LIFParams(1.0, 0.95, 0.0, 1)
+    r = forward(g, events(ev), 30, lif, backend, capture_raster=True, capture_v=True)
+    ref = O.forward(g, events(ev), 30, lif)
+    assert ref["spikes"] >= 90000
+    if backend == Backend.HipAtomic:
+        _check_atomic(r, ref)
+    else:
+        _check_exact(r, ref, g.num_neurons)
+
+
+@pytest.mark.parametrize("cfg", [PlanConfig(Backend.HipBucket, 256, 512),
+                                 PlanConfig(Backend.HipBucket, 128, 64),
+                                 PlanConfig(Backend.HipAtomic, 512, 512),
+                                 None])
+def test_batched_forward_equals_one_sample_at_a_time(cfg):
+    """Five samples with different drives in one batched plan (grid.y =
+    sample): each sample's raster, counts, logits and membranes equal the
+    oracle's single-sample forward (atomic: counts, voltages within 1e-5);
+    cfg None takes the autotuner's batched search."""
+    g = G.hub_sheet_graph(60, 50, 40, hub_fraction=0.02, hub_min=40, hub_max=1500,
+                          w_std=0.06, max_delay=20, seed=9)
+    from snc_amd import inputs as I
+    lif = K.lif_tau20()
+    T, B = 90, 5
+    evs = [I.background_drive(g.num_neurons, T, 0.01 + 0.004 * b, 1.0, seed=20 + b)
+           for b in range(B)]
+    allev = np.concatenate([e.copy() for e in evs])
+    off = 0
+    for b, e in enumerate(evs):
+        allev["sample_id"][off:off + e.size] = b
+        off += e.size
+    hg = HipGraph(g)
+    plan = ForwardPlan(hg, allev, T, lif, Backend.HipAuto, config=cfg, num_samples=B)
+    r = plan.run(capture_raster=True, capture_v=True)
+    sp, ev = plan.sample_stats()
+    assert r.raster.shape == (B, T, (g.num_neurons + 31) // 32)
+    assert sp.sum() == r.stats.spikes and ev.sum() == r.stats.synaptic_events
+    variant = plan.variant
+    for b in range(B):
+        ref = O.forward(g, evs[b], T, lif)
+        assert sp[b] == ref["spikes"] and ev[b] == ref["events"]
+        assert np.array_equal(r.neuron_spikes[b], ref["neuron_spikes"])
+        if variant == Backend.HipAtomic:
+            np.testing.assert_allclose(r.final_v[b], ref["final_v"], rtol=V_RTOL, atol=V_ATOL)
+        else:
+            assert np.array_equal(r.raster[b], ref["raster"])
+            assert np.array_equal(r.final_v[b].view(np.uint32), ref["final_v"].view(np.uint32))
+            assert np.array_equal(r.logits[b], ref["logits"])
+    if cfg is None:
+        rep = hg.tune_report(batched=True)
+        assert all(c.variant != Backend.HipSort and not c.persistent for c, _ in rep["candidates"])
