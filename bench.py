@@ -62,6 +62,22 @@ def kernel_units(g, heavy_threshold, neuron_spikes, steps):
     }
 
 
+def measured_traffic(kernel_symbol):
+    """HBM bytes per dispatch of a kernel from the committed counter summaries
+    (profiles/*pmc_traffic*.md: calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    import glob
+    import re
+
+    pat = re.compile(r"<!-- traffic_bytes_per_dispatch (.+) (\S+) (\d+) -->")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc_traffic*.md"))):
+        for line in open(path):
+            m = pat.search(line)
+            if m and m.group(1) == kernel_symbol:
+                return {"bytes_per_launch": float(m.group(3)),
+                        "source": os.path.relpath(path, ROOT)}
+    return None
+
+
 def model_bytes(neuron_steps, events, fired):
     return 16.0 * neuron_steps + 9.0 * events + 8.0 * fired
 
@@ -281,6 +297,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", default="hip-auto")
+    ap.add_argument("--block", type=int, default=0,
+                    help="with an explicit --variant: neurons per step-kernel block")
+    ap.add_argument("--hub-threshold", type=int, default=512,
+                    help="with --block: hip-bucket hub in-degree threshold")
     ap.add_argument("--compare", default="hip-atomic,hip-bucket,hip-sort,hip-auto-exact",
                     help="other variants to time (reported under 'variants')")
     ap.add_argument("--width", type=int, default=500)
@@ -310,7 +330,8 @@ def main():
         sys.exit(2)
 
     from snc_amd import configs as K
-    from snc_amd.runtime import ForwardPlan, HipGraph, available, parse_backend, synchronize
+    from snc_amd.runtime import (ForwardPlan, HipGraph, PlanConfig, available, parse_backend,
+                                 synchronize)
 
     if not available():
         raise SystemExit("HIP backend unavailable (no gfx950 device or library not built)")
@@ -319,7 +340,7 @@ def main():
         res = {}
         t = time.time()
         res["c2"] = eprop_bench(K.c2(), args.eprop_epochs, args.plain_batches, 1)
-        res["c3"] = eprop_bench(K.c3(), 1, args.plain_batches, 1)
+        res["c3"] = eprop_bench(K.c3(), max(3, args.eprop_epochs), args.plain_batches, 1)
         log(f"e-prop measurements: {time.time() - t:.1f}s")
         return res
 
@@ -327,14 +348,24 @@ def main():
         print(json.dumps({"eprop": eprop_all()}))
         return
     if args.pmc_probe:
-        import torch
+        # A plain device-to-device copy of 256 MiB through the HIP runtime
+        # (its copy kernel): 256 MiB read + 256 MiB written per dispatch.
+        import ctypes as C
 
-        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
-        y = torch.empty_like(x)
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        hip.hipFree.argtypes = [C.c_void_p]
+        nbytes = 256 << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(src), nbytes) or hip.hipMalloc(C.byref(dst), nbytes):
+            raise SystemExit("hipMalloc failed")
         for _ in range(3):
-            y.copy_(x)  # 256 MiB read + 256 MiB written per dispatch
-        torch.cuda.synchronize()
-        del x, y
+            if hip.hipMemcpy(dst, src, nbytes, 3):  # hipMemcpyDeviceToDevice
+                raise SystemExit("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
     t0 = time.time()
     w = K.c1(args.width, args.height, args.timesteps)
     g = w.graph
@@ -343,6 +374,9 @@ def main():
     hg = HipGraph(g)
 
     def make_plan(name):
+        if args.block and name == args.variant and not name.startswith("hip-auto"):
+            cfg = PlanConfig(parse_backend(name), args.block, args.hub_threshold, False)
+            return ForwardPlan(hg, w.samples[0], w.num_steps, w.lif, config=cfg)
         return ForwardPlan(hg, w.samples[0], w.num_steps, w.lif, parse_backend(name))
 
     t_plan = time.time()
@@ -390,9 +424,15 @@ def main():
                          "gbs": (by / (avg / 1e3) / 1e9) if avg > 0 else 0.0}
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     dk = kernels[dom]
+    cfgp = plan.config
+    sym = (f"snc_amd::k_step<{'true' if cfgp.variant == parse_backend('hip-bucket') else 'false'}, "
+           f"{cfgp.block_neurons}>" if dom == "k_step" else f"snc_amd::{dom}")
+    traffic = measured_traffic(sym)
     step_bytes = 16.0 * g.num_neurons * w.num_steps + 9.0 * events + 8.0 * spikes
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": dk["gbs"] / HBM_PEAK_GBS,
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
+                "traffic_source": traffic["source"] if traffic else None,
                 "bytes_per_launch": dk["bytes_per_launch"], "avg_launch_ms": dk["avg_ms"],
                 "kernels": kernels,
                 "whole_run": {"bytes": step_bytes, "ms": ms_per_step,

@@ -171,7 +171,9 @@ int snc_hip_mlp_score(snc_hip_mlp* m, const float* features, int64_t M, float* s
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
     cus = prop.multiProcessorCount;
   const int64_t tiles = (M + 15) / 16;
-  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, (int64_t)cus * 8));
+  // 4 waves per block, several rounds of prefetched tiles per wave; about
+  // as many blocks as stay resident (three per CU at the kernel's register use).
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 15) / 16, (int64_t)cus * 3));
   (void)hipEventRecord(m->ev0, m->stream);
   rc = mlp_launch_score(m->dev(), m->X, M, m->out, blocks, m->stream);
   if (rc) return rc;

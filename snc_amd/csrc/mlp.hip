@@ -31,6 +31,8 @@
 namespace snc_amd {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kScoreDepth = 4;  // candidate tiles whose loads a wave keeps in flight
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
@@ -84,42 +86,51 @@ __global__ __launch_bounds__(256) void k_mlp_score(const MlpDev p, const float* 
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int64_t tiles = (M + 15) / 16;
-  for (int64_t tile = wave; tile < tiles; tile += nwaves) {
-    const int64_t cand = tile * 16 + r;
-    const bool valid = cand < M;
-    // B operand of layer 1: X^T, lane holds features kq..kq+3 of candidate r.
-    bf16x4 bx;
-    if (valid) {
-      const float4 x4 = *reinterpret_cast<const float4*>(X + cand * kMlpIn + kq);
-      bx[0] = (short)f32_to_bf16(x4.x);
-      bx[1] = (short)f32_to_bf16(x4.y);
-      bx[2] = (short)f32_to_bf16(x4.z);
-      bx[3] = (short)f32_to_bf16(x4.w);
-    } else {
-      bx[0] = bx[1] = bx[2] = bx[3] = 0;
+  // Each wave takes kScoreDepth tiles per round and issues all their feature
+  // loads before the first MFMA, so one memory latency covers the round.
+  for (int64_t base = wave; base < tiles; base += kScoreDepth * nwaves) {
+    float4 xs[kScoreDepth];
+#pragma unroll
+    for (int u = 0; u < kScoreDepth; ++u) {
+      const int64_t cand = (base + u * nwaves) * 16 + r;
+      xs[u] = cand < M ? *reinterpret_cast<const float4*>(X + cand * kMlpIn + kq)
+                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    bf16x4 h1[4];
 #pragma unroll
-    for (int ob = 0; ob < 4; ++ob) {
-      f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ob], bx, c, 0, 0, 0);
+    for (int u = 0; u < kScoreDepth; ++u) {
+      const int64_t tile = base + u * nwaves;
+      if (tile >= tiles) break;  // uniform across the wave
+      const int64_t cand = tile * 16 + r;
+      // B operand of layer 1: X^T, lane holds features kq..kq+3 of candidate r.
+      bf16x4 bx;
+      bx[0] = (short)f32_to_bf16(xs[u].x);
+      bx[1] = (short)f32_to_bf16(xs[u].y);
+      bx[2] = (short)f32_to_bf16(xs[u].z);
+      bx[3] = (short)f32_to_bf16(xs[u].w);
+      bf16x4 h1[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) h1[ob][i] = (short)f32_to_bf16(fmaxf(c[i] + b1v[ob][i], 0.0f));
+      for (int ob = 0; ob < 4; ++ob) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ob], bx, c, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          h1[ob][i] = (short)f32_to_bf16(fmaxf(c[i] + b1v[ob][i], 0.0f));
+      }
+      float part = 0.0f;
+#pragma unroll
+      for (int ob = 0; ob < 4; ++ob) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb)
+          c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[ob][kb], h1[kb], c, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part += w3v[ob][i] * fmaxf(c[i] + b2v[ob][i], 0.0f);
+      }
+      // Lanes r, r+16, r+32, r+48 hold the four row quads of candidate r.
+      part += __shfl_xor(part, 16, kWave);
+      part += __shfl_xor(part, 32, kWave);
+      if (lane < 16 && cand < M) out[cand] = part + b3;
     }
-    float part = 0.0f;
-#pragma unroll
-    for (int ob = 0; ob < 4; ++ob) {
-      f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb)
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[ob][kb], h1[kb], c, 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) part += w3v[ob][i] * fmaxf(c[i] + b2v[ob][i], 0.0f);
-    }
-    // Lanes r, r+16, r+32, r+48 hold the four row quads of candidate r.
-    part += __shfl_xor(part, 16, kWave);
-    part += __shfl_xor(part, 32, kWave);
-    if (lane < 16 && valid) out[cand] = part + b3;
   }
 }
 
