@@ -1,0 +1,82 @@
+"""The command-line drivers on the GPU: the bench harness's self-test and its
+per-sample spike counts against the oracle's forward of the same graph and
+encodings; short training and co-training runs."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from snc_amd import graph as G
+from snc_amd import inputs as I
+from snc_amd.runtime import LIFParams
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "snc_amd", "bin")
+
+
+def _run(args, timeout=240):
+    print(" ".join(args), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    return r
+
+
+def test_bench_self_test():
+    r = _run([os.path.join(BIN, "snc_hip_bench"), "--self-test"])
+    assert r.returncode == 0 and "self-test: PASS" in r.stdout
+
+
+@pytest.mark.parametrize("backend,batched", [("hip-bucket", False), ("hip-sort", False),
+                                             ("hip-atomic", True), ("hip-auto-exact", True)])
+def test_bench_matches_oracle_per_sample(tmp_path, backend, batched):
+    n, dim, hidden, classes, T, budget, seed = 12, 196, 96, 10, 40, 6000, 3
+    out = tmp_path / "bench.json"
+    args = [os.path.join(BIN, "snc_hip_bench"), "--backend", backend, "--num-samples", str(n),
+            "--dim", str(dim), "--hidden", str(hidden), "--classes", str(classes),
+            "--num-steps", str(T), "--synapse-budget", str(budget), "--seed", str(seed),
+            "--log-json", str(out)]
+    if batched:
+        args.append("--batched")
+    r = _run(args)
+    assert r.returncode == 0
+    res = json.loads(out.read_text())
+    g = G.static_snc([dim, hidden, classes], budget, 0.5, 1, 0.2, seed)
+    X, _ = I.make_synthetic(n, dim, classes, 0.15, seed)
+    lif = LIFParams(1.0, 0.9, 0.0, 1)
+    want = [O.forward(g, I.encode("poisson", X[i], T, gain=1.0, max_rate=0.5, seed=seed + i), T,
+                      lif)["spikes"] for i in range(n)]
+    assert res["per_sample_spikes"] == want
+    assert res["synapses"] == g.num_synapses
+
+
+def test_bench_compare_backends_all_exact():
+    r = _run([os.path.join(BIN, "snc_hip_bench"), "--compare-backends", "--num-samples", "6",
+              "--num-steps", "30"])
+    assert r.returncode == 0
+    rows = [l for l in r.stdout.splitlines() if l.startswith("hip-")]
+    assert len(rows) == 5 and all(l.rstrip().endswith("exact") for l in rows)
+
+
+def test_train_driver_learns():
+    r = _run([os.path.join(BIN, "snc_hip_train"), "--dim", "196", "--hidden", "96",
+              "--num-train", "400", "--num-test", "200", "--num-steps", "40", "--min-steps", "40",
+              "--batch", "32", "--epochs", "4", "--w-std", "0.3", "--max-rate", "0.3",
+              "--fan-in", "24", "--rec-fan-in", "16"])
+    assert r.returncode == 0
+    acc = [float(l.split("test_acc=")[1].split()[0]) for l in r.stdout.splitlines()
+           if l.startswith("epoch")]
+    assert len(acc) == 5 and acc[-1] >= 0.4 and acc[-1] > acc[0] + 0.2
+
+
+@pytest.mark.parametrize("grow", ["heuristic", "learned"])
+def test_cotrain_driver_runs(grow):
+    r = _run([os.path.join(BIN, "snc_hip_cotrain"), "--grow", grow, "--outer", "4",
+              "--num-train", "256", "--num-test", "128", "--num-steps", "40", "--min-steps", "30",
+              "--batch", "64", "--structural-budget", "1500", "--grow-n", "60", "--log-runs", "1"])
+    assert r.returncode == 0 and "final: grow=" + grow in r.stdout
+    assert r.stdout.count("round ") == 4
