@@ -196,7 +196,7 @@ int self_test() {
   const uint8_t role[3] = {SNC_ROLE_INPUT, SNC_ROLE_INTERNAL, SNC_ROLE_OUTPUT};
   const int32_t channel[3] = {0, -1, 0};
   const snc_graph_view v{3, 3, row_ptr, post, w, delays, sign, role, channel};
-  bool ok = snc_graph_validate(&v) == SNC_OK;
+  bool ok = snc_graph_validate(&v) == 1;  // 1: valid
   DeviceGraph dg(v);
   const snc_event ev{0, 0, 0, 2.0f};
   const snc_lif lif{1.0f, 0.9f, 0.0f, 1, SNC_RESET_TO_VALUE};
@@ -277,7 +277,10 @@ int main(int argc, char** argv) {
   std::vector<int32_t> y(o.num_samples);
   snc_make_synthetic(o.num_samples, o.dim, o.classes, 0.15f, o.seed, X.data(), y.data());
   Graph g = build_graph(o);
-  check(snc_graph_validate(&g.view), "graph validation");
+  if (snc_graph_validate(&g.view) != 1) {
+    std::fprintf(stderr, "graph validation failed: %s\n", snc_hip_last_error());
+    return 1;
+  }
   DeviceGraph dg(g.view);
 
   if (o.compare) {

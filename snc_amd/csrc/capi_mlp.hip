@@ -69,7 +69,7 @@ constexpr int64_t kGradChunk = 8192;
 
 int grad_device(snc_hip_mlp* m, int64_t off, int64_t M, std::vector<float>& grad_acc,
                 double& loss_acc) {
-  // G is [params][chunk]; chunks of at most kGradChunk examples, combined on
+  // G is [chunk][params]; chunks of at most kGradChunk examples, combined on
   // the host in chunk order weighted by chunk size.
   int rc = grow((void**)&m->G, m->cap_g, (int64_t)kMlpParams * std::min(M, kGradChunk),
                 sizeof(float));

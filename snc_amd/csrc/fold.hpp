@@ -11,12 +11,6 @@
 
 namespace snc_amd {
 
-// Orders LDS traffic between lanes of one wave (no block barrier needed).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 // One LIF step (cpu_backend.cpp:129-143 with the reset-mode extension).
 __device__ __forceinline__ bool lif_step(float& v, int& refr, float ext, float slot,
                                          const LifK& p) {

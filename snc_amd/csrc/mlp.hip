@@ -15,11 +15,13 @@
 // H^T = W X^T -- the weight block is operand A, the activation tile operand B --
 // so the accumulator layout of one layer (lane l: rows 4*(l/16)+i, column
 // l%16) is exactly the B-operand layout of the next layer. Training runs one
-// example per lane on the VALU and writes per-example gradients param-major,
-// reduced per parameter in example order (deterministic).
+// example per wave on the VALU (lane j = hidden unit j) and writes
+// per-example gradients example-major, reduced per parameter in example
+// order (deterministic).
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -138,16 +140,19 @@ __global__ __launch_bounds__(256) void k_mlp_score(const MlpDev p, const float* 
 // training (VALU, one example per lane)
 // ---------------------------------------------------------------------------
 
-// Forward + backward of example e; writes its gradient to G[param][e] and
-// its loss to loss[e]. BF16 rounding is treated as identity in the backward
-// pass (straight-through).
+// Forward + backward of one example per wave (lane j owns hidden unit j of
+// both layers); writes the example's gradient to G[e][param] and its loss to
+// loss[e]. BF16 rounding is treated as identity in the backward pass
+// (straight-through).
 __global__ __launch_bounds__(256) void k_mlp_example_grad(const MlpDev p,
                                                           const float* __restrict__ X,
                                                           const float* __restrict__ y, int64_t M,
                                                           float* __restrict__ G,
                                                           float* __restrict__ loss) {
+  constexpr int kWaves = 256 / kWave;
   __shared__ float s_w1[kMlpH * kMlpIn], s_w2[kMlpH * kMlpH], s_w3[kMlpH], s_b1[kMlpH],
       s_b2[kMlpH];
+  __shared__ float s_h1[kWaves][kMlpH], s_d2[kWaves][kMlpH];
   for (int k = threadIdx.x; k < kMlpH * kMlpIn; k += blockDim.x) s_w1[k] = bf16_to_f32(p.w1b[k]);
   for (int k = threadIdx.x; k < kMlpH * kMlpH; k += blockDim.x) s_w2[k] = bf16_to_f32(p.w2b[k]);
   for (int k = threadIdx.x; k < kMlpH; k += blockDim.x) {
@@ -156,62 +161,61 @@ __global__ __launch_bounds__(256) void k_mlp_example_grad(const MlpDev p,
     s_b2[k] = p.b2[k];
   }
   __syncthreads();
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= M) return;
-  float x[kMlpIn];
+  const int wave = threadIdx.x >> 6, j = lane_id();
+  const float b3 = p.b3[0];
+  for (int64_t e = (int64_t)blockIdx.x * kWaves + wave; e < M; e += (int64_t)gridDim.x * kWaves) {
+    float x[kMlpIn];
 #pragma unroll
-  for (int k = 0; k < kMlpIn; ++k) x[k] = q(X[e * kMlpIn + k]);
-  float h1[kMlpH];
-#pragma unroll 4
-  for (int j = 0; j < kMlpH; ++j) {
-    float s = 0.0f;
+    for (int k = 0; k < kMlpIn; ++k) x[k] = q(X[e * kMlpIn + k]);
+    float s1 = s_b1[j];
 #pragma unroll
-    for (int k = 0; k < kMlpIn; ++k) s += s_w1[j * kMlpIn + k] * x[k];
-    h1[j] = q(fmaxf(s + s_b1[j], 0.0f));
-  }
-  float h2[kMlpH];
-  float out = p.b3[0];
-  for (int j = 0; j < kMlpH; ++j) {
-    float s = 0.0f;
+    for (int k = 0; k < kMlpIn; ++k) s1 += s_w1[j * kMlpIn + k] * x[k];
+    const float h1 = q(fmaxf(s1, 0.0f));
+    s_h1[wave][j] = h1;
+    wave_lds_sync();
+    float s2 = s_b2[j];
 #pragma unroll 16
-    for (int k = 0; k < kMlpH; ++k) s += s_w2[j * kMlpH + k] * h1[k];
-    h2[j] = fmaxf(s + s_b2[j], 0.0f);
-    out += s_w3[j] * h2[j];
-  }
-  // mean binary cross-entropy with logits
-  const float t = y[e];
-  const float sg = 1.0f / (1.0f + expf(-out));
-  loss[e] = fmaxf(out, 0.0f) - out * t + log1pf(expf(-fabsf(out)));
-  const float dout = (sg - t) / (float)M;
-  G[(int64_t)kMlpOffB3 * M + e] = dout;
-  float dh1[kMlpH];
+    for (int k = 0; k < kMlpH; ++k) s2 += s_w2[j * kMlpH + k] * s_h1[wave][k];
+    const float h2 = fmaxf(s2, 0.0f);
+    float out = s_w3[j] * h2;  // fixed-order wave sum
 #pragma unroll
-  for (int k = 0; k < kMlpH; ++k) dh1[k] = 0.0f;
-  for (int j = 0; j < kMlpH; ++j) {
-    G[(int64_t)(kMlpOffW3 + j) * M + e] = dout * h2[j];
-    const float d2 = h2[j] > 0.0f ? dout * s_w3[j] : 0.0f;
-    G[(int64_t)(kMlpOffB2 + j) * M + e] = d2;
-#pragma unroll 16
-    for (int k = 0; k < kMlpH; ++k) {
-      G[(int64_t)(kMlpOffW2 + j * kMlpH + k) * M + e] = d2 * h1[k];
-      dh1[k] += d2 * s_w2[j * kMlpH + k];
+    for (int off = kWave / 2; off > 0; off >>= 1) out += __shfl_xor(out, off, kWave);
+    out += b3;
+    // mean binary cross-entropy with logits
+    const float t = y[e];
+    const float sg = 1.0f / (1.0f + expf(-out));
+    const float dout = (sg - t) / (float)M;
+    float* g = G + e * kMlpParams;
+    if (j == 0) {
+      loss[e] = fmaxf(out, 0.0f) - out * t + log1pf(expf(-fabsf(out)));
+      g[kMlpOffB3] = dout;
     }
-  }
-  for (int k = 0; k < kMlpH; ++k) {
-    const float d1 = h1[k] > 0.0f ? dh1[k] : 0.0f;
-    G[(int64_t)(kMlpOffB1 + k) * M + e] = d1;
+    g[kMlpOffW3 + j] = dout * h2;
+    const float d2 = h2 > 0.0f ? dout * s_w3[j] : 0.0f;
+    g[kMlpOffB2 + j] = d2;
+    s_d2[wave][j] = d2;
+    wave_lds_sync();
+    // dW2[r][c] = d2[r] * h1[c]: lane j writes column j of every row (coalesced rows)
+#pragma unroll 8
+    for (int r = 0; r < kMlpH; ++r) g[kMlpOffW2 + r * kMlpH + j] = s_d2[wave][r] * h1;
+    float dh1 = 0.0f;
+#pragma unroll 16
+    for (int r = 0; r < kMlpH; ++r) dh1 += s_d2[wave][r] * s_w2[r * kMlpH + j];
+    const float d1 = h1 > 0.0f ? dh1 : 0.0f;
+    g[kMlpOffB1 + j] = d1;
 #pragma unroll
-    for (int f = 0; f < kMlpIn; ++f) G[(int64_t)(kMlpOffW1 + k * kMlpIn + f) * M + e] = d1 * x[f];
+    for (int f = 0; f < kMlpIn; ++f) g[kMlpOffW1 + j * kMlpIn + f] = d1 * x[f];
+    wave_lds_sync();  // s_h1 / s_d2 are rewritten by the next example
   }
 }
 
-// grad[p] = sum_e G[p][e] (examples in order); loss_out = sum_e loss[e] / M.
+// grad[p] = sum_e G[e][p] (examples in order); loss_out = sum_e loss[e] / M.
 __global__ void k_mlp_reduce(const float* __restrict__ G, const float* __restrict__ loss,
                              int64_t M, float* __restrict__ grad, float* __restrict__ loss_out) {
   const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (pidx < kMlpParams) {
     float s = 0.0f;
-    for (int64_t e = 0; e < M; ++e) s += G[(int64_t)pidx * M + e];
+    for (int64_t e = 0; e < M; ++e) s += G[e * kMlpParams + pidx];
     grad[pidx] = s;
   }
   if (pidx == kMlpParams) {
@@ -255,7 +259,7 @@ int mlp_launch_score(const MlpDev& p, const float* X, int64_t M, float* out, int
 
 int mlp_launch_grad(const MlpDev& p, const float* X, const float* y, int64_t M, float* G,
                     float* loss, float* grad, float* loss_out, hipStream_t st) {
-  const int blocks = (int)((M + 255) / 256);
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((M + 3) / 4, 4096));
   hipLaunchKernelGGL(k_mlp_example_grad, dim3(blocks), dim3(256), 0, st, p, X, y, M, G, loss);
   hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, G, loss,
                      M, grad, loss_out);

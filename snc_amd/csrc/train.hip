@@ -65,10 +65,10 @@ __device__ __forceinline__ float input_drive(const TrainArgs& a, int j, int samp
 
 // LIF + surrogate of one neuron (trainer.cpp:96-125 plus the reset-mode and
 // leaky-readout extensions). Returns fired; writes psi.
-__device__ __forceinline__ bool train_lif(const TrainArgs& a, int j, float& v, int& refr,
+__device__ __forceinline__ bool train_lif(const TrainArgs& a, uint8_t role, float& v, int& refr,
                                           float ext, float slot, float& psi) {
   const float thr = a.lif.threshold;
-  if (a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT) {
+  if (a.readout_leaky && role == SNC_ROLE_OUTPUT) {
     v = __fadd_rn(__fadd_rn(__fmul_rn(a.lif.decay, v), ext), slot);
     psi = a.readout_psi;
     return false;
@@ -216,7 +216,14 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   float* s_logit = (float*)(s_bits + a.bits_words);  // [classes]
   float* s_delta = s_logit + a.classes;              // [classes]
   const size_t state_bytes = train_state_bytes(n, a.words, a.bits_words, a.classes);
-  float* s_ring = (float*)(smem + state_bytes);      // [L][n] when RING_LDS
+  const size_t const_bytes = train_const_bytes(n, a.n_inputs);
+  int32_t* s_rowp = (int32_t*)(smem + state_bytes);  // [n + 1] row_ptr
+  int32_t* s_ccol = s_rowp + n + 1;                  // [n + 1] column starts
+  int32_t* s_chan = s_ccol + n + 1;                  // [n] channel
+  int32_t* s_inp = s_chan + n;                       // [n_inputs] input neuron ids
+  float* s_inpx = (float*)(s_inp + a.n_inputs);      // [n_inputs] drive value / probability
+  uint8_t* s_role = (uint8_t*)(s_inpx + a.n_inputs); // [n] role
+  float* s_ring = (float*)(smem + state_bytes + const_bytes);  // [L][n] when RING_LDS
   __shared__ int s_nf;
   __shared__ unsigned long long s_ev, s_sp;
 
@@ -233,7 +240,27 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     s_tr[j] = 0.0f;
     s_ext[j] = 0.0f;
     s_refr[j] = 0;
+    s_chan[j] = a.channel[j];
+    s_role[j] = a.role[j];
   }
+  for (int j = tid; j <= n; j += kTrainThreads) {
+    s_rowp[j] = a.row_ptr[j];
+    s_ccol[j] = a.ccol[j];
+  }
+  // Input drive of this sample: the direct value, or the Poisson spike
+  // probability (0: the channel never fires), from the features once.
+  if (!a.ev_base)
+    for (int q = tid; q < a.n_inputs; q += kTrainThreads) {
+      const int j = a.inputs[q];
+      const int c = a.channel[j];
+      float val = 0.0f;
+      if (c >= 0 && c < a.dim) {
+        const float x = a.X[(size_t)sample * a.dim + c];
+        if (x > a.enc_thr) val = a.encoder == SNC_ENC_DIRECT ? x * a.gain : fminf(1.0f, x * a.max_rate);
+      }
+      s_inp[q] = j;
+      s_inpx[q] = val;
+    }
   for (int k = tid; k < a.bits_words; k += kTrainThreads) s_bits[k] = 0u;
   for (int k = tid; k < a.classes; k += kTrainThreads) s_logit[k] = 0.0f;
   for (size_t k = tid; k < (size_t)L * n; k += kTrainThreads) ring[k] = 0.0f;
@@ -250,11 +277,17 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
       for (int k = a.ev_step_ptr[ev0 + t] + tid; k < a.ev_step_ptr[ev0 + t + 1];
            k += kTrainThreads)
         s_ext[a.ev_nid[k]] = a.ev_amp[k];
+    } else if (a.encoder == SNC_ENC_DIRECT) {
+      for (int q = tid; q < a.n_inputs; q += kTrainThreads)
+        if (s_inpx[q] != 0.0f) s_ext[s_inp[q]] = s_inpx[q];
     } else {
       for (int q = tid; q < a.n_inputs; q += kTrainThreads) {
-        const int j = a.inputs[q];
-        const float x = input_drive(a, j, sample, t, a.epoch);
-        if (x != 0.0f) s_ext[j] = x;
+        const float p = s_inpx[q];
+        if (p == 0.0f) continue;
+        const int j = s_inp[q], c = s_chan[j];
+        const Philox4 r = philox4x32_10((uint32_t)(c >> 2), (uint32_t)t, (uint32_t)sample,
+                                        (uint32_t)a.epoch, a.seed_lo, a.seed_hi);
+        if (r.v[c & 3] < prob_to_u32((double)p)) s_ext[j] = a.gain;
       }
     }
   };
@@ -273,7 +306,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
         float v = s_v[j];
         int r = s_refr[j];
         float psi;
-        f = train_lif(a, j, v, r, s_ext[j], slot[j], psi);
+        const uint8_t role = s_role[j];
+        f = train_lif(a, role, v, r, s_ext[j], slot[j], psi);
         s_v[j] = v;
         s_refr[j] = r;
         s_ext[j] = 0.0f;
@@ -284,13 +318,13 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
           hrow[j] = psi;
           hrow[a.hn + j] = tr;
         }
+        const int ch = s_chan[j];
         if (f) {
           if (a.neur_fire) atomicAdd(a.neur_fire + j, 1ull);
-          if (!a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT && a.channel[j] >= 0)
-            atomicAdd(&s_logit[a.channel[j]], 1.0f);
+          if (!a.readout_leaky && role == SNC_ROLE_OUTPUT && ch >= 0) atomicAdd(&s_logit[ch], 1.0f);
         }
-        if (a.readout_leaky && a.role[j] == SNC_ROLE_OUTPUT && a.channel[j] >= 0)
-          s_logit[a.channel[j]] = __fadd_rn(s_logit[a.channel[j]], v);  // one readout per channel
+        if (a.readout_leaky && role == SNC_ROLE_OUTPUT && ch >= 0)
+          s_logit[ch] = __fadd_rn(s_logit[ch], v);  // one readout per channel
       }
       const uint64_t m = ballot(f);
       if (lane_id() == 0 && j - lane_id() < n) {
@@ -315,7 +349,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     unsigned long long ev = 0;
     for (int k = wave; k < nf; k += kTrainThreads / kWave) {
       const int i = s_fired[k];
-      const int rb = a.row_ptr[i], re = a.row_ptr[i + 1];
+      const int rb = s_rowp[i], re = s_rowp[i + 1];
       for (int e = rb + lane; e < re; e += kWave) {
         const int p = a.cpos[e];
         atomicOr(&s_bits[p >> 5], 1u << (p & 31));
@@ -326,7 +360,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     __syncthreads();
     // (C) ordered fold into the ring; next step's injections
     for (int j = tid; j < n; j += kTrainThreads) {
-      const int w0 = a.ccol[j] >> 5, w1 = a.ccol[j + 1] >> 5;
+      const int w0 = s_ccol[j] >> 5, w1 = s_ccol[j + 1] >> 5;
       int cur = -1;
       float acc = 0.0f;
       for (int w = w0; w < w1; ++w) {
@@ -366,8 +400,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   sample_epilogue(a, b, s_logit, Ls, s_delta, label, n, tid, kTrainThreads);
   if (a.C) {
     const float scale = __fdiv_rn(a.lr, (float)max(1, T));
-    const int chunk_steps =
-        (int)(((size_t)a.lds_bytes - state_bytes) / (sizeof(float) * 2 * (size_t)a.hn));
+    const int chunk_steps = (int)(((size_t)a.lds_bytes - state_bytes - const_bytes) /
+                                  (sizeof(float) * 2 * (size_t)a.hn));
     elig_contrib<ET>(a, b, T, scale, Ls, s_ring, max(1, chunk_steps), tid);
   }
   if (tid == 0) {
@@ -411,7 +445,7 @@ __global__ void k_p_lif(TrainArgs a, PlainState ps, int t, int tm) {
     int r = ps.refr[j];
     float psi;
     float* slot = ps.ring + (size_t)tm * a.n + j;
-    f = train_lif(a, j, v, r, ps.ext[j], *slot, psi);
+    f = train_lif(a, a.role[j], v, r, ps.ext[j], *slot, psi);
     ps.v[j] = v;
     ps.refr[j] = r;
     ps.psi[j] = psi;
@@ -531,7 +565,8 @@ __global__ void k_gather_wc(const float* __restrict__ w, const int32_t* __restri
 static inline int blocks_for(int64_t n, int t) { return (int)std::max<int64_t>(1, (n + t - 1) / t); }
 
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds) {
-  size_t b = train_state_bytes(a.n, a.words, a.bits_words, a.classes);
+  size_t b = train_state_bytes(a.n, a.words, a.bits_words, a.classes) +
+             train_const_bytes(a.n, a.n_inputs);
   if (ring_lds) b += sizeof(float) * (size_t)a.L * a.n;
   return b;
 }
@@ -551,7 +586,8 @@ int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, siz
     return fail(SNC_E_HIP, "hipFuncGetAttributes of the batched trainer failed");
   if (attr.sharedSizeBytes >= lds_max) return fail(SNC_E_UNSUPPORTED, "no dynamic LDS left");
   const size_t dyn_max = lds_max - attr.sharedSizeBytes;
-  const size_t state = train_state_bytes(a.n, a.words, a.bits_words, a.classes);
+  const size_t state = train_state_bytes(a.n, a.words, a.bits_words, a.classes) +
+                       train_const_bytes(a.n, a.n_inputs);
   const size_t step_bytes = sizeof(float) * 2 * (size_t)a.hn;
   size_t lds = std::max(train_lds_bytes(a, ring_lds), state + step_bytes);
   lds = std::max(lds, std::min(dyn_max, state + kChunkTarget * step_bytes));

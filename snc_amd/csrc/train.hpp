@@ -81,6 +81,15 @@ __host__ __device__ inline size_t train_state_bytes(int n, int words, int bits_w
   return (b + 15) / 16 * 16;
 }
 
+// LDS bytes of the per-sample copies of read-only graph data the step loop
+// touches every step (row pointers, column starts, channels, roles, input
+// drive), rounded up to 16; they follow the state.
+__host__ __device__ inline size_t train_const_bytes(int n, int n_inputs) {
+  const size_t b = sizeof(int32_t) * (2 * ((size_t)n + 1) + (size_t)n + (size_t)n_inputs) +
+                   sizeof(float) * (size_t)n_inputs + (size_t)n;
+  return (b + 15) / 16 * 16;
+}
+
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds);
 int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
                          hipStream_t st);

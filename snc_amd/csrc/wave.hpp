@@ -42,4 +42,10 @@ __device__ __forceinline__ long long wave_sum_ll(long long x) {
   return x;
 }
 
+// Orders LDS traffic between lanes of one wave (no block barrier needed).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 }  // namespace snc_amd
