@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +20,7 @@
 
 #include "autotune.hpp"
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "device_graph.hpp"
 #include "forward_plan.hpp"
 #include "persist.hpp"
@@ -337,6 +339,7 @@ int snc_hip_graph_profile_get(const snc_hip_graph* g, snc_graph_profile* out) {
 static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_events, int B,
                       int32_t num_steps, const snc_lif* lif, const snc_plan_config* cfg,
                       snc_hip_plan** out) {
+  hip_clear();
   if (!G || !lif || !out || !cfg || (n_events > 0 && !events) || num_steps < 0 || B < 1)
     return fail(SNC_E_INVALID, "bad plan arguments");
   *out = nullptr;
@@ -444,9 +447,9 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   P->h_inj_ptr = ip;
 
   // State layout. hip-atomic and hip-sort: ring [B][L][n]. hip-bucket with
-  // the fused kernels: one arrival bitmap and one arrival-flag array per
-  // ring slot, [B][L][...], and no ring. Persistent: ring plus bitmap and
-  // flags double-buffered by step parity.
+  // the fused kernels: one arrival bitmap per ring slot, [B][L][P/32], and no
+  // ring. Persistent: ring plus bitmap and dirty flags double-buffered by
+  // step parity.
   P->persistent = B == 1 && persist_choose(g, variant, cfg->persistent != 0, P->pcfg);
   P->cfg.persistent = P->persistent ? 1 : 0;
   const bool bucket = variant == SNC_VARIANT_BUCKET;
@@ -456,8 +459,9 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   P->n_bits_words = (size_t)(g.P / 32);
   P->dirty_stride = dirty_stride;
   const size_t planes = slots ? (size_t)g.L : (bucket ? 2 : 0);
+  const size_t dirty_planes = bucket && !slots ? 2 : 0;
   P->bits_total = (size_t)B * planes * P->n_bits_words;
-  P->dirty_total = (size_t)B * planes * dirty_stride;
+  P->dirty_total = (size_t)B * dirty_planes * dirty_stride;
   P->ring_total = slots ? 0 : (size_t)B * g.L * n;
   int rc = SNC_OK;
   if (!rc) rc = dupload(&P->heavy_ids, P->heavy_host.data(), P->heavy_host.size());
@@ -552,11 +556,10 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   a.bits_half = (int64_t)P->n_bits_words;
   a.dirty_half = (int64_t)dirty_stride;
   a.bits_slot = (int64_t)P->n_bits_words;
-  a.dirty_slot = (int64_t)dirty_stride;
   a.B = B;
   a.ring_stride = (int64_t)g.L * n;
   a.bits_stride = (int64_t)(planes * P->n_bits_words);
-  a.dirty_stride = (int64_t)(planes * dirty_stride);
+  a.dirty_stride = (int64_t)(dirty_planes * dirty_stride);
   a.blk_stride = (int64_t)P->n_blk_slots;
   a.inj_stride = (int64_t)num_steps * (R + 1);
   a.hinj_stride = (int64_t)num_steps * n_heavy;
@@ -690,24 +693,24 @@ int launch_loop(snc_hip_plan* P, bool raster) {
   hipGraphExec_t& ge = P->graph_exec[raster ? 1 : 0];
   if (!ge) {
     hipGraph_t graph = nullptr;
-    if (hipStreamBeginCapture(P->core.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
-      return fail(SNC_E_HIP, "hipStreamBeginCapture failed");
+    const hipError_t eb = hipStreamBeginCapture(P->core.stream, hipStreamCaptureModeThreadLocal);
+    if (eb != hipSuccess) return hip_fail(eb, "hipStreamBeginCapture");
     int rc = launch_chain(P, a, true);
     hipError_t e = hipStreamEndCapture(P->core.stream, &graph);
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);
       return rc;
     }
-    if (e != hipSuccess || !graph) return fail(SNC_E_HIP, "hipStreamEndCapture failed");
+    if (e != hipSuccess || !graph) return hip_fail(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
       ge = nullptr;
-      return fail(SNC_E_HIP, "hipGraphInstantiate failed");
+      return hip_fail(e, "hipGraphInstantiate");
     }
   }
-  return hipGraphLaunch(ge, P->core.stream) == hipSuccess ? SNC_OK
-                                                           : fail(SNC_E_HIP, "hipGraphLaunch failed");
+  const hipError_t el = hipGraphLaunch(ge, P->core.stream);
+  return el == hipSuccess ? SNC_OK : hip_fail(el, "hipGraphLaunch");
 }
 
 }  // namespace
@@ -716,6 +719,7 @@ extern "C" {
 
 int snc_hip_plan_launch(snc_hip_plan* P) {
   if (!P) return fail(SNC_E_INVALID, "null plan");
+  hip_clear();
   int rc = reset_state(P, false);
   if (!rc) rc = launch_loop(P, false);
   return rc;
@@ -723,6 +727,7 @@ int snc_hip_plan_launch(snc_hip_plan* P) {
 
 int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stats* stats) {
   if (!P || !stats) return fail(SNC_E_INVALID, "null argument");
+  hip_clear();
   const DevGraph& g = P->G->g;
   const bool want_raster = cap && (cap->raster || cap->step_spikes);
   int rc = SNC_OK;
@@ -734,7 +739,8 @@ int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stat
   rc = launch_loop(P, want_raster);
   if (rc) return rc;
   if (hipEventRecord(P->ev1, st) != hipSuccess) return fail(SNC_E_HIP, "hipEventRecord failed");
-  if (hipStreamSynchronize(st) != hipSuccess) return fail(SNC_E_HIP, "step loop failed");
+  const hipError_t es = hipStreamSynchronize(st);
+  if (es != hipSuccess) return hip_fail(es, "step loop");
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, P->ev0, P->ev1);
   P->last_ms = ms;
@@ -915,6 +921,12 @@ static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_
     if (!rc) rc = snc_hip_plan_run(P, nullptr, &st);
     if (!rc) rc = snc_hip_plan_run(P, nullptr, &st);
     snc_hip_plan_free(P);
+    if (rc) {
+      (void)hipGetLastError();  // a candidate that cannot run leaves no error behind
+      fprintf(stderr, "snc_hip autotuner: candidate variant %d block %d hub %d persistent %d "
+                      "skipped: %s\n",
+              c.variant, c.block_neurons, c.heavy_threshold, c.persistent, snc_hip_last_error());
+    }
     const double ms = rc ? -1.0 : st.ms / rep.trial_steps;  // -1: could not run here
     if (rep.n_candidates >= SNC_TUNE_MAX_CANDIDATES) break;
     rep.cand[rep.n_candidates] = c;

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "mlp.hpp"
 #include "rng.hpp"
 #include "snc_hip.h"
@@ -161,6 +162,7 @@ int snc_hip_mlp_score(snc_hip_mlp* m, const float* features, int64_t M, float* s
                       double* kernel_ms) {
   if (!m || (M > 0 && (!features || !scores)) || M < 0) return fail(SNC_E_INVALID, "bad arguments");
   if (M == 0) return SNC_OK;
+  hip_clear();
   int rc = ensure_x(m, M);
   if (rc) return rc;
   if (hipMemcpyAsync(m->X, features, sizeof(float) * M * kMlpIn, hipMemcpyHostToDevice,
@@ -193,6 +195,7 @@ int snc_hip_mlp_score(snc_hip_mlp* m, const float* features, int64_t M, float* s
 int snc_hip_mlp_grad(snc_hip_mlp* m, const float* features, const float* labels, int64_t M,
                      float* grad, double* loss) {
   if (!m || !features || !labels || !grad || M < 1) return fail(SNC_E_INVALID, "bad arguments");
+  hip_clear();
   int rc = ensure_x(m, M);
   if (rc) return rc;
   if (hipMemcpy(m->X, features, sizeof(float) * M * kMlpIn, hipMemcpyHostToDevice) != hipSuccess ||
@@ -211,6 +214,7 @@ int snc_hip_mlp_train(snc_hip_mlp* m, const float* features, const float* labels
                       int32_t epochs, int32_t batch, float lr, uint64_t seed, double* final_loss) {
   if (!m || !features || !labels || M < 1 || epochs < 0 || batch < 1)
     return fail(SNC_E_INVALID, "bad arguments");
+  hip_clear();
   int rc = ensure_x(m, M);
   if (rc) return rc;
   // Shuffle on the host each epoch and upload the permuted examples once per

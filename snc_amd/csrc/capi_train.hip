@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "forward_plan.hpp"
 #include "snc_hip.h"
 #include "train.hpp"
@@ -468,6 +469,7 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
                        int32_t mode, int32_t apply, float* logits_out, uint32_t* raster_out,
                        snc_train_stats* st) {
   if (!s || !idx || B < 1 || B > s->batch) return fail(SNC_E_INVALID, "bad minibatch");
+  hip_clear();
   if (!s->n_data) return fail(SNC_E_INVALID, "no dataset uploaded");
   for (int b = 0; b < B; ++b)
     if (idx[b] < 0 || idx[b] >= s->n_data) return fail(SNC_E_INVALID, "sample index out of range");
@@ -601,6 +603,7 @@ int snc_hip_train_epoch(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t
                         int32_t mode, snc_train_stats* st) {
   if (!s || !s->n_data) return fail(SNC_E_INVALID, "no dataset uploaded");
   if (idx && n < 0) return fail(SNC_E_INVALID, "negative sample count");
+  hip_clear();
   std::vector<int32_t> order;
   if (idx) {
     order.assign(idx, idx + n);
@@ -631,6 +634,7 @@ int snc_hip_train_epoch(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t
 int snc_hip_train_evaluate(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t mode,
                            snc_train_stats* st) {
   if (!s || !idx || n < 0) return fail(SNC_E_INVALID, "bad arguments");
+  hip_clear();
   snc_train_stats tot{};
   for (int off = 0; off < n; off += s->batch) {
     const int B = std::min(s->batch, n - off);

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "device_graph.hpp"
 #include "fold.hpp"
 #include "forward_plan.hpp"
@@ -128,19 +129,19 @@ int fwd_launch_fill(hipStream_t st, void* p, uint32_t value, size_t words) {
   if (!words) return SNC_OK;
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int64_t)words, 256, 2048)), dim3(256), 0, st,
                      (uint32_t*)p, value, words);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "fill launch failed");
+  return hip_last("fill launch failed");
 }
 
 int fwd_launch_lif(const FwdPlanCore& pc, const StepArgs& a, int t) {
   hipLaunchKernelGGL(k_lif, dim3(pc.lif_grid), dim3(256), 0, pc.stream, a, t, t % a.L);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "LIF launch failed");
+  return hip_last("LIF launch failed");
 }
 
 int fwd_launch_first_injection(const FwdPlanCore& pc, const StepArgs& a) {
   if (a.T <= 0) return SNC_OK;
   hipLaunchKernelGGL(k_scatter_inj, dim3(grid_for(pc.max_inj_per_step, 256, 1024)), dim3(256), 0,
                      pc.stream, a.inj_ptr, a.inj_nid, a.inj_amp, 0, a.ext);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "injection launch failed");
+  return hip_last("injection launch failed");
 }
 
 }  // namespace snc_amd

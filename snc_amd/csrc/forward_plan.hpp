@@ -69,7 +69,6 @@ struct StepArgs {
   int64_t bits_half;           // persistent: words of one bitmap half (step parity)
   int64_t dirty_half;          // persistent: bytes of one dirty-flag half
   int64_t bits_slot;           // fused hip-bucket: words of one arrival bitmap (per ring slot)
-  int64_t dirty_slot;          // fused hip-bucket: bytes of one arrival-flag array
   // batched plans: samples are grid.y; per-sample strides of the state
   int B;
   int64_t ring_stride;    // L * n

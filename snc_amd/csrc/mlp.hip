@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "mlp.hpp"
 #include "wave.hpp"
 
@@ -254,7 +255,7 @@ __global__ void k_mlp_quantize(const float* __restrict__ w, MlpDev p) {
 int mlp_launch_score(const MlpDev& p, const float* X, int64_t M, float* out, int blocks,
                      hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_score, dim3(blocks), dim3(256), 0, st, p, X, M, out);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "MLP score launch failed");
+  return hip_last("MLP score launch failed");
 }
 
 int mlp_launch_grad(const MlpDev& p, const float* X, const float* y, int64_t M, float* G,
@@ -263,7 +264,7 @@ int mlp_launch_grad(const MlpDev& p, const float* X, const float* y, int64_t M, 
   hipLaunchKernelGGL(k_mlp_example_grad, dim3(blocks), dim3(256), 0, st, p, X, y, M, G, loss);
   hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 1 + 255) / 256), dim3(256), 0, st, G, loss,
                      M, grad, loss_out);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "MLP grad launch failed");
+  return hip_last("MLP grad launch failed");
 }
 
 int mlp_launch_adam(float* w, float* m, float* v, const float* g, float lr, int step,
@@ -272,12 +273,12 @@ int mlp_launch_adam(float* w, float* m, float* v, const float* g, float lr, int 
   const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
   hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, w, m, v, g, lr,
                      b1, b2, eps, c1, c2, p);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "MLP update launch failed");
+  return hip_last("MLP update launch failed");
 }
 
 int mlp_launch_quantize(const float* w, const MlpDev& p, hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_quantize, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, w, p);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "MLP quantize launch failed");
+  return hip_last("MLP quantize launch failed");
 }
 
 uint16_t mlp_bf16(float f) { return f32_to_bf16(f); }

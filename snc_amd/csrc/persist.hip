@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "fold.hpp"
 #include "persist.hpp"
 
@@ -229,9 +230,9 @@ int persist_launch(const DevGraph& g, const FwdPlanCore& pc, const StepArgs& a,
   void* args[] = {(void*)&aa, (void*)&q};
   const void* fn = pc.variant == SNC_VARIANT_BUCKET ? (const void*)k_persist<true>
                                                     : (const void*)k_persist<false>;
-  if (hipLaunchCooperativeKernel(fn, dim3(cfg.blocks), dim3(cfg.threads), args, 0, pc.stream) !=
-      hipSuccess)
-    return fail(SNC_E_HIP, "cooperative launch of the persistent kernel failed");
+  const hipError_t e =
+      hipLaunchCooperativeKernel(fn, dim3(cfg.blocks), dim3(cfg.threads), args, 0, pc.stream);
+  if (e != hipSuccess) return hip_fail(e, "cooperative launch of the persistent kernel");
   return SNC_OK;
 }
 

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "forward_plan.hpp"
 #include "wave.hpp"
 
@@ -265,7 +266,7 @@ int sort_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t) {
     vin = tv;
   }
   hipLaunchKernelGGL(k_sort_fold, dim3(kSortBlocks), dim3(kSortThreads), 0, st, a, kin, vin);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "sort step launch failed");
+  return hip_last("sort step launch failed");
 }
 
 }  // namespace snc_amd

@@ -5,7 +5,7 @@
 // choice of the autotuner), one per thread; grid.y is the sample of a batched
 // plan. Within step t a block
 //   1. issues every load that depends only on the neuron (state, row
-//      pointers, column bounds, arrival flag) before its first barrier,
+//      pointers, column bounds) before its first barrier,
 //   2. stages its own injections of step t in LDS (per-(step, block) ranges),
 //   3. gathers the input arriving at step t:
 //        hip-atomic  the ring slot t % L (read and cleared);
@@ -16,8 +16,7 @@
 //   4. integrates/fires,
 //   5. expands its fired rows, one wave per row, lanes striding the row:
 //        hip-atomic  float atomics into ring slots (t + d) % L;
-//        hip-bucket  presence bits in the arrival bitmap of step t + d and the
-//                    target's arrival flag for that step.
+//        hip-bucket  presence bits in the arrival bitmap of step t + d.
 // Arrival state of step t is only read (and cleared) at step t and only
 // written at steps t - L + 1 .. t - 1 (delays are 1 .. L - 1), so within one
 // launch readers and writers never touch the same bitmap or ring slot.
@@ -31,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "fold.hpp"
 #include "forward_plan.hpp"
 #include "wave.hpp"
@@ -58,7 +58,6 @@ __device__ __forceinline__ int expand_row(const StepArgs& a, int rb, int re, int
     if (BUCKET) {
       const int p = a.cpos[e];
       atomicOr(a.bits + (size_t)s * a.bits_slot + (p >> 5), 1u << (p & 31));
-      a.dirty[(size_t)s * a.dirty_slot + a.post[e]] = 1;
     } else {
       atomicAdd(a.ring + (size_t)s * a.n + a.post[e], a.wsig[e]);
     }
@@ -82,9 +81,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a0, int t, int tm) {
   bool mine = j < n;
   float v = 0.0f, slot = 0.0f;
   int r = 0, rb = 0, re = 0, c0 = 0, c1 = 0;
-  uint8_t arrived = 0;
   float* slotp = BUCKET ? nullptr : a.ring + (size_t)tm * n + j;
-  uint8_t* dnow = BUCKET ? a.dirty + (size_t)tm * a.dirty_slot : nullptr;
   if (mine) {
     if (BUCKET) mine = a.is_heavy[j] == 0;
     v = a.v[j];
@@ -92,7 +89,6 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a0, int t, int tm) {
     rb = a.row_ptr[j];
     re = a.row_ptr[j + 1];
     if (BUCKET) {
-      arrived = dnow[j];
       c0 = a.ccol[j];
       c1 = a.ccol[j + 1];
     } else {
@@ -111,8 +107,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a0, int t, int tm) {
 
   bool f = false;
   if (mine) {
-    if (BUCKET && arrived) {
-      dnow[j] = 0;
+    if (BUCKET) {  // neighbouring lanes read neighbouring columns: the slot's
+                   // bitmap streams through once per step
       uint32_t* bnow = a.bits + (size_t)tm * a.bits_slot;
       const int w0 = c0 >> 5, nw = (c1 >> 5) - w0;
       uint32_t wv[kLightMaxWords];
@@ -174,15 +170,13 @@ __global__ __launch_bounds__(kStepThreads) void k_hub_step(StepArgs a0, int t, i
   const int hidx = blockIdx.x;
   const int h = a.heavy_ids[hidx];
   const int slot_base = a.n_reg_blocks;  // stats slots after the regular blocks
-  uint8_t* dnow = a.dirty + (size_t)tm * a.dirty_slot;
   // neuron-only loads first (the same addresses in every lane)
-  const uint8_t arrived = dnow[h];
   const int c0 = a.ccol[h], c1 = a.ccol[h + 1];
   float v = a.v[h];
   int r = a.refr[h];
   const float x_in = a.inj_heavy[(size_t)t * a.n_heavy + hidx];
   float in = 0.0f;  // sum of the arriving values, kept by thread 0
-  if (arrived) {    // same value for every thread of the block
+  {
     uint32_t* bnow = a.bits + (size_t)tm * a.bits_slot;
     const int w0 = c0 >> 5, w1 = c1 >> 5;
     for (int sb = w0; sb < w1; sb += kHubSuperWords) {  // uniform trip count
@@ -248,7 +242,6 @@ __global__ __launch_bounds__(kStepThreads) void k_hub_step(StepArgs a0, int t, i
       }
       __syncthreads();  // s_wsum is reused by the next super-round
     }
-    if (tid == 0) dnow[h] = 0;
   }
   if (tid == 0) {
     const bool f = lif_step(v, r, x_in, in, a.lif);
@@ -272,7 +265,7 @@ int fused_launch_hub(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_
   if (pc.variant == SNC_VARIANT_BUCKET && a.n_heavy > 0)
     hipLaunchKernelGGL(k_hub_step, dim3(a.n_heavy, a.B), dim3(kStepThreads), 0, st, a, t,
                        t % a.L);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "hub step launch failed");
+  return hip_last("hub step launch failed");
 }
 
 template <int BLOCK>
@@ -293,7 +286,7 @@ int fused_launch_regular(const FwdPlanCore& pc, const StepArgs& a, int t) {
     case 512: launch_step_block<512>(pc, a, t, tm); break;
     default: return fail(SNC_E_INVALID, "step block must be 128, 256 or 512 neurons");
   }
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "step launch failed");
+  return hip_last("step launch failed");
 }
 
 int fused_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_t hub_stream) {

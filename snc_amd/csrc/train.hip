@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "hip_err.hpp"
 #include "fold.hpp"
 #include "forward_plan.hpp"
 #include "rng.hpp"
@@ -582,8 +583,8 @@ int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, siz
                                    : (const void*)k_train_batched<float, false>);
   // The kernel's own static LDS counts against the same per-block limit.
   hipFuncAttributes attr{};
-  if (hipFuncGetAttributes(&attr, fn) != hipSuccess)
-    return fail(SNC_E_HIP, "hipFuncGetAttributes of the batched trainer failed");
+  const hipError_t eg = hipFuncGetAttributes(&attr, fn);
+  if (eg != hipSuccess) return hip_fail(eg, "hipFuncGetAttributes of the batched trainer");
   if (attr.sharedSizeBytes >= lds_max) return fail(SNC_E_UNSUPPORTED, "no dynamic LDS left");
   const size_t dyn_max = lds_max - attr.sharedSizeBytes;
   const size_t state = train_state_bytes(a.n, a.words, a.bits_words, a.classes) +
@@ -592,19 +593,22 @@ int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, siz
   size_t lds = std::max(train_lds_bytes(a, ring_lds), state + step_bytes);
   lds = std::max(lds, std::min(dyn_max, state + kChunkTarget * step_bytes));
   if (lds > dyn_max) return fail(SNC_E_UNSUPPORTED, "batched trainer state exceeds LDS");
-  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-    return fail(SNC_E_UNSUPPORTED, "LDS request of the batched trainer rejected");
+  const hipError_t ea = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (ea != hipSuccess) {
+    hip_fail(ea, "LDS request of the batched trainer");
+    return SNC_E_UNSUPPORTED;
+  }
   TrainArgs aa = a;
   aa.lds_bytes = (int)lds;
   void* args[] = {(void*)&aa};
-  if (hipLaunchKernel(fn, dim3(B), dim3(kTrainThreads), args, lds, st) != hipSuccess)
-    return fail(SNC_E_HIP, "batched trainer launch failed");
+  const hipError_t e = hipLaunchKernel(fn, dim3(B), dim3(kTrainThreads), args, lds, st);
+  if (e != hipSuccess) return hip_fail(e, "batched trainer launch");
   return SNC_OK;
 }
 
 int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, hipStream_t st) {
   hipLaunchKernelGGL(k_p_begin, dim3(1), dim3(1), 0, st, ps, b, sample, epoch);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "plain begin launch failed");
+  return hip_last("plain begin launch failed");
 }
 
 int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int T, bool f64,
@@ -633,20 +637,20 @@ int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int T, b
     else
       hipLaunchKernelGGL(k_p_contrib<float>, dim3(gs), dim3(256), 0, st, a, ps, T);
   }
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "plain trainer launch failed");
+  return hip_last("plain trainer launch failed");
 }
 
 int train_launch_apply(const TrainArgs& a, float* w, int B, const int32_t* csc_e, float* wc,
                        int64_t P, hipStream_t st) {
   hipLaunchKernelGGL(k_apply, dim3(blocks_for(a.S, 256)), dim3(256), 0, st, a, w, B);
   hipLaunchKernelGGL(k_gather_wc, dim3(blocks_for(P, 256)), dim3(256), 0, st, w, csc_e, wc, P);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "update launch failed");
+  return hip_last("update launch failed");
 }
 
 int train_launch_gather(const float* w, const int32_t* csc_e, float* wc, int64_t P,
                         hipStream_t st) {
   hipLaunchKernelGGL(k_gather_wc, dim3(blocks_for(P, 256)), dim3(256), 0, st, w, csc_e, wc, P);
-  return hipGetLastError() == hipSuccess ? SNC_OK : fail(SNC_E_HIP, "gather launch failed");
+  return hip_last("gather launch failed");
 }
 
 }  // namespace snc_amd
