@@ -39,27 +39,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_units(g, heavy_threshold, neuron_spikes, steps):
+def kernel_units(g, neuron_spikes, steps):
     """Work per kernel class over one run: (neurons stepped, events expanded,
-    rows fired, launches). The hub kernel steps the heavy targets and expands
-    their own rows; the step kernel does the rest."""
+    rows fired). One fused step kernel steps every neuron, hub targets
+    included."""
     import numpy as np
 
-    n = g.num_neurons
-    indeg = np.bincount(g.post_ids, minlength=n) if g.num_synapses else np.zeros(n, np.int64)
     outdeg = np.diff(g.row_ptr).astype(np.int64)
     spk = neuron_spikes.astype(np.int64)
-    hub = indeg > heavy_threshold
-    ev_hub, f_hub = int((spk * outdeg)[hub].sum()), int(spk[hub].sum())
     ev_all, f_all = int((spk * outdeg).sum()), int(spk.sum())
-    n_hub = int(hub.sum())
-    return {
-        "k_hub_step": (n_hub * steps, ev_hub, f_hub),
-        "k_step_with_hubs": ((n - n_hub) * steps, ev_all - ev_hub, f_all - f_hub),
-        "k_step": (n * steps, ev_all, f_all),
-        "sort_step": (n * steps, ev_all, f_all),
-        "k_persist": (n * steps, ev_all, f_all),
-    }
+    n = g.num_neurons
+    return {name: (n * steps, ev_all, f_all) for name in ("k_step", "sort_step", "k_persist")}
 
 
 def measured_traffic(kernel_symbol):
@@ -262,11 +252,10 @@ def batched_forward(w, hg, B, steps, warmup, variant):
     synchronize()
     wall = (time.perf_counter() - t1) / max(1, steps)
     prof = plan.profile()
-    units = kernel_units(g, plan.config.heavy_threshold, r.neuron_spikes.sum(axis=0), w.num_steps)
+    units = kernel_units(g, r.neuron_spikes.sum(axis=0), w.num_steps)
     kernels = {}
     for name, (tot, k) in prof.items():
-        key = "k_step_with_hubs" if name == "k_step" and "k_hub_step" in prof else name
-        nu, ev_u, f_u = units[key]
+        nu, ev_u, f_u = units[name]
         by = model_bytes(B * nu, ev_u, f_u) / k  # neuron terms scale with B
         kernels[name] = {"avg_ms": tot / k, "launches": k, "bytes_per_launch": by,
                          "gbs": by / (tot / k / 1e3) / 1e9}
@@ -413,11 +402,10 @@ def main():
         return
 
     prof = plan.profile()
-    units = kernel_units(g, plan.config.heavy_threshold, ref.neuron_spikes, w.num_steps)
+    units = kernel_units(g, ref.neuron_spikes, w.num_steps)
     kernels = {}
     for name, (tot, k) in prof.items():
-        key = "k_step_with_hubs" if name == "k_step" and "k_hub_step" in prof else name
-        by = model_bytes(*units[key]) / k
+        by = model_bytes(*units[name]) / k
         avg = tot / k
         kernels[name] = {"avg_ms": avg, "launches": k, "total_ms": tot,
                          "bytes_per_launch": by,

@@ -208,13 +208,12 @@ int snc_hip_plan_kernel_time(const snc_hip_plan* p, double* avg_ms, int64_t* lau
 void snc_hip_plan_free(snc_hip_plan* p);
 
 /* Per-kernel device time of one instrumented run: eager launches with a HIP
- * event around every kernel on the plan's stream (the hub and the regular
- * step kernel one after the other, where the launched graph overlaps them).
- * kernel_ms_total[k] and launches[k] are filled for each class
- * k = SNC_KERNEL_*:
- *   HUB_STEP    k_hub_step, one block per heavy target (hip-bucket)
- *   STEP        k_step: fold, integrate/fire and expansion of the regular
- *               neurons (all neurons for hip-atomic)
+ * event around every kernel on the plan's stream. kernel_ms_total[k] and
+ * launches[k] are filled for each class k = SNC_KERNEL_*:
+ *   HUB_STEP    kept for compatibility; hub targets are stepped inside k_step
+ *               (its leading blocks), so this class reports no launches
+ *   STEP        k_step: fold, integrate/fire and expansion of every neuron,
+ *               one launch per step (hip-atomic and hip-bucket)
  *   SORT_STEP   the whole kernel chain of one hip-sort step
  *   PERSISTENT  k_persist, the whole run in one cooperative launch */
 #define SNC_KERNEL_HUB_STEP 0

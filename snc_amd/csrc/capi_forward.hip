@@ -119,8 +119,7 @@ struct snc_hip_plan {
   int32_t* heavy_blk_ptr = nullptr;
   int32_t *inj_blk_ptr = nullptr, *blk_nid = nullptr;
   float *blk_amp = nullptr, *inj_heavy = nullptr;
-  hipStream_t side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  int2* inj_tab = nullptr;
   size_t n_blk_slots = 0;
   int halves = 1;
   size_t dirty_stride = 0;
@@ -161,12 +160,10 @@ struct snc_hip_plan {
     dfree(inj_blk_ptr);
     dfree(blk_nid);
     dfree(blk_amp);
+    dfree(inj_tab);
     dfree(inj_heavy);
     dfree(heavy_ids);
     dfree(is_heavy);
-    if (fork) (void)hipEventDestroy(fork);
-    if (join) (void)hipEventDestroy(join);
-    if (side) (void)hipStreamDestroy(side);
     dfree(keys);
     dfree(keys_alt);
     dfree(vals);
@@ -488,6 +485,9 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   P->n_blk_slots = (size_t)std::max(4096, R + n_heavy + 1);
   if (!rc) rc = dalloc(&P->blk_spikes, (size_t)B * P->n_blk_slots);
   if (!rc) rc = dalloc(&P->blk_events, (size_t)B * P->n_blk_slots);
+  const bool fused = !P->persistent && variant != SNC_VARIANT_SORT;
+  if (!rc && fused && num_steps > 0)
+    rc = dalloc(&P->inj_tab, (size_t)B * num_steps * R * kInjInline);
   if (variant == SNC_VARIANT_SORT && (uint64_t)n * (uint64_t)g.L >= (1ull << 32))
     return fail(SNC_E_UNSUPPORTED, "hip-sort keys need n * ring_len < 2^32");
   if (!rc && variant == SNC_VARIANT_SORT) rc = sort_alloc(P.get());
@@ -501,9 +501,6 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   P->halves = halves;
   if (rc) return rc;
   if (hipStreamCreateWithFlags(&P->core.stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&P->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&P->join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&P->ev0) != hipSuccess || hipEventCreate(&P->ev1) != hipSuccess)
     return fail(SNC_E_HIP, "stream/event creation failed");
 
@@ -549,6 +546,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   a.blk_nid = P->blk_nid;
   a.blk_amp = P->blk_amp;
   a.inj_heavy = P->inj_heavy;
+  a.inj_tab = P->inj_tab;
   a.is_heavy = P->is_heavy;
   a.n_reg_blocks = R;
   a.blk = nb;
@@ -563,7 +561,16 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   a.blk_stride = (int64_t)P->n_blk_slots;
   a.inj_stride = (int64_t)num_steps * (R + 1);
   a.hinj_stride = (int64_t)num_steps * n_heavy;
+  a.tab_stride = (int64_t)num_steps * R * kInjInline;
   a.raster_stride = (int64_t)num_steps * a.words;
+  if (P->inj_tab) {
+    rc = fwd_launch_inj_table(P->core.stream, a);
+    if (!rc) {
+      const hipError_t es = hipStreamSynchronize(P->core.stream);
+      if (es != hipSuccess) rc = hip_fail(es, "injection table build");
+    }
+    if (rc) return rc;
+  }
 
   *out = P.release();
   return SNC_OK;
@@ -655,26 +662,12 @@ int ensure_raster(snc_hip_plan* P) {
   return dalloc(&P->raster, (size_t)P->B * std::max(1, P->T) * P->a.words);
 }
 
-// The step chain. While capturing, each step's hub kernel runs on a forked
-// side stream so it overlaps the regular step kernel; eagerly both go to the
-// plan's stream.
-int launch_chain(snc_hip_plan* P, const StepArgs& a, bool capturing) {
-  const bool fork = capturing && a.n_heavy > 0 && P->variant == SNC_VARIANT_BUCKET;
+// The step chain: one fused launch per step (hip-atomic, hip-bucket) or the
+// hip-sort kernel chain of each step, all on the plan's stream.
+int launch_chain(snc_hip_plan* P, const StepArgs& a) {
   for (int t = 0; t < P->T; ++t) {
-    int rc;
-    if (P->variant == SNC_VARIANT_SORT) {
-      rc = sort_launch_step(P->core, a, t);
-    } else if (fork) {
-      if (hipEventRecord(P->fork, P->core.stream) != hipSuccess ||
-          hipStreamWaitEvent(P->side, P->fork, 0) != hipSuccess)
-        return fail(SNC_E_HIP, "fork failed");
-      rc = fused_launch_step(P->core, a, t, P->side);
-      if (!rc && (hipEventRecord(P->join, P->side) != hipSuccess ||
-                  hipStreamWaitEvent(P->core.stream, P->join, 0) != hipSuccess))
-        rc = fail(SNC_E_HIP, "join failed");
-    } else {
-      rc = fused_launch_step(P->core, a, t, nullptr);
-    }
+    const int rc = P->variant == SNC_VARIANT_SORT ? sort_launch_step(P->core, a, t)
+                                                   : fused_launch_step(P->core, a, t);
     if (rc) return rc;
   }
   return SNC_OK;
@@ -689,13 +682,13 @@ int launch_loop(snc_hip_plan* P, bool raster) {
   StepArgs a = P->a;
   a.raster = raster ? P->raster : nullptr;
   if (P->persistent) return persist_launch(P->G->g, P->core, a, P->pcfg);
-  if (!use_graphs() || P->T == 0) return launch_chain(P, a, false);
+  if (!use_graphs() || P->T == 0) return launch_chain(P, a);
   hipGraphExec_t& ge = P->graph_exec[raster ? 1 : 0];
   if (!ge) {
     hipGraph_t graph = nullptr;
     const hipError_t eb = hipStreamBeginCapture(P->core.stream, hipStreamCaptureModeThreadLocal);
     if (eb != hipSuccess) return hip_fail(eb, "hipStreamBeginCapture");
-    int rc = launch_chain(P, a, true);
+    int rc = launch_chain(P, a);
     hipError_t e = hipStreamEndCapture(P->core.stream, &graph);
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -852,9 +845,6 @@ int snc_hip_plan_profile(snc_hip_plan* P, double* kernel_ms, int64_t* launches) 
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
     if (!rc) span(SNC_KERNEL_PERSISTENT, ev[0], ev[1]);
   } else {
-    // Fused path: the hub kernel (hip-bucket) and the step kernel timed
-    // separately, one after the other on the plan's stream.
-    const bool heavy = P->variant == SNC_VARIANT_BUCKET && a.n_heavy > 0;
     for (int t = 0; t < P->T && !rc; ++t) {
       (void)hipEventRecord(ev[0], st);
       if (P->variant == SNC_VARIANT_SORT) {
@@ -864,14 +854,11 @@ int snc_hip_plan_profile(snc_hip_plan* P, double* kernel_ms, int64_t* launches) 
         if (!rc) span(SNC_KERNEL_SORT_STEP, ev[0], ev[3]);
         continue;
       }
-      rc = fused_launch_hub(P->core, a, t, st);
+      rc = fused_launch_step(P->core, a, t);
       (void)hipEventRecord(ev[1], st);
-      if (!rc) rc = fused_launch_regular(P->core, a, t);
-      (void)hipEventRecord(ev[2], st);
       if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
       if (rc) break;
-      if (heavy) span(SNC_KERNEL_HUB_STEP, ev[0], ev[1]);
-      span(SNC_KERNEL_STEP, ev[1], ev[2]);
+      span(SNC_KERNEL_STEP, ev[0], ev[1]);
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);

@@ -18,6 +18,13 @@ struct LifK {
   int subtract;
 };
 
+// Inline injection table of the fused step kernel: per (sample, step, block)
+// kInjInline entries {block-local neuron, amplitude bits}, unused entries
+// {-1, 0}. A (step, block) with more injections marks its last entry
+// {kInjSpill, 0} and the kernel reads the per-(step, block) list instead.
+constexpr int kInjInline = 16;
+constexpr int kInjSpill = -2;
+
 // Everything a step kernel touches; passed by value.
 struct StepArgs {
   int n, L, T, words;
@@ -61,6 +68,7 @@ struct StepArgs {
   const int32_t* inj_blk_ptr;  // [T][n_reg_blocks + 1] per-(step, block) injection ranges
   const int32_t* blk_nid;      // injections in (step, block) order, heavy targets excluded
   const float* blk_amp;
+  const int2* inj_tab;         // [T][n_reg_blocks][kInjInline] inline injection table
   const float* inj_heavy;      // [T][n_heavy] injections of heavy targets (hip-bucket)
   const uint8_t* is_heavy;     // [n]
   int n_reg_blocks;
@@ -76,6 +84,7 @@ struct StepArgs {
   int64_t dirty_stride;   // both dirty halves of one sample
   int64_t blk_stride;     // stats slots of one sample
   int64_t inj_stride;     // T * (n_reg_blocks + 1)
+  int64_t tab_stride;     // T * n_reg_blocks * kInjInline
   int64_t hinj_stride;    // T * n_heavy
   int64_t raster_stride;  // T * words
 };
@@ -93,6 +102,7 @@ __host__ __device__ inline StepArgs sample_args(const StepArgs& a0, int smp) {
     a.blk_spikes += (size_t)smp * a.blk_stride;
     a.blk_events += (size_t)smp * a.blk_stride;
     a.inj_blk_ptr += (size_t)smp * a.inj_stride;
+    if (a.inj_tab) a.inj_tab += (size_t)smp * a.tab_stride;
     if (a.inj_heavy) a.inj_heavy += (size_t)smp * a.hinj_stride;
     if (a.raster) a.raster += (size_t)smp * a.raster_stride;
   }
@@ -111,11 +121,11 @@ struct FwdPlanCore {
 int fwd_launch_fill(hipStream_t st, void* p, uint32_t value, size_t words);
 int fwd_launch_lif(const FwdPlanCore& pc, const StepArgs& a, int t);  // hip-sort
 int fwd_launch_first_injection(const FwdPlanCore& pc, const StepArgs& a);
+// Fills a.inj_tab from the per-(sample, step, block) lists.
+int fwd_launch_inj_table(hipStream_t st, const StepArgs& a);
 
-// fused per-step kernels (step.hip); hub_stream may be null (same stream)
-int fused_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_t hub_stream);
-int fused_launch_hub(const FwdPlanCore& pc, const StepArgs& a, int t, hipStream_t st);
-int fused_launch_regular(const FwdPlanCore& pc, const StepArgs& a, int t);
+// fused per-step kernel (step.hip): one launch per step, hub blocks included
+int fused_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t);
 bool fused_block_ok(int block);
 int fused_light_max_in_degree();  // largest in-degree the step kernel folds per lane
 
