@@ -144,6 +144,35 @@ int snc_encode(int32_t enc, const float* features, int32_t n_channels, int32_t n
 void snc_make_synthetic(int32_t n, int32_t dim, int32_t classes, float noise, uint64_t seed,
                         float* X, int32_t* y);
 
+/* ---- SNCG flat graph files (the reference's PyTorch-bridge format) ----
+ * Layout, little-endian int32 (src/bench/snc_export.cpp:10-12, 107-123;
+ * python/snc/graph.py:11-24): magic 0x534E4347 'SNCG', N, S,
+ * num_input_channels, num_output_channels, pre[S], post[S], delays[S],
+ * role[N], channel[N]. Weights and signs are not stored. */
+
+/* num_input_channels / num_output_channels as SNNGraph::rebuild_io_index
+ * computes them (src/graph/snn_graph.cpp:13-28). */
+int snc_graph_io_channels(const snc_graph_view* g, int32_t* n_in, int32_t* n_out);
+/* snc_export's file write (snc_export.cpp:107-123); the graph is validated first. */
+int snc_graph_write_sncg(const snc_graph_view* g, const char* path);
+/* load_graph (python/snc/graph.py:11-24) into a CSR buffer: rows in file order
+ * per pre, every weight = `weight`, every sign +1. Rejects a bad magic, a size
+ * that does not match N and S, ids out of range, delays < 1 and unknown roles.
+ * n_in / n_out (nullable) receive the header's channel counts. */
+int snc_graph_read_sncg(const char* path, float weight, snc_graph_buf** out, int32_t* n_in,
+                        int32_t* n_out);
+
+/* snc_export's heterogeneous delays (snc_export.cpp:81-101), applied in place
+ * when delay_max > 1:
+ *   SNC_DELAYS_RANDOM    uniform in [1, delay_max] from mt19937_64(seed ^ 0xDE1A0)
+ *   SNC_DELAYS_DISTANCE  1 + round(|pre - (post - A)| / (max(A, B) - 1) * (delay_max - 1)),
+ *                        A, B = layers[0], layers[1]; needs n_layers >= 2, else the
+ *                        random rule applies, as in the exporter */
+#define SNC_DELAYS_RANDOM 0
+#define SNC_DELAYS_DISTANCE 1
+int snc_graph_spread_delays(snc_graph_buf* b, int32_t mode, int32_t delay_max,
+                            const int32_t* layers, int32_t n_layers, uint64_t seed);
+
 #ifdef __cplusplus
 }
 #endif

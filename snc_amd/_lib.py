@@ -216,6 +216,14 @@ _SIGS = {
                              C.c_float, C.c_float, C.c_uint64, C.c_int32, C.POINTER(_P)]),
     "snc_make_synthetic": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
                                   C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
+    # SNCG files (snc_gen.h)
+    "snc_graph_io_channels": (C.c_int, [C.POINTER(GraphView), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)]),
+    "snc_graph_write_sncg": (C.c_int, [C.POINTER(GraphView), C.c_char_p]),
+    "snc_graph_read_sncg": (C.c_int, [C.c_char_p, C.c_float, C.POINTER(_P),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "snc_graph_spread_delays": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                          C.c_int32, C.c_uint64]),
 }
 
 _lib = None

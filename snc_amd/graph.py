@@ -81,6 +81,40 @@ class SNNGraph:
         return np.repeat(np.arange(self.num_neurons, dtype=np.int32), np.diff(self.row_ptr))
 
 
+DELAYS_RANDOM, DELAYS_DISTANCE = 0, 1
+SNCG_MAGIC = 0x534E4347
+
+
+def write_sncg(g: SNNGraph, path) -> None:
+    """Write g as an SNCG file, the format of the reference's snc_export
+    (src/bench/snc_export.cpp:107-123): topology, delays, roles and channels;
+    no weights."""
+    L.check(L.lib().snc_graph_write_sncg(C.byref(g.view()), str(path).encode()))
+
+
+def read_sncg(path, weight: float = 0.5) -> tuple[SNNGraph, int, int]:
+    """load_graph (python/snc/graph.py:11-24) into an SNNGraph: every weight
+    is `weight` and every sign +1, since the file holds neither. Returns
+    (graph, num_input_channels, num_output_channels) from the header."""
+    out, n_in, n_out = C.c_void_p(), C.c_int32(), C.c_int32()
+    L.check(L.lib().snc_graph_read_sncg(str(path).encode(), weight, C.byref(out),
+                                        C.byref(n_in), C.byref(n_out)))
+    return _from_buf(out), int(n_in.value), int(n_out.value)
+
+
+def spread_delays(g: SNNGraph, mode: int, delay_max: int, layers=(), seed: int = 1) -> SNNGraph:
+    """snc_export's heterogeneous delays (snc_export.cpp:81-101) on a copy of g."""
+    lib = L.lib()
+    buf = C.c_void_p()
+    L.check(lib.snc_graph_buf_from_view(C.byref(g.view()), C.byref(buf)))
+    lay = np.ascontiguousarray(layers, dtype=np.int32)
+    rc = lib.snc_graph_spread_delays(buf, mode, delay_max, L.ptr(lay, C.c_int32), lay.size, seed)
+    if rc:
+        lib.snc_graph_buf_free(buf)
+        L.check(rc)
+    return _from_buf(buf)
+
+
 def _from_buf(buf) -> SNNGraph:
     lib = L.lib()
     v = L.GraphView()
