@@ -52,6 +52,38 @@ def kernel_units(g, neuron_spikes, steps):
     return {name: (n * steps, ev_all, f_all) for name in ("k_step", "sort_step", "k_persist")}
 
 
+def scatter_floor(g, cfg, events_per_step, kernel_ms):
+    """The device's time floor for the fused step kernel's delivery: one
+    launch of the same shape doing events_per_step scattered device-scope
+    atomics of the kernel's kind over a buffer the size of its target (the
+    ring, or the arrival bitmaps), with nothing else (snc_hip_measure_scatter).
+    frac = floor / kernel time: how close the step is to what its atomics
+    alone cost on this device."""
+    import numpy as np
+
+    from snc_amd.runtime import (SCATTER_ATOMIC_ADD, SCATTER_ATOMIC_OR, measure_scatter,
+                                 parse_backend)
+
+    n = g.num_neurons
+    blocks = -(-n // cfg.block_neurons)
+    L = g.max_delay + 1
+    if cfg.variant == parse_backend("hip-bucket"):
+        indeg = np.bincount(g.post_ids, minlength=n)
+        P = int((((indeg + 31) // 32) * 32).sum())
+        kind, target, what = SCATTER_ATOMIC_OR, L * P // 8, "u32 atomicOr into arrival bitmaps"
+    else:
+        kind, target, what = SCATTER_ATOMIC_ADD, L * n * 4, "float atomicAdd into the ring"
+    ops = int(round(events_per_step))
+    floor_us = measure_scatter(kind, target, ops, blocks)
+    empty_us = measure_scatter(kind, target, 0, blocks)
+    k_us = kernel_ms * 1e3
+    return {"access": what, "ops_per_launch": ops, "target_bytes": int(target),
+            "blocks": blocks, "floor_us": floor_us, "empty_launch_us": empty_us,
+            "kernel_us": k_us, "frac": floor_us / k_us if k_us > 0 else None,
+            "ops_per_s": ops / (k_us * 1e-6) if k_us > 0 else None,
+            "floor_ops_per_s": ops / (floor_us * 1e-6) if floor_us > 0 else None}
+
+
 def measured_traffic(kernel_symbol):
     """HBM bytes per dispatch of a kernel from the committed counter summaries
     (profiles/*pmc_traffic*.md: calibrated FETCH_SIZE + WRITE_SIZE), or None."""
@@ -426,6 +458,9 @@ def main():
                 "whole_run": {"bytes": step_bytes, "ms": ms_per_step,
                               "gbs": step_bytes / (ms_per_step / 1e3) / 1e9,
                               "frac": step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS}}
+    if dom == "k_step":
+        roofline["scatter_floor"] = scatter_floor(g, cfgp, events / w.num_steps, dk["avg_ms"])
+        log(f"scatter floor: {roofline['scatter_floor']}")
 
     variants = {}
     for name in [v for v in args.compare.split(",") if v]:

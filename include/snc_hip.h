@@ -226,6 +226,20 @@ int snc_hip_plan_profile(snc_hip_plan* p, double* kernel_ms_total, int64_t* laun
 /* Wait until all work the library queued on the device has finished. */
 int snc_hip_synchronize(void);
 
+/* Device ceiling for scattered delivery (no counterpart in the reference: a
+ * measurement the benchmark's roofline uses). Times `reps` graph-launched
+ * launches, each doing `ops` accesses of one kind at hashed addresses over a
+ * fresh buffer of target_bytes, from `blocks` blocks of 512 threads (the
+ * fused step kernel's shape), and returns the best average microseconds per
+ * launch. The step kernel's delivery does one such access per synaptic event,
+ * so this is the time floor of a step with that many events. */
+#define SNC_SCATTER_ATOMIC_ADD 0 /* float atomicAdd (hip-atomic delivery)    */
+#define SNC_SCATTER_ATOMIC_OR 1  /* u32 atomicOr (hip-bucket delivery)       */
+#define SNC_SCATTER_STORE 2      /* plain 4-byte store                       */
+#define SNC_SCATTER_LOAD 3       /* plain 4-byte load (hip-bucket's gathers) */
+int snc_hip_measure_scatter(int32_t kind, int64_t target_bytes, int64_t ops, int32_t blocks,
+                            int32_t reps, double* us_per_launch);
+
 /* One-shot forward, the shape of cuda::forward: upload, run, read back, free. */
 int snc_hip_forward(const snc_graph_view* g, const snc_event* events, int64_t n_events,
                     int32_t num_steps, const snc_lif* lif, int32_t variant,
@@ -238,6 +252,9 @@ int snc_hip_forward(const snc_graph_view* g, const snc_event* events, int64_t n_
 
 #define SNC_ENC_DIRECT 0
 #define SNC_ENC_POISSON 1 /* on-device Philox Bernoulli, see snc_hip_train_batch */
+#define SNC_ENC_LATENCY 2 /* one spike of `gain` per channel at step
+                             lround((T_i - 1) * (1 - min(1, x))), encoders.cpp:56-66,
+                             T_i the sample's length */
 
 /* TrainConfig (include/snc/trainer.hpp:30-48) plus the readout mode. */
 typedef struct snc_train_config {
@@ -283,7 +300,7 @@ int snc_hip_train_get_weights(const snc_hip_train* s, float* w, int64_t n);
 
 /* Upload a whole dataset once; later steps index it. Features: [n][dim]
  * values in [0, 1] encoded on the device (cfg.encoder: Philox Poisson with
- * counter (channel / 4, step, sample index, epoch), or Direct). Events:
+ * counter (channel / 4, step, sample index, epoch), Direct, or Latency). Events:
  * per-sample InputEvent lists (sample_id = dataset index). lengths[i] is T_i
  * (NULL: cfg.num_steps for every sample). */
 int snc_hip_train_upload_features(snc_hip_train* s, const float* X, int32_t n, int32_t dim,

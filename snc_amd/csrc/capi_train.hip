@@ -152,6 +152,9 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
                          int32_t max_steps, int32_t elig, snc_hip_train** out) {
   if (!g || !cfg || !out || batch < 1 || max_steps < 1) return fail(SNC_E_INVALID, "bad arguments");
   *out = nullptr;
+  if (cfg->encoder != SNC_ENC_DIRECT && cfg->encoder != SNC_ENC_POISSON &&
+      cfg->encoder != SNC_ENC_LATENCY)
+    return fail(SNC_E_INVALID, "unknown encoder");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(SNC_E_NODEVICE, "no device");
   const std::string err = validate_view(*g);

@@ -51,6 +51,15 @@ constexpr int kTrainThreads = 1024;
 // ---------------------------------------------------------------------------
 
 // Drive of input neuron j at step t of dataset sample `sample`.
+// Time-to-first-spike step of a feature (encoders.cpp:56-66): brighter is
+// earlier, lround((T - 1) * (1 - min(1, x))) clamped to [0, T - 1], in the
+// host's float arithmetic (one product, rounded half away from zero).
+__device__ __forceinline__ int latency_step(float x, int T) {
+  const float xc = fminf(1.0f, x);
+  const int ts = (int)lroundf(__fmul_rn((float)(T - 1), __fsub_rn(1.0f, xc)));
+  return min(max(ts, 0), T - 1);
+}
+
 __device__ __forceinline__ float input_drive(const TrainArgs& a, int j, int sample, int t,
                                              int epoch) {
   const int c = a.channel[j];
@@ -58,6 +67,7 @@ __device__ __forceinline__ float input_drive(const TrainArgs& a, int j, int samp
   const float x = a.X[(size_t)sample * a.dim + c];
   if (!(x > a.enc_thr)) return 0.0f;
   if (a.encoder == SNC_ENC_DIRECT) return x * a.gain;
+  if (a.encoder == SNC_ENC_LATENCY) return t == latency_step(x, a.lengths[sample]) ? a.gain : 0.0f;
   const Philox4 r = philox4x32_10((uint32_t)(c >> 2), (uint32_t)t, (uint32_t)sample,
                                   (uint32_t)epoch, a.seed_lo, a.seed_hi);
   const float p = fminf(1.0f, x * a.max_rate);
@@ -248,16 +258,21 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     s_rowp[j] = a.row_ptr[j];
     s_ccol[j] = a.ccol[j];
   }
-  // Input drive of this sample: the direct value, or the Poisson spike
-  // probability (0: the channel never fires), from the features once.
+  // Input drive of this sample, from the features once: the direct value,
+  // the Poisson spike probability (0: the channel never fires), or the
+  // latency spike step as int bits (-1: never).
   if (!a.ev_base)
     for (int q = tid; q < a.n_inputs; q += kTrainThreads) {
       const int j = a.inputs[q];
       const int c = a.channel[j];
-      float val = 0.0f;
+      const bool lat = a.encoder == SNC_ENC_LATENCY;
+      float val = lat ? __int_as_float(-1) : 0.0f;
       if (c >= 0 && c < a.dim) {
         const float x = a.X[(size_t)sample * a.dim + c];
-        if (x > a.enc_thr) val = a.encoder == SNC_ENC_DIRECT ? x * a.gain : fminf(1.0f, x * a.max_rate);
+        if (x > a.enc_thr)
+          val = a.encoder == SNC_ENC_DIRECT ? x * a.gain
+                : lat                       ? __int_as_float(latency_step(x, T))
+                                            : fminf(1.0f, x * a.max_rate);
       }
       s_inp[q] = j;
       s_inpx[q] = val;
@@ -281,6 +296,9 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     } else if (a.encoder == SNC_ENC_DIRECT) {
       for (int q = tid; q < a.n_inputs; q += kTrainThreads)
         if (s_inpx[q] != 0.0f) s_ext[s_inp[q]] = s_inpx[q];
+    } else if (a.encoder == SNC_ENC_LATENCY) {
+      for (int q = tid; q < a.n_inputs; q += kTrainThreads)
+        if (__float_as_int(s_inpx[q]) == t) s_ext[s_inp[q]] = a.gain;
     } else {
       for (int q = tid; q < a.n_inputs; q += kTrainThreads) {
         const float p = s_inpx[q];

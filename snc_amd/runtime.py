@@ -98,6 +98,20 @@ def synchronize():
     L.check(L.lib().snc_hip_synchronize())
 
 
+SCATTER_ATOMIC_ADD, SCATTER_ATOMIC_OR, SCATTER_STORE, SCATTER_LOAD = 0, 1, 2, 3
+
+
+def measure_scatter(kind: int, target_bytes: int, ops: int, blocks: int, reps: int = 200) -> float:
+    """Best average microseconds per launch of `ops` scattered accesses of one
+    kind (SCATTER_*) over target_bytes, from `blocks` blocks of 512 threads,
+    graph-launched `reps` times: the device's floor for a step kernel that
+    delivers `ops` events (snc_hip_measure_scatter)."""
+    us = C.c_double()
+    L.check(L.lib().snc_hip_measure_scatter(kind, int(target_bytes), int(ops), int(blocks),
+                                            int(reps), C.byref(us)))
+    return us.value
+
+
 def available() -> bool:
     """hip::available(): library present and a gfx950 device usable."""
     try:

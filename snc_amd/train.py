@@ -50,7 +50,7 @@ class TrainConfig:
     train_hidden: bool = True
     readout_mode: int = READOUT_SPIKING
     readout_psi: float = 1.0
-    encoder: int = 1           # 0 direct, 1 poisson (Philox on device)
+    encoder: int = 1           # 0 direct, 1 poisson (Philox on device), 2 latency
     enc_gain: float = 1.0
     enc_max_rate: float = 0.5
     enc_threshold: float = 0.0

@@ -123,3 +123,13 @@ def test_background_drive_rate_and_determinism():
         i, t = int(a["channel_id"][k]), int(a["time"][k])
         r = O.philox4x32([i - i % 4, t, 0x5EED, 0], [4, 0])
         assert r[i % 4] < thr
+
+
+def test_trainer_rejects_unknown_encoder():
+    """snc_hip_train_create checks the encoder (direct, poisson, latency)
+    before it looks for a device."""
+    from snc_amd.train import HipTrainSession, TrainConfig
+
+    g = G.static_snc([20, 10, 4], 100, 0.5, 1, 0.2, 1)
+    with pytest.raises(L.HipError, match="unknown encoder"):
+        HipTrainSession(g, TrainConfig(num_steps=10, encoder=3), batch=2, max_steps=10)

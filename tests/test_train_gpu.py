@@ -38,8 +38,8 @@ def _events_for(X, lengths, enc=0, max_rate=0.5, seed=3, epoch=0, ids=None):
     evs = []
     for b in range(len(lengths)):
         sid = b if ids is None else int(ids[b])
-        if enc == 0:
-            e = O.encode(0, X[b], int(lengths[b]))
+        if enc in (0, 2):
+            e = O.encode(enc, X[b], int(lengths[b]))
         else:
             e = O.encode_philox(X[b], int(lengths[b]), max_rate=max_rate, seed=seed, sample=sid,
                                 epoch=epoch)
@@ -149,6 +149,27 @@ def test_device_poisson_and_extensions_match_oracle(readout, reset_mode):
     w0 = orc.weights
     ev = _events_for(X, lengths, enc=1, max_rate=0.3, seed=3, epoch=0, ids=ids)
     _compare(sess, orc, w0, ids, ev, lengths, y, BATCHED)
+
+
+@pytest.mark.parametrize("mode", [BATCHED, PLAIN])
+def test_device_latency_matches_oracle(mode):
+    """Latency encoding on the device (one spike per channel at
+    lround((T_b - 1)(1 - x))) against the reference encode() of each sample's
+    own length, uneven T_b."""
+    g = G.sheet_graph([196, 96, 10], 24, recurrent_fan_in=12, radius=2.5, sheet=14.0,
+                      w_std=0.3, delay=1, seed=7)
+    B = 10
+    Xu, y = I.prototype_images(B, 196, 10, 0.6, 0.2, seed=11)
+    X = Xu.astype(np.float32) / np.float32(255.0)
+    lengths = np.array([30, 45, 12, 50, 33, 40, 27, 50, 18, 44], np.int32)
+    gcfg, ocfg = _cfgs(50, encoder=2, lr=0.5, w_init=0.6)
+    sess = HipTrainSession(g, gcfg, batch=B, max_steps=50)
+    sess.upload_features(X, y, lengths)
+    orc = O.Trainer(g, ocfg)
+    w0 = orc.weights
+    ev = _events_for(X, lengths, enc=2)
+    assert ev.size == sum(int((X[b] > 0).sum()) for b in range(B))  # one spike per lit channel
+    _compare(sess, orc, w0, np.arange(B), ev, lengths, y, mode)
 
 
 def test_delay_rich_recurrent_ring_outside_lds():
