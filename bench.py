@@ -506,7 +506,7 @@ def main():
            "roofline": roofline, "variants": variants, "autotune": autotune}
     out["batched_forward"] = {}
     for bs in [int(x) for x in args.batch_samples.split(",") if x]:
-        for vn in ("hip-auto", "hip-auto-exact"):
+        for vn in ("hip-auto", "hip-auto-exact", "hip-pull"):
             out["batched_forward"][f"{vn}_x{bs}"] = batched_forward(w, hg, bs, max(1, args.steps // 2),
                                                                     1, vn)
     if not args.no_eprop:

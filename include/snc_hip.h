@@ -48,6 +48,10 @@ extern "C" {
 #define SNC_VARIANT_AUTO 8   /* hip-auto: autotuner picks per graph           */
 #define SNC_VARIANT_AUTO_EXACT 9 /* hip-auto-exact: autotuner over the bit-exact
                                     strategies (bucket, sort) only            */
+#define SNC_VARIANT_PULL 10  /* hip-pull: batched gather over sliced incoming
+                                columns, all samples of a plan in one pass;
+                                light targets bit-exact, hub targets within
+                                the atomic tolerance (deterministic) */
 
 /* ---- graph roles (GraphRole, snn_graph.hpp:29) ---------------------------- */
 #define SNC_ROLE_INTERNAL 0

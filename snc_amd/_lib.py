@@ -24,6 +24,7 @@ VARIANT_BUCKET = 6
 VARIANT_SORT = 7
 VARIANT_AUTO = 8
 VARIANT_AUTO_EXACT = 9
+VARIANT_PULL = 10
 
 ROLE_INTERNAL, ROLE_INPUT, ROLE_OUTPUT = 0, 1, 2
 KERNEL_NAMES = ["k_hub_step", "k_step", "sort_step", "k_persist"]

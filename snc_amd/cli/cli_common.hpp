@@ -59,6 +59,7 @@ inline int parse_variant(const std::string& s) {
   if (s == "hip-sort") return SNC_VARIANT_SORT;
   if (s == "hip-auto") return SNC_VARIANT_AUTO;
   if (s == "hip-auto-exact") return SNC_VARIANT_AUTO_EXACT;
+  if (s == "hip-pull") return SNC_VARIANT_PULL;
   return -1;
 }
 inline const char* variant_name(int v) {
@@ -68,6 +69,7 @@ inline const char* variant_name(int v) {
     case SNC_VARIANT_SORT: return "hip-sort";
     case SNC_VARIANT_AUTO: return "hip-auto";
     case SNC_VARIANT_AUTO_EXACT: return "hip-auto-exact";
+    case SNC_VARIANT_PULL: return "hip-pull";
     default: return "?";
   }
 }

@@ -34,7 +34,7 @@ void usage() {
   std::printf(
       "snc_hip_bench -- SNN forward benchmark on the MI355X backend\n"
       "  --encoder    direct|poisson|latency     (default poisson)\n"
-      "  --backend    hip-atomic|hip-bucket|hip-sort|hip-auto|hip-auto-exact\n"
+      "  --backend    hip-atomic|hip-bucket|hip-sort|hip-pull|hip-auto|hip-auto-exact\n"
       "  --structure  dense|random-sparse|static-snc\n"
       "  --num-steps N  --num-samples N  --hidden N  --dim N  --classes N\n"
       "  --synapse-budget S  --delay D  --threshold T  --decay D  --refractory R\n"

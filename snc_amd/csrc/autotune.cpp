@@ -14,6 +14,8 @@
 //     for single-sample plans.
 //   * persistent variants only when cooperative launch is available, for
 //     single-sample plans.
+//   * hip-pull for batched plans of up to 32 samples, at the 99th-percentile
+//     in-degree cap and at twice it.
 #include "autotune.hpp"
 
 #include <algorithm>
@@ -26,17 +28,26 @@ constexpr int kMaxHubs = 16384;
 constexpr int kThresholds[] = {64, 128, 256, 512};
 }  // namespace
 
-snc_plan_config default_plan_config(int variant) {
+int pull_default_cap(const DevGraph& g) {
+  if (g.in_degree.empty()) return 32;
+  std::vector<int32_t> d(g.in_degree);
+  const size_t q = std::min(d.size() - 1, d.size() * 99 / 100);
+  std::nth_element(d.begin(), d.begin() + (ptrdiff_t)q, d.end());
+  const int cap = (d[q] + 31) / 32 * 32;
+  return std::min(1024, std::max(32, cap));
+}
+
+snc_plan_config default_plan_config(const DevGraph& g, int variant) {
   snc_plan_config c{};
   c.variant = variant;
   c.block_neurons = 256;
-  c.heavy_threshold = kLightLimit;
+  c.heavy_threshold = variant == SNC_VARIANT_PULL ? pull_default_cap(g) : kLightLimit;
   c.persistent = 0;
   return c;
 }
 
 std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop,
-                                                 bool allow_sort) {
+                                                 bool allow_sort, int num_samples) {
   std::vector<snc_plan_config> out;
   auto add = [&](int variant, int block, int thr, int persistent) {
     if ((int)out.size() >= SNC_TUNE_MAX_CANDIDATES) return;
@@ -69,6 +80,11 @@ std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_o
   if (coop) {
     add(SNC_VARIANT_BUCKET, 256, kLightLimit, 1);
     if (!exact_only) add(SNC_VARIANT_ATOMIC, 256, kLightLimit, 1);
+  }
+  if (!exact_only && num_samples > 1 && num_samples <= 32) {
+    const int cap = pull_default_cap(g);
+    add(SNC_VARIANT_PULL, 256, cap, 0);
+    if (std::min(1024, 2 * cap) != cap) add(SNC_VARIANT_PULL, 256, std::min(1024, 2 * cap), 0);
   }
   return out;
 }

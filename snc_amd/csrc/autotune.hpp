@@ -14,14 +14,20 @@
 
 namespace snc_amd {
 
-// Default configuration of an explicitly named strategy.
-snc_plan_config default_plan_config(int variant);
+// Default configuration of an explicitly named strategy on graph g.
+snc_plan_config default_plan_config(const DevGraph& g, int variant);
 
-// Candidates for the search. exact_only drops hip-atomic; coop says whether
-// the persistent (cooperative) kernel may run; allow_sort whether hip-sort
-// may (single-sample plans only).
+// hip-pull's default light in-degree cap: the 99th percentile in-degree,
+// rounded up to a multiple of 32, within [32, 1024]. Targets above it are
+// stepped as hubs.
+int pull_default_cap(const DevGraph& g);
+
+// Candidates for the search. exact_only drops hip-atomic and hip-pull; coop
+// says whether the persistent (cooperative) kernel may run; allow_sort
+// whether hip-sort may (single-sample plans only); num_samples > 1 adds
+// hip-pull (plans of up to 32 samples).
 std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop,
-                                                 bool allow_sort);
+                                                 bool allow_sort, int num_samples);
 
 // Steps of the trial window for a sample of num_steps steps.
 int autotune_trial_steps(int num_steps);

@@ -21,6 +21,7 @@
 #include "autotune.hpp"
 #include "common.hpp"
 #include "hip_err.hpp"
+#include "pull.hpp"
 #include "device_graph.hpp"
 #include "forward_plan.hpp"
 #include "persist.hpp"
@@ -120,6 +121,9 @@ struct snc_hip_plan {
   int32_t *inj_blk_ptr = nullptr, *blk_nid = nullptr;
   float *blk_amp = nullptr, *inj_heavy = nullptr;
   int2* inj_tab = nullptr;
+  uint32_t* F = nullptr;      // hip-pull: fired words [L][n]
+  float* hub_part = nullptr;  // hip-pull: hub chunk partials
+  PullArgs pa{};
   size_t n_blk_slots = 0;
   int halves = 1;
   size_t dirty_stride = 0;
@@ -161,6 +165,8 @@ struct snc_hip_plan {
     dfree(blk_nid);
     dfree(blk_amp);
     dfree(inj_tab);
+    dfree(F);
+    dfree(hub_part);
     dfree(inj_heavy);
     dfree(heavy_ids);
     dfree(is_heavy);
@@ -343,14 +349,25 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   if (lif->reset_mode != SNC_RESET_TO_VALUE && lif->reset_mode != SNC_RESET_SUBTRACT)
     return fail(SNC_E_INVALID, "unknown reset mode");
   const int variant = cfg->variant;
-  if (variant != SNC_VARIANT_ATOMIC && variant != SNC_VARIANT_BUCKET && variant != SNC_VARIANT_SORT)
+  if (variant != SNC_VARIANT_ATOMIC && variant != SNC_VARIANT_BUCKET &&
+      variant != SNC_VARIANT_SORT && variant != SNC_VARIANT_PULL)
     return fail(SNC_E_INVALID, "unknown variant");
-  if (!fused_block_ok(cfg->block_neurons))
-    return fail(SNC_E_INVALID, "block_neurons must be 128, 256 or 512");
-  if (cfg->heavy_threshold < 1 || cfg->heavy_threshold > fused_light_max_in_degree())
-    return fail(SNC_E_INVALID, "heavy_threshold must be in [1, 512]");
+  const bool pull = variant == SNC_VARIANT_PULL;
+  if (pull) {
+    if (cfg->block_neurons != kPullBlock)
+      return fail(SNC_E_INVALID, "hip-pull runs 256-neuron blocks");
+    if (cfg->heavy_threshold < 1 || cfg->heavy_threshold > kPullMaxCap)
+      return fail(SNC_E_INVALID, "hip-pull heavy_threshold (light in-degree cap) must be in [1, 1024]");
+    if (cfg->persistent) return fail(SNC_E_UNSUPPORTED, "hip-pull has no persistent kernel");
+    if (B > kPullMaxSamples) return fail(SNC_E_UNSUPPORTED, "hip-pull plans hold at most 32 samples");
+  } else {
+    if (!fused_block_ok(cfg->block_neurons))
+      return fail(SNC_E_INVALID, "block_neurons must be 128, 256 or 512");
+    if (cfg->heavy_threshold < 1 || cfg->heavy_threshold > fused_light_max_in_degree())
+      return fail(SNC_E_INVALID, "heavy_threshold must be in [1, 512]");
+  }
   if (B > 1 && (variant == SNC_VARIANT_SORT || cfg->persistent))
-    return fail(SNC_E_UNSUPPORTED, "batched plans run hip-atomic or hip-bucket step kernels");
+    return fail(SNC_E_UNSUPPORTED, "batched plans run hip-atomic, hip-bucket or hip-pull");
   const DevGraph& g = G->g;
   std::unique_ptr<snc_hip_plan> P(new snc_hip_plan());
   P->G = G;
@@ -393,7 +410,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   const int n_in_ch = (int)g.chan_to_inputs.size();
   const int nb = cfg->block_neurons;
   const int R = std::max(1, (n + nb - 1) / nb);
-  const bool hubs = variant == SNC_VARIANT_BUCKET && n_heavy > 0;
+  const bool hubs = (variant == SNC_VARIANT_BUCKET || pull) && n_heavy > 0;
   std::vector<int32_t> hub_index(hubs ? n : 0, -1);
   for (int k = 0; hubs && k < n_heavy; ++k) hub_index[P->heavy_host[k]] = k;
   std::vector<int32_t> blk_ptr((size_t)B * num_steps * (R + 1), 0), bnid;
@@ -447,7 +464,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   // the fused kernels: one arrival bitmap per ring slot, [B][L][P/32], and no
   // ring. Persistent: ring plus bitmap and dirty flags double-buffered by
   // step parity.
-  P->persistent = B == 1 && persist_choose(g, variant, cfg->persistent != 0, P->pcfg);
+  P->persistent = B == 1 && !pull && persist_choose(g, variant, cfg->persistent != 0, P->pcfg);
   P->cfg.persistent = P->persistent ? 1 : 0;
   const bool bucket = variant == SNC_VARIANT_BUCKET;
   const bool slots = bucket && !P->persistent;
@@ -459,7 +476,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   const size_t dirty_planes = bucket && !slots ? 2 : 0;
   P->bits_total = (size_t)B * planes * P->n_bits_words;
   P->dirty_total = (size_t)B * dirty_planes * dirty_stride;
-  P->ring_total = slots ? 0 : (size_t)B * g.L * n;
+  P->ring_total = slots || pull ? 0 : (size_t)B * g.L * n;
   int rc = SNC_OK;
   if (!rc) rc = dupload(&P->heavy_ids, P->heavy_host.data(), P->heavy_host.size());
   if (!rc) rc = dupload(&P->is_heavy, is_heavy_h.data(), is_heavy_h.size());
@@ -485,7 +502,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   P->n_blk_slots = (size_t)std::max(4096, R + n_heavy + 1);
   if (!rc) rc = dalloc(&P->blk_spikes, (size_t)B * P->n_blk_slots);
   if (!rc) rc = dalloc(&P->blk_events, (size_t)B * P->n_blk_slots);
-  const bool fused = !P->persistent && variant != SNC_VARIANT_SORT;
+  const bool fused = !P->persistent && variant != SNC_VARIANT_SORT && !pull;
   if (!rc && fused && num_steps > 0)
     rc = dalloc(&P->inj_tab, (size_t)B * num_steps * R * kInjInline);
   if (variant == SNC_VARIANT_SORT && (uint64_t)n * (uint64_t)g.L >= (1ull << 32))
@@ -563,6 +580,79 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   a.hinj_stride = (int64_t)num_steps * n_heavy;
   a.tab_stride = (int64_t)num_steps * R * kInjInline;
   a.raster_stride = (int64_t)num_steps * a.words;
+  if (pull) {
+    // The light block's LDS injection pool holds the largest (step, block).
+    int64_t pool = 64;
+    for (int t = 0; t < num_steps; ++t)
+      for (int q = 0; q < R; ++q) {
+        int64_t tot = 0;
+        for (int smp = 0; smp < B; ++smp) {
+          const int32_t* bp = &blk_ptr[((size_t)smp * num_steps + t) * (R + 1) + q];
+          tot += bp[1] - bp[0];
+        }
+        pool = std::max(pool, tot);
+      }
+    pool = (pool + 63) / 64 * 64;
+    if (pool > (1 << 20)) return fail(SNC_E_UNSUPPORTED, "hip-pull: injection pool exceeds LDS");
+    int lrc = pull_set_lds(B, (int)pool);
+    if (lrc) return lrc;
+    // Sliced columns per light cap, built once per graph and kept while it
+    // lives (plans of different caps may be alive together).
+    PullGraph* pgp = nullptr;
+    for (PullGraph* q : G->pulls)
+      if (q->cap == cfg->heavy_threshold) pgp = q;
+    if (!pgp) {
+      std::unique_ptr<PullGraph> fresh(new PullGraph());
+      const int brc = pull_prepare(g, cfg->heavy_threshold, *fresh);
+      if (brc) return brc;
+      pgp = fresh.release();
+      G->pulls.push_back(pgp);
+    }
+    PullGraph& pg = *pgp;
+    if (pg.hub_host != P->heavy_host) return fail(SNC_E_UNSUPPORTED, "hip-pull hub sets differ");
+    int prc = dalloc(&P->F, (size_t)g.L * n);
+    if (!prc) prc = dalloc(&P->hub_part, (size_t)std::max(1, pg.n_chunks) * kPullMaxSamples);
+    if (prc) return prc;
+    PullArgs& pa = P->pa;
+    pa.n = n;
+    pa.L = g.L;
+    pa.B = B;
+    pa.words = a.words;
+    pa.n_light_blocks = R;
+    pa.lif = a.lif;
+    pa.sample_mask = B >= 32 ? 0xFFFFFFFFu : ((1u << B) - 1u);
+    pa.pool = (int)pool;
+    pa.row_ptr = g.row_ptr;
+    pa.slab = pg.slab;
+    pa.slab_w = pg.slab_w;
+    pa.slice_ptr = pg.slice_ptr;
+    pa.slice_len = pg.slice_len;
+    pa.is_hub = pg.is_hub;
+    pa.hub_ids = pg.hub_ids;
+    pa.hub_e = pg.hub_e;
+    pa.hub_w = pg.hub_w;
+    pa.chunk_hub = pg.chunk_hub;
+    pa.chunk_beg = pg.chunk_beg;
+    pa.hub_chunk_ptr = pg.hub_chunk_ptr;
+    pa.n_hub = pg.n_hub;
+    pa.n_chunks = pg.n_chunks;
+    pa.inj_blk_ptr = P->inj_blk_ptr;
+    pa.blk_nid = P->blk_nid;
+    pa.blk_amp = P->blk_amp;
+    pa.inj_hub = P->inj_heavy;
+    pa.inj_stride = a.inj_stride;
+    pa.hinj_stride = a.hinj_stride;
+    pa.F = P->F;
+    pa.hub_part = P->hub_part;
+    pa.v = P->v;
+    pa.refr = P->refr;
+    pa.nspk = P->nspk;
+    pa.blk_spikes = P->blk_spikes;
+    pa.blk_events = P->blk_events;
+    pa.blk_stride = (int64_t)P->n_blk_slots;
+    pa.raster = nullptr;
+    pa.raster_stride = a.raster_stride;
+  }
   if (P->inj_tab) {
     rc = fwd_launch_inj_table(P->core.stream, a);
     if (!rc) {
@@ -609,10 +699,11 @@ int snc_hip_plan_create_batch(snc_hip_graph* G, const snc_event* events, int64_t
                                    variant == SNC_VARIANT_AUTO_EXACT, &c);
     if (rc) return rc;
   } else if (variant == SNC_VARIANT_ATOMIC || variant == SNC_VARIANT_BUCKET ||
-             (variant == SNC_VARIANT_SORT && num_samples == 1)) {
-    c = default_plan_config(variant);
+             variant == SNC_VARIANT_PULL || (variant == SNC_VARIANT_SORT && num_samples == 1)) {
+    c = default_plan_config(G->g, variant);
   } else {
-    return fail(SNC_E_INVALID, "unknown variant (batched plans: hip-atomic, hip-bucket, auto)");
+    return fail(SNC_E_INVALID,
+                "unknown variant (batched plans: hip-atomic, hip-bucket, hip-pull, auto)");
   }
   return plan_build(G, events, n_events, num_samples, num_steps, lif, &c, out);
 }
@@ -649,6 +740,7 @@ int reset_state(snc_hip_plan* P, bool raster) {
   fill(P->nspk, sizeof(int32_t) * B * n);
   fill(P->dirty, P->dirty_total);
   fill(P->bits, sizeof(uint32_t) * P->bits_total);
+  if (P->F) fill(P->F, sizeof(uint32_t) * (size_t)P->G->g.L * P->G->g.n);
   fill(P->blk_spikes, sizeof(unsigned long long) * B * P->n_blk_slots);
   fill(P->blk_events, sizeof(unsigned long long) * B * P->n_blk_slots);
   if (raster) fill(P->raster, sizeof(uint32_t) * B * (size_t)P->T * P->a.words);
@@ -666,8 +758,16 @@ int ensure_raster(snc_hip_plan* P) {
 // hip-sort kernel chain of each step, all on the plan's stream.
 int launch_chain(snc_hip_plan* P, const StepArgs& a) {
   for (int t = 0; t < P->T; ++t) {
-    const int rc = P->variant == SNC_VARIANT_SORT ? sort_launch_step(P->core, a, t)
-                                                   : fused_launch_step(P->core, a, t);
+    int rc;
+    if (P->variant == SNC_VARIANT_SORT) {
+      rc = sort_launch_step(P->core, a, t);
+    } else if (P->variant == SNC_VARIANT_PULL) {
+      PullArgs pa = P->pa;
+      pa.raster = a.raster;
+      rc = pull_launch_step(pa, t, P->core.stream);
+    } else {
+      rc = fused_launch_step(P->core, a, t);
+    }
     if (rc) return rc;
   }
   return SNC_OK;
@@ -854,7 +954,13 @@ int snc_hip_plan_profile(snc_hip_plan* P, double* kernel_ms, int64_t* launches) 
         if (!rc) span(SNC_KERNEL_SORT_STEP, ev[0], ev[3]);
         continue;
       }
-      rc = fused_launch_step(P->core, a, t);
+      if (P->variant == SNC_VARIANT_PULL) {
+        PullArgs pa = P->pa;
+        pa.raster = a.raster;
+        rc = pull_launch_step(pa, t, st);
+      } else {
+        rc = fused_launch_step(P->core, a, t);
+      }
       (void)hipEventRecord(ev[1], st);
       if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(SNC_E_HIP, "run failed");
       if (rc) break;
@@ -896,7 +1002,7 @@ static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_
   const auto t0 = std::chrono::steady_clock::now();
   const std::vector<snc_plan_config> cands =
       autotune_candidates(G->g, exact_only, num_samples == 1 && persist_supported(),
-                          num_samples == 1);
+                          num_samples == 1, num_samples);
   snc_tune_report rep{};
   rep.trial_steps = autotune_trial_steps(num_steps);
   int best = -1;

@@ -27,9 +27,10 @@ struct DevGraph {
   uint8_t* delay = nullptr;    // [S]
 
   // Incoming view for the ordered fold. Column j holds j's incoming synapses
-  // in (pre ascending, CSR position ascending) order -- the order in which the
-  // CPU path adds them -- padded to a multiple of 32 so every 32-bit word of
-  // the presence bitmap belongs to exactly one target.
+  // in (delay descending, pre ascending, CSR position ascending) order -- the
+  // order in which the CPU path adds the deposits that arrive together --
+  // padded to a multiple of 32 so every 32-bit word of the presence bitmap
+  // belongs to exactly one target.
   int32_t* cpos = nullptr;   // [S] padded column position of synapse e
   int32_t* ccol = nullptr;   // [n+1] padded column starts (multiples of 32)
   float* cval = nullptr;     // [P] wsig in column order (0 in padding)
@@ -54,8 +55,17 @@ struct DevGraph {
 
 }  // namespace snc_amd
 
+namespace snc_amd {
+struct PullGraph;
+void pull_free(PullGraph* pg);
+}  // namespace snc_amd
+
 struct snc_hip_graph {
   snc_amd::DevGraph g;
+  std::vector<snc_amd::PullGraph*> pulls;  // hip-pull's sliced columns, one per light cap
+  ~snc_hip_graph() {
+    for (snc_amd::PullGraph* p : pulls) snc_amd::pull_free(p);
+  }
   // Autotuner results: [exact_only + 2 * batched].
   bool tuned[4] = {false, false, false, false};
   snc_tune_report tune[4] = {};

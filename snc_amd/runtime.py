@@ -34,6 +34,7 @@ class Backend(IntEnum):
     HipSort = L.VARIANT_SORT
     HipAuto = L.VARIANT_AUTO
     HipAutoExact = L.VARIANT_AUTO_EXACT
+    HipPull = L.VARIANT_PULL
 
 
 _NAMES = {
@@ -41,7 +42,7 @@ _NAMES = {
     Backend.CudaBucket: "cuda-bucket", Backend.CudaSort: "cuda-sort",
     Backend.HipAtomic: "hip-atomic", Backend.HipBucket: "hip-bucket",
     Backend.HipSort: "hip-sort", Backend.HipAuto: "hip-auto",
-    Backend.HipAutoExact: "hip-auto-exact",
+    Backend.HipAutoExact: "hip-auto-exact", Backend.HipPull: "hip-pull",
 }
 
 
@@ -58,7 +59,7 @@ def backend_name(b: Backend) -> str:
 
 def backend_is_hip(b: Backend) -> bool:
     return Backend(b) in (Backend.HipAtomic, Backend.HipBucket, Backend.HipSort, Backend.HipAuto,
-                          Backend.HipAutoExact)
+                          Backend.HipAutoExact, Backend.HipPull)
 
 
 @dataclass

@@ -1,0 +1,133 @@
+"""hip-pull (batched gather over sliced incoming columns) through the C ABI.
+
+Light targets (in-degree up to the plan's cap) sum their inputs in column
+order, so without hubs every sample's raster, counts, logits and membranes
+equal the CPU oracle's bit for bit (and hip-bucket's). Hub targets add
+per-chunk partial sums: counts identical, voltages within 1e-5 relative, and
+the same bits on every run.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from snc_amd import _lib as L
+from snc_amd import configs as K
+from snc_amd import graph as G
+from snc_amd import inputs as I
+from snc_amd.runtime import Backend, ForwardPlan, HipGraph, PlanConfig
+
+pytestmark = pytest.mark.gpu
+
+V_RTOL, V_ATOL = 1e-5, 1e-6
+
+
+def _drives(n, T, B, seed, p0=0.01, dp=0.003):
+    evs = [I.background_drive(n, T, p0 + dp * (b % 5), 1.0, seed=seed + b) for b in range(B)]
+    allev = np.concatenate([e.copy() for e in evs])
+    off = 0
+    for b, e in enumerate(evs):
+        allev["sample_id"][off:off + e.size] = b
+        off += e.size
+    return evs, allev
+
+
+def _pull(hg, allev, T, lif, B, cap):
+    print(f"hip-pull plan: B={B} T={T} cap={cap}", flush=True)
+    return ForwardPlan(hg, allev, T, lif, config=PlanConfig(Backend.HipPull, 256, cap, False),
+                       num_samples=B)
+
+
+@pytest.mark.parametrize("B", [1, 7, 32])
+def test_pull_without_hubs_is_bit_exact(B):
+    g = G.sheet_graph([400, 300, 20], 24, recurrent_fan_in=16, radius=3.0, sheet=20.0,
+                      w_std=0.35, max_delay=9, seed=3)
+    cap = int(g.in_degree().max())
+    lif = K.lif_tau20()
+    T = 60
+    evs, allev = _drives(g.num_neurons, T, B, seed=40, p0=0.05, dp=0.01)
+    hg = HipGraph(g)
+    r = _pull(hg, allev, T, lif, B, cap).run(capture_raster=True, capture_v=True)
+    assert r.used_backend == Backend.HipPull
+    for b in range(B):
+        ref = O.forward(g, evs[b], T, lif)
+        assert ref["spikes"] > 0
+        assert np.array_equal(r.raster[b], ref["raster"]), b
+        assert np.array_equal(r.neuron_spikes[b], ref["neuron_spikes"])
+        assert np.array_equal(r.logits[b], ref["logits"])
+        assert np.array_equal(r.final_v[b].view(np.uint32), ref["final_v"].view(np.uint32))
+        assert np.array_equal(r.step_spikes[b], ref["step_spikes"])
+
+
+def test_pull_with_hubs_within_tolerance_and_deterministic():
+    g = G.hub_sheet_graph(60, 50, 40, hub_fraction=0.02, hub_min=40, hub_max=5000,
+                          w_std=0.06, max_delay=20, seed=9)
+    lif = K.lif_tau20()
+    T, B = 90, 12
+    evs, allev = _drives(g.num_neurons, T, B, seed=20)
+    hg = HipGraph(g)
+    cap = 64
+    assert int((g.in_degree() > cap).sum()) > 20  # hubs in play, some of several chunks
+    plan = _pull(hg, allev, T, lif, B, cap)
+    r = plan.run(capture_raster=True, capture_v=True)
+    sp, ev = plan.sample_stats()
+    assert sp.sum() == r.stats.spikes and ev.sum() == r.stats.synaptic_events
+    for b in range(B):
+        ref = O.forward(g, evs[b], T, lif)
+        assert sp[b] == ref["spikes"] and ev[b] == ref["events"]
+        assert np.array_equal(r.neuron_spikes[b], ref["neuron_spikes"])
+        np.testing.assert_allclose(r.final_v[b], ref["final_v"], rtol=V_RTOL, atol=V_ATOL)
+    r2 = plan.run(capture_raster=True, capture_v=True)
+    assert np.array_equal(r2.raster, r.raster)
+    assert np.array_equal(r2.final_v.view(np.uint32), r.final_v.view(np.uint32))
+
+
+def test_pull_rejects_what_it_cannot_hold():
+    g = G.sheet_graph([64, 48, 8], 12, seed=2)
+    hg = HipGraph(g)
+    lif = K.lif_tau20()
+    _, allev = _drives(g.num_neurons, 10, 33, seed=1)
+    with pytest.raises(L.HipError, match="at most 32"):
+        _pull(hg, allev, 10, lif, 33, 64)
+    with pytest.raises(L.HipError, match="256-neuron"):
+        ForwardPlan(hg, allev[allev["sample_id"] < 2], 10, lif,
+                    config=PlanConfig(Backend.HipPull, 512, 64, False), num_samples=2)
+
+
+def test_pull_dense_input_fills_the_largest_pool():
+    """Every input neuron driven on every step in all 32 samples: 8192 staged
+    injections per (step, block), the largest pool, still bit-exact."""
+    g = G.sheet_graph([256, 64, 8], 12, seed=6, w_std=0.2)
+    lif = K.lif_tau20()
+    T, B = 12, 32
+    evs = []
+    for b in range(B):
+        e = np.zeros(256 * T, dtype=I.EVENT_DTYPE)
+        e["channel_id"] = np.tile(np.arange(256), T)
+        e["time"] = np.repeat(np.arange(T), 256)
+        e["amplitude"] = 0.05 + 0.01 * b
+        evs.append(e)
+    allev = np.concatenate([e.copy() for e in evs])
+    allev["sample_id"] = np.repeat(np.arange(B), 256 * T)
+    hg = HipGraph(g)
+    r = _pull(hg, allev, T, lif, B, int(g.in_degree().max())).run(capture_raster=True,
+                                                                   capture_v=True)
+    for b in range(B):
+        ref = O.forward(g, evs[b], T, lif)
+        assert np.array_equal(r.raster[b], ref["raster"])
+        assert np.array_equal(r.final_v[b].view(np.uint32), ref["final_v"].view(np.uint32))
+
+
+def test_batched_autotuner_considers_pull():
+    g = G.hub_sheet_graph(60, 50, 40, hub_fraction=0.02, hub_min=40, hub_max=1500,
+                          w_std=0.06, max_delay=20, seed=9)
+    lif = K.lif_tau20()
+    T, B = 60, 16
+    evs, allev = _drives(g.num_neurons, T, B, seed=5)
+    hg = HipGraph(g)
+    plan = ForwardPlan(hg, allev, T, lif, Backend.HipAuto, num_samples=B)
+    rep = hg.tune_report(batched=True)
+    assert any(c.variant == Backend.HipPull for c, _ in rep["candidates"])
+    r = plan.run(capture_v=True)
+    for b in range(B):
+        ref = O.forward(g, evs[b], T, lif)
+        assert np.array_equal(r.neuron_spikes[b], ref["neuron_spikes"])
