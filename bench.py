@@ -503,7 +503,11 @@ def main():
                                "persistent": plan.config.persistent},
                       "spikes_per_step": spikes, "events_per_step": events,
                       "parallelism": "single-gpu"},
-           "roofline": roofline, "variants": variants, "autotune": autotune}
+           "roofline": roofline, "variants": variants, "autotune": autotune,
+           "memory": {"graph_bytes": hg.memory(), "plan_bytes": plan.memory(),
+                      "graph_bytes_per_synapse": hg.memory() / max(1, g.num_synapses),
+                      "plan_bytes_per_neuron": plan.memory() / max(1, g.num_neurons),
+                      "overflow": ref.stats.overflow}}
     out["batched_forward"] = {}
     for bs in [int(x) for x in args.batch_samples.split(",") if x]:
         for vn in ("hip-auto", "hip-auto-exact", "hip-pull"):

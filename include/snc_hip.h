@@ -230,6 +230,13 @@ int snc_hip_plan_profile(snc_hip_plan* p, double* kernel_ms_total, int64_t* laun
 /* Wait until all work the library queued on the device has finished. */
 int snc_hip_synchronize(void);
 
+/* Device memory in bytes held by a graph (its arrays, plus hip-pull's sliced
+ * columns once a hip-pull plan built them) and by a plan (state, schedule,
+ * scratch, the raster once captured). No counterpart in the reference: the
+ * memory fields its observability plan asks for (docs/new-plan.md:727-741). */
+int snc_hip_graph_memory(const snc_hip_graph* g, int64_t* bytes);
+int snc_hip_plan_memory(const snc_hip_plan* p, int64_t* bytes);
+
 /* Device ceiling for scattered delivery (no counterpart in the reference: a
  * measurement the benchmark's roofline uses). Times `reps` graph-launched
  * launches, each doing `ops` accesses of one kind at hashed addresses over a

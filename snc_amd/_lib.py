@@ -217,6 +217,8 @@ _SIGS = {
                              C.c_float, C.c_float, C.c_uint64, C.c_int32, C.POINTER(_P)]),
     "snc_make_synthetic": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
                                   C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
+    "snc_hip_graph_memory": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "snc_hip_plan_memory": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "snc_hip_measure_scatter": (C.c_int, [C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
                                           C.POINTER(C.c_double)]),
     # SNCG files (snc_gen.h)

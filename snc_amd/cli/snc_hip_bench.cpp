@@ -101,6 +101,8 @@ struct Eval {
   double ms = 0.0;
   int correct = 0, total = 0;
   int used = -1;
+  int64_t overflow = 0;
+  int64_t graph_bytes = 0, plan_bytes = 0;  // device memory (largest plan)
   std::vector<int64_t> per_sample_spikes;
 };
 
@@ -150,6 +152,8 @@ Eval evaluate(const Options& o, const Graph& g, DeviceGraph& dg, const std::vect
     r.events = st.synaptic_events;
     r.ms = st.ms;
     r.used = st.used_variant;
+    r.overflow = st.overflow;
+    check(snc_hip_plan_memory(p, &r.plan_bytes), "plan memory");
     snc_hip_plan_free(p);
   } else {
     logits.assign((size_t)o.num_samples * std::max(C, 1), 0.0f);
@@ -166,10 +170,15 @@ Eval evaluate(const Options& o, const Graph& g, DeviceGraph& dg, const std::vect
       r.events += st.synaptic_events;
       r.ms += st.ms;
       r.used = st.used_variant;
+      r.overflow += st.overflow;
       r.per_sample_spikes.push_back(st.spikes);
+      int64_t pb = 0;
+      check(snc_hip_plan_memory(p, &pb), "plan memory");
+      r.plan_bytes = std::max(r.plan_bytes, pb);
       snc_hip_plan_free(p);
     }
   }
+  check(snc_hip_graph_memory(dg.h, &r.graph_bytes), "graph memory");
   for (int i = 0; i < o.num_samples; ++i) {
     const float* l = logits.data() + (size_t)i * std::max(C, 1);
     int pred = 0;
@@ -243,6 +252,11 @@ void report(const Options& o, const snc_graph_view& g, const Eval& r) {
               secs > 0 ? r.events / secs : 0.0);
   std::printf("ml:    argmax-acc=%.3f (%d/%d, untrained forward pass)\n",
               r.total ? (double)r.correct / r.total : 0.0, r.correct, r.total);
+  std::printf("mem:   graph=%.2f MB (%.1f B/synapse)  plan=%.2f MB (%.1f B/neuron)  "
+              "overflow=%lld\n",
+              r.graph_bytes / 1e6, g.num_synapses ? (double)r.graph_bytes / g.num_synapses : 0.0,
+              r.plan_bytes / 1e6, g.num_neurons ? (double)r.plan_bytes / g.num_neurons : 0.0,
+              (long long)r.overflow);
 }
 
 void write_json(const Options& o, const snc_graph_view& g, const Eval& r) {
@@ -257,6 +271,13 @@ void write_json(const Options& o, const snc_graph_view& g, const Eval& r) {
     << g.num_neurons << ",\n  \"synapses\": " << g.num_synapses << ",\n  \"spikes\": "
     << r.spikes << ",\n  \"synaptic_events\": " << r.events << ",\n  \"sim_ms\": " << r.ms
     << ",\n  \"argmax_acc\": " << (r.total ? (double)r.correct / r.total : 0.0)
+    << ",\n  \"spikes_per_s\": " << (r.ms > 0 ? r.spikes / (r.ms / 1e3) : 0.0)
+    << ",\n  \"events_per_s\": " << (r.ms > 0 ? r.events / (r.ms / 1e3) : 0.0)
+    << ",\n  \"overflow\": " << r.overflow << ",\n  \"device_graph_bytes\": " << r.graph_bytes
+    << ",\n  \"device_plan_bytes\": " << r.plan_bytes << ",\n  \"bytes_per_synapse\": "
+    << (g.num_synapses ? (double)r.graph_bytes / g.num_synapses : 0.0)
+    << ",\n  \"bytes_per_neuron\": "
+    << (g.num_neurons ? (double)r.plan_bytes / g.num_neurons : 0.0)
     << ",\n  \"per_sample_spikes\": [";
   for (size_t i = 0; i < r.per_sample_spikes.size(); ++i)
     f << (i ? ", " : "") << r.per_sample_spikes[i];

@@ -37,8 +37,10 @@ int dalloc(T** p, size_t count) {
   if (count == 0) count = 1;
   if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
     *p = nullptr;
+    hip_clear();
     return fail(SNC_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
   }
+  account_alloc(count * sizeof(T));
   return SNC_OK;
 }
 
@@ -135,6 +137,7 @@ struct snc_hip_plan {
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // [raster off, on]
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
+  int64_t dev_bytes = 0;  // device memory this plan allocated
   double kernel_avg_ms = 0.0;
   int64_t kernel_launches = 0;
 
@@ -213,6 +216,7 @@ int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
   const int64_t S = gv->num_synapses;
   std::unique_ptr<snc_hip_graph> H(new snc_hip_graph());
   DevGraph& g = H->g;
+  AllocAccount acct(&g.dev_bytes);
   g.n = n;
   g.S = S;
   int md = 1;
@@ -370,6 +374,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     return fail(SNC_E_UNSUPPORTED, "batched plans run hip-atomic, hip-bucket or hip-pull");
   const DevGraph& g = G->g;
   std::unique_ptr<snc_hip_plan> P(new snc_hip_plan());
+  AllocAccount acct(&P->dev_bytes);
   P->G = G;
   P->T = num_steps;
   P->B = B;
@@ -603,6 +608,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
       if (q->cap == cfg->heavy_threshold) pgp = q;
     if (!pgp) {
       std::unique_ptr<PullGraph> fresh(new PullGraph());
+      AllocAccount gacct(&fresh->dev_bytes);
       const int brc = pull_prepare(g, cfg->heavy_threshold, *fresh);
       if (brc) return brc;
       pgp = fresh.release();
@@ -751,6 +757,7 @@ int reset_state(snc_hip_plan* P, bool raster) {
 
 int ensure_raster(snc_hip_plan* P) {
   if (P->raster) return SNC_OK;
+  AllocAccount acct(&P->dev_bytes);
   return dalloc(&P->raster, (size_t)P->B * std::max(1, P->T) * P->a.words);
 }
 
@@ -892,6 +899,20 @@ int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stat
 }
 
 int32_t snc_hip_plan_num_samples(const snc_hip_plan* p) { return p ? p->B : -1; }
+
+int snc_hip_graph_memory(const snc_hip_graph* g, int64_t* bytes) {
+  if (!g || !bytes) return fail(SNC_E_INVALID, "null argument");
+  int64_t b = g->g.dev_bytes;
+  for (const PullGraph* q : g->pulls) b += q->dev_bytes;
+  *bytes = b;
+  return SNC_OK;
+}
+
+int snc_hip_plan_memory(const snc_hip_plan* p, int64_t* bytes) {
+  if (!p || !bytes) return fail(SNC_E_INVALID, "null argument");
+  *bytes = p->dev_bytes;
+  return SNC_OK;
+}
 
 int snc_hip_plan_sample_stats(const snc_hip_plan* p, int64_t* spikes, int64_t* events) {
   if (!p || !spikes || !events) return fail(SNC_E_INVALID, "null argument");

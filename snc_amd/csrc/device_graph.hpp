@@ -49,6 +49,7 @@ struct DevGraph {
   std::vector<std::vector<int32_t>> chan_to_inputs;  // INPUT neurons per channel, ascending
   std::vector<int32_t> heavy_host;                   // heavy target ids, ascending
   snc_graph_profile profile{};
+  int64_t dev_bytes = 0;  // device memory of the arrays above
 
   ~DevGraph();
 };

@@ -17,6 +17,18 @@ inline int hip_last(const char* what) {
   return fail(SNC_E_HIP, std::string(what) + ": " + hipGetErrorName(e));
 }
 
+// Device bytes allocated while an AllocAccount is alive on this thread are
+// added to its counter (graph upload, plan build, lazily built buffers).
+inline thread_local int64_t* g_alloc_account = nullptr;
+struct AllocAccount {
+  int64_t* prev;
+  explicit AllocAccount(int64_t* counter) : prev(g_alloc_account) { g_alloc_account = counter; }
+  ~AllocAccount() { g_alloc_account = prev; }
+};
+inline void account_alloc(size_t bytes) {
+  if (g_alloc_account) *g_alloc_account += (int64_t)bytes;
+}
+
 // Drops an error left by an earlier call outside this library (or by a call
 // whose failure was already reported), so the launch checks of the entry point
 // that follows report only its own launches.

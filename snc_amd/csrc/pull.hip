@@ -35,6 +35,7 @@ int dalloc(T** p, size_t count) {
     hip_clear();
     return fail(SNC_E_NOMEM, "device allocation failed (pull columns)");
   }
+  account_alloc(count * sizeof(T));
   return SNC_OK;
 }
 

@@ -51,6 +51,7 @@ struct PullGraph {
   uint8_t* is_hub = nullptr;     // [n]
   std::vector<int32_t> hub_host;
   int64_t entries = 0;
+  int64_t dev_bytes = 0;  // device memory of these arrays
   void release();
   ~PullGraph() { release(); }
 };

@@ -152,6 +152,12 @@ class HipGraph:
         L.check(L.lib().snc_hip_graph_profile_get(self._h, C.byref(p)))
         return {k: getattr(p, k) for k, _ in p._fields_}
 
+    def memory(self) -> int:
+        """Device bytes this graph holds (snc_hip_graph_memory)."""
+        b = C.c_int64()
+        L.check(L.lib().snc_hip_graph_memory(self._h, C.byref(b)))
+        return int(b.value)
+
     def tune_report(self, exact_only=False, batched=False) -> dict:
         """The autotuner's search for this graph (after a hip-auto plan):
         candidates with their measured ms per step, and the chosen one."""
@@ -252,6 +258,12 @@ class ForwardPlan:
         L.check(L.lib().snc_hip_plan_sample_stats(self._h, L.ptr(sp, C.c_int64),
                                                   L.ptr(ev, C.c_int64)))
         return sp, ev
+
+    def memory(self) -> int:
+        """Device bytes this plan holds (snc_hip_plan_memory)."""
+        b = C.c_int64()
+        L.check(L.lib().snc_hip_plan_memory(self._h, C.byref(b)))
+        return int(b.value)
 
     def launch(self):
         """Enqueue one run without synchronising (throughput timing)."""

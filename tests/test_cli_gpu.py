@@ -52,6 +52,9 @@ def test_bench_matches_oracle_per_sample(tmp_path, backend, batched):
                       lif)["spikes"] for i in range(n)]
     assert res["per_sample_spikes"] == want
     assert res["synapses"] == g.num_synapses
+    # observability: device memory, rates, overflow
+    assert res["device_graph_bytes"] >= 9 * g.num_synapses and res["device_plan_bytes"] > 0
+    assert res["overflow"] == 0 and res["events_per_s"] > 0
 
 
 def test_bench_compare_backends_all_exact():

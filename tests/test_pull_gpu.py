@@ -46,7 +46,12 @@ def test_pull_without_hubs_is_bit_exact(B):
     T = 60
     evs, allev = _drives(g.num_neurons, T, B, seed=40, p0=0.05, dp=0.01)
     hg = HipGraph(g)
-    r = _pull(hg, allev, T, lif, B, cap).run(capture_raster=True, capture_v=True)
+    m0 = hg.memory()
+    plan = _pull(hg, allev, T, lif, B, cap)
+    assert hg.memory() > m0  # the sliced columns count toward the graph
+    p0 = plan.memory()
+    r = plan.run(capture_raster=True, capture_v=True)
+    assert plan.memory() >= p0 + B * T * ((g.num_neurons + 31) // 32) * 4  # the raster
     assert r.used_backend == Backend.HipPull
     for b in range(B):
         ref = O.forward(g, evs[b], T, lif)
