@@ -48,6 +48,11 @@ typedef struct snc_struct_config {
    * most-firing one, so the log also holds outcomes of non-greedy choices.
    * Drawn from a generator of its own; 0 leaves the heuristic unchanged. */
   float explore;
+  /* Sheet connectomes only: select the prune set on the GPU, and in learned
+   * mode generate the window candidates, their features and scores there
+   * (candidates never leave the device before scoring). Same result as the
+   * host path, which restates it. */
+  int32_t on_device;
 } snc_struct_config;
 
 /* StructReport (connectome.hpp:50-54) plus the candidates scored. */
@@ -63,6 +68,18 @@ typedef struct snc_connectome snc_connectome;
 
 int snc_conn_layered_local(const int32_t* layers, int32_t n_layers, int32_t synapse_budget,
                            int32_t delay, float w_init, uint64_t seed, snc_connectome** out);
+/* A layered connectome on 2-D sheets with recurrent hidden layers (no
+ * counterpart in the reference, whose Connectome is layered feed-forward with
+ * 1-D positions, connectome.cpp:30-62). Every layer is a side x side grid over
+ * the unit square (side = ceil(sqrt(count))); each neuron of layer l >= 1 draws
+ * fan_in partners from layer l-1 and, in a hidden layer, rec_fan_in from its
+ * own layer (no self-synapse), with probability proportional to
+ * exp(-d / radius), SplitMix64 draws. Grow windows are discs of radius
+ * locality_window around the post over layer l-1 and (hidden posts) layer l.
+ * Pruning removes the K smallest (|w|, CSR position). */
+int snc_conn_sheet_recurrent(const int32_t* layers, int32_t n_layers, int32_t fan_in,
+                             int32_t rec_fan_in, float radius, int32_t delay, float w_init,
+                             uint64_t seed, snc_connectome** out);
 void snc_conn_free(snc_connectome* c);
 int32_t snc_conn_num_synapses(const snc_connectome* c);
 int snc_conn_to_graph(const snc_connectome* c, snc_graph_buf** out);

@@ -109,7 +109,7 @@ RunResult run(const Options& o, const Data& d, uint64_t seed, int grow_mode, snc
   snc_connectome* c = nullptr;
   check(snc_conn_layered_local(layers, 3, o.budget, 1, 0.02f, seed, &c), "connectome");
   const snc_struct_config sc{o.grow_n, 1, o.w_grow, o.locality, o.protect, 8, 0, 0.1f, 0.1f,
-                             seed, explore};
+                             seed, explore, 0};
   const snc_train_config cfg{o.num_steps, {1.0f, 0.95122942f, 0.0f, 1, SNC_RESET_SUBTRACT},
                              o.lr, o.w_init, 4.0f, 0.3f, 1.0f, 1, SNC_READOUT_LEAKY, 1.0f,
                              SNC_ENC_POISSON, 1.0f, 0.2f, 0.0f, seed};

@@ -99,6 +99,34 @@ int grad_device(snc_hip_mlp* m, int64_t off, int64_t M, std::vector<float>& grad
 
 }  // namespace
 
+namespace snc_amd {
+
+int mlp_score_device(snc_hip_mlp* m, const float* X, int64_t M, float* out, double* kernel_ms) {
+  if (M <= 0) return SNC_OK;
+  int dev = 0, cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    cus = prop.multiProcessorCount;
+  const int64_t tiles = (M + 15) / 16;
+  // 4 waves per block, several rounds of prefetched tiles per wave; about
+  // as many blocks as stay resident (three per CU at the kernel's register use).
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 15) / 16, (int64_t)cus * 3));
+  (void)hipEventRecord(m->ev0, m->stream);
+  const int rc = mlp_launch_score(m->dev(), X, M, out, blocks, m->stream);
+  if (rc) return rc;
+  (void)hipEventRecord(m->ev1, m->stream);
+  const hipError_t e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) return hip_fail(e, "scoring");
+  if (kernel_ms) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, m->ev0, m->ev1);
+    *kernel_ms = ms;
+  }
+  return SNC_OK;
+}
+
+}  // namespace snc_amd
+
 extern "C" {
 
 int snc_hip_mlp_create(uint64_t seed, snc_hip_mlp** out) {
@@ -168,27 +196,10 @@ int snc_hip_mlp_score(snc_hip_mlp* m, const float* features, int64_t M, float* s
   if (hipMemcpyAsync(m->X, features, sizeof(float) * M * kMlpIn, hipMemcpyHostToDevice,
                      m->stream) != hipSuccess)
     return fail(SNC_E_HIP, "feature upload failed");
-  int dev = 0, cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-    cus = prop.multiProcessorCount;
-  const int64_t tiles = (M + 15) / 16;
-  // 4 waves per block, several rounds of prefetched tiles per wave; about
-  // as many blocks as stay resident (three per CU at the kernel's register use).
-  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 15) / 16, (int64_t)cus * 3));
-  (void)hipEventRecord(m->ev0, m->stream);
-  rc = mlp_launch_score(m->dev(), m->X, M, m->out, blocks, m->stream);
+  rc = mlp_score_device(m, m->X, M, m->out, kernel_ms);
   if (rc) return rc;
-  (void)hipEventRecord(m->ev1, m->stream);
-  if (hipMemcpyAsync(scores, m->out, sizeof(float) * M, hipMemcpyDeviceToHost, m->stream) !=
-          hipSuccess ||
-      hipStreamSynchronize(m->stream) != hipSuccess)
-    return fail(SNC_E_HIP, "scoring failed");
-  if (kernel_ms) {
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, m->ev0, m->ev1);
-    *kernel_ms = ms;
-  }
+  if (hipMemcpy(scores, m->out, sizeof(float) * M, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(SNC_E_HIP, "score download failed");
   return SNC_OK;
 }
 

@@ -6,7 +6,11 @@
 //   to_graph            src/structure/connectome.cpp:78-102
 //   structural_update   src/structure/connectome.cpp:121-219
 // so its result equals the reference's for the same statistics. The learned
-// mode and the grow log are this project's additions.
+// mode and the grow log are this project's additions, and so is the sheet
+// connectome (snc_conn_sheet_recurrent): layers on 2-D grids, recurrent
+// hidden layers, disc-shaped grow windows over the previous and (hidden) the
+// own layer, and a total prune order -- (|w|, synapse position in CSR order)
+// ascending -- that the device path (conn_gpu.hip) reproduces exactly.
 
 #include <algorithm>
 #include <cmath>
@@ -16,6 +20,8 @@
 #include <vector>
 
 #include "common.hpp"
+#include "conn_gpu.hpp"
+#include "rng.hpp"
 #include "snc_gen.h"
 #include "snc_hip.h"
 #include "snc_struct.h"
@@ -41,6 +47,10 @@ struct snc_connectome {
   int n = 0, num_layers = 0;
   std::vector<int> layer, layer_begin, layer_count;
   std::vector<float> pos;
+  // Sheet connectome: every layer a side x side grid over the unit square.
+  bool sheet = false;
+  std::vector<int> side;       // per layer
+  std::vector<float> px, py;   // per neuron
   std::vector<uint8_t> role;
   std::vector<int> channel;
   std::vector<std::vector<Edge>> adj;
@@ -68,6 +78,31 @@ struct snc_connectome {
     klo = std::clamp((int)std::ceil((pos[p] - w) * (bn - 1)), 0, bn - 1);
     khi = std::clamp((int)std::floor((pos[p] + w) * (bn - 1)), 0, bn - 1);
     return true;
+  }
+  // 2-D distance of two sheet neurons, in double (the device computes the same).
+  double dist2d(int q, int p) const {
+    const double dx = (double)px[q] - (double)px[p], dy = (double)py[q] - (double)py[p];
+    return std::sqrt(dx * dx + dy * dy);
+  }
+  // Candidate pres of post p in order: the layered window range, or on a
+  // sheet every neuron of layer l-1 and (hidden p) of layer l, p excluded,
+  // within distance w, by layer then id.
+  void window_list(int p, float w, std::vector<int>& out) const {
+    out.clear();
+    if (!sheet) {
+      int b0, klo, khi;
+      if (window(p, w, b0, klo, khi))
+        for (int k = klo; k <= khi; ++k) out.push_back(b0 + k);
+      return;
+    }
+    const int l = layer[p];
+    if (l < 1) return;
+    const bool hidden = l < num_layers - 1;
+    for (int ll = l - 1; ll <= (hidden ? l : l - 1); ++ll)
+      for (int k = 0; k < layer_count[ll]; ++k) {
+        const int q = layer_begin[ll] + k;
+        if (q != p && dist2d(q, p) <= (double)w) out.push_back(q);
+      }
   }
 };
 
@@ -120,7 +155,7 @@ struct FeatureCtx {
     f[0] = (float)rq;
     f[1] = (float)rp;
     f[2] = (float)(rq * rp * 100.0);
-    f[3] = std::fabs(c->pos[q] - c->pos[p]);
+    f[3] = c->sheet ? (float)c->dist2d(q, p) : std::fabs(c->pos[q] - c->pos[p]);
     f[4] = (float)delay / 32.0f;
     f[5] = (float)(indeg[p] / mean_in);
     f[6] = (float)(outdeg[q] / mean_out);
@@ -137,8 +172,49 @@ struct FeatureCtx {
   }
 };
 
+// The device path's inputs: the sheet layout and the per-neuron columns of
+// the features, computed here as FeatureCtx computes them.
+snc_amd::ConnSheetView sheet_view(const snc_connectome* c, const FeatureCtx& fx) {
+  snc_amd::ConnSheetView v;
+  const int n = c->n;
+  v.n = n;
+  v.num_layers = c->num_layers;
+  v.layer.assign(c->layer.begin(), c->layer.end());
+  v.layer_begin.assign(c->layer_begin.begin(), c->layer_begin.end());
+  v.layer_count.assign(c->layer_count.begin(), c->layer_count.end());
+  v.side.assign(c->side.begin(), c->side.end());
+  v.px = c->px;
+  v.py = c->py;
+  v.fire.resize(n);
+  v.rate.resize(n);
+  v.f5.resize(n);
+  v.f6.resize(n);
+  v.f7.resize(n);
+  v.f8.resize(n);
+  v.f9.resize(n);
+  v.f10.resize(n);
+  v.f14.resize(n);
+  for (int i = 0; i < n; ++i) {
+    v.fire[i] = fx.fire(i);
+    v.rate[i] = fx.fire(i) / fx.norm;
+    v.f5[i] = (float)(fx.indeg[i] / fx.mean_in);
+    v.f6[i] = (float)(fx.outdeg[i] / fx.mean_out);
+    v.f7[i] = (float)fx.in_w[i];
+    v.f8[i] = (float)fx.out_w[i];
+    v.f9[i] = c->num_layers > 1 ? (float)c->layer[i] / (c->num_layers - 1) : 0.0f;
+    v.f10[i] = (float)std::log1p(fx.fire(i));
+    const double lm = fx.layer_mean_fire[c->layer[i]];
+    v.f14[i] = lm > 0 ? (float)(fx.fire(i) / lm) : 0.0f;
+  }
+  v.f4 = (float)fx.delay / 32.0f;
+  return v;
+}
+
 }  // namespace
 
+using snc_amd::conn_gpu_candidates;
+using snc_amd::conn_gpu_prune_select;
+using snc_amd::ConnSheetView;
 using snc_amd::fail;
 
 extern "C" {
@@ -189,6 +265,81 @@ int snc_conn_layered_local(const int32_t* layers, int32_t n_layers, int32_t budg
       const int lo = std::clamp(centre - k / 2, 0, std::max(0, to_n - k));
       for (int j = 0; j < k; ++j) c->adj[from0 + a].push_back({to0 + lo + j, delay, w_init, kAgeInit, -1});
     }
+  }
+  *out = c.release();
+  return SNC_OK;
+}
+
+int snc_conn_sheet_recurrent(const int32_t* layers, int32_t n_layers, int32_t fan_in,
+                             int32_t rec_fan_in, float radius, int32_t delay, float w_init,
+                             uint64_t seed, snc_connectome** out) {
+  if (!layers || n_layers < 2 || !out || fan_in < 0 || rec_fan_in < 0 || !(radius > 0.0f) ||
+      delay < 1)
+    return fail(SNC_E_INVALID, "bad sheet connectome arguments");
+  for (int l = 0; l < n_layers; ++l)
+    if (layers[l] < 1) return fail(SNC_E_INVALID, "empty layer");
+  std::unique_ptr<snc_connectome> c(new snc_connectome());
+  c->sheet = true;
+  c->num_layers = n_layers;
+  for (int l = 0; l < n_layers; ++l) c->n += layers[l];
+  const int n = c->n;
+  c->layer.resize(n);
+  c->pos.resize(n);
+  c->px.resize(n);
+  c->py.resize(n);
+  c->role.assign(n, SNC_ROLE_INTERNAL);
+  c->channel.assign(n, -1);
+  c->adj.assign(n, {});
+  c->layer_begin.resize(n_layers);
+  c->layer_count.resize(n_layers);
+  c->side.resize(n_layers);
+  c->rng.seed(seed);
+  c->explore_rng.seed(seed ^ 0xA5A5A5A5DEADBEEFULL);
+  int idx = 0;
+  for (int l = 0; l < n_layers; ++l) {
+    const int cnt = layers[l];
+    const int side = (int)std::ceil(std::sqrt((double)cnt));
+    c->layer_begin[l] = idx;
+    c->layer_count[l] = cnt;
+    c->side[l] = side;
+    for (int k = 0; k < cnt; ++k, ++idx) {
+      c->layer[idx] = l;
+      c->pos[idx] = layer_pos(k, cnt);
+      c->px[idx] = side > 1 ? (float)(k % side) / (float)(side - 1) : 0.5f;
+      c->py[idx] = side > 1 ? (float)(k / side) / (float)(side - 1) : 0.5f;
+      if (l == 0) {
+        c->role[idx] = SNC_ROLE_INPUT;
+        c->channel[idx] = k;
+      } else if (l == n_layers - 1) {
+        c->role[idx] = SNC_ROLE_OUTPUT;
+        c->channel[idx] = k;
+      }
+    }
+  }
+  // Partners of each post: the k smallest exponential race times
+  // -log(u) * exp(d / radius) among the candidates (a draw without
+  // replacement with probability proportional to exp(-d / radius)).
+  snc_amd::SplitMix64 rng(seed ^ 0x5EE7C0DEULL);
+  std::vector<std::pair<double, int>> race;
+  std::vector<std::vector<int>> pres(n);
+  for (int p = c->layer_begin[1]; p < n; ++p) {
+    const int l = c->layer[p];
+    for (int pass = 0; pass < 2; ++pass) {
+      const int ll = pass == 0 ? l - 1 : l;
+      const int k = pass == 0 ? fan_in : (l < n_layers - 1 ? rec_fan_in : 0);
+      if (k <= 0) continue;
+      race.clear();
+      for (int kk = 0; kk < c->layer_count[ll]; ++kk) {
+        const int q = c->layer_begin[ll] + kk;
+        if (q == p) continue;
+        race.push_back({-std::log(rng.uniform_pos()) * std::exp(c->dist2d(q, p) / radius), q});
+      }
+      const int take = std::min(k, (int)race.size());
+      std::partial_sort(race.begin(), race.begin() + take, race.end());
+      for (int t = 0; t < take; ++t) pres[p].push_back(race[t].second);
+    }
+    std::sort(pres[p].begin(), pres[p].end());
+    for (int q : pres[p]) c->adj[q].push_back({p, delay, w_init, kAgeInit, -1});
   }
   *out = c.release();
   return SNC_OK;
@@ -270,12 +421,37 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
     for (int k = 0; k < (int)c->adj[i].size(); ++k)
       if (c->adj[i][k].age >= cfg->protect_rounds) cand.push_back({std::fabs(c->adj[i][k].weight), i, k});
   const int K = std::min(grow_target, (int)cand.size());
-  if (K > 0)
-    std::nth_element(cand.begin(), cand.begin() + K, cand.end(),
-                     [](const Cand& a, const Cand& b) { return a.score < b.score; });
   std::vector<std::vector<char>> rm(c->n);
   for (int i = 0; i < c->n; ++i) rm[i].assign(c->adj[i].size(), 0);
-  for (int j = 0; j < K; ++j) rm[cand[j].i][cand[j].k] = 1;
+  if (c->sheet && cfg->on_device) {
+    // The K smallest (|w|, CSR position) on the device.
+    std::vector<float> w;
+    std::vector<int32_t> age;
+    for (const auto& a : c->adj)
+      for (const Edge& e : a) {
+        w.push_back(e.weight);
+        age.push_back(e.age);
+      }
+    std::vector<uint8_t> mask(w.size(), 0);
+    const int rc = conn_gpu_prune_select(w.data(), age.data(), (int64_t)w.size(),
+                                         cfg->protect_rounds, K, mask.data());
+    if (rc) return rc;
+    int64_t q = 0;
+    for (int i = 0; i < c->n; ++i)
+      for (size_t k = 0; k < c->adj[i].size(); ++k) rm[i][k] = (char)mask[q++];
+  } else if (c->sheet) {
+    // Total order (|w|, CSR position): cand is built in CSR order.
+    if (K > 0)
+      std::nth_element(cand.begin(), cand.begin() + K, cand.end(), [&](const Cand& a, const Cand& b) {
+        return a.score != b.score ? a.score < b.score : (a.i != b.i ? a.i < b.i : a.k < b.k);
+      });
+    for (int j = 0; j < K; ++j) rm[cand[j].i][cand[j].k] = 1;
+  } else {
+    if (K > 0)
+      std::nth_element(cand.begin(), cand.begin() + K, cand.end(),
+                       [](const Cand& a, const Cand& b) { return a.score < b.score; });
+    for (int j = 0; j < K; ++j) rm[cand[j].i][cand[j].k] = 1;
+  }
   // Grow-log labels: an edge decides its outcome at its first eligible prune.
   for (const Cand& x : cand) {
     const Edge& e = c->adj[x.i][x.k];
@@ -301,13 +477,12 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
 
   // --- grow ---
   auto fire_of = [&](int id) -> long long { return id < n_neur ? neuron_fires[id] : 0LL; };
+  std::vector<int> wl;
   auto grow_local = [&](int p) -> int {
-    int b0, klo, khi;
-    if (!c->window(p, cfg->locality_window, b0, klo, khi)) return -1;
+    c->window_list(p, cfg->locality_window, wl);
     int best = -1;
     long long best_fire = -1;
-    for (int kk = klo; kk <= khi; ++kk) {
-      const int q = b0 + kk;
+    for (int q : wl) {
       if (c->connected(q, p)) continue;
       if (fire_of(q) > best_fire) {
         best_fire = fire_of(q);
@@ -317,15 +492,15 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
     return best;
   };
   FeatureCtx fx(c, neuron_fires, n_neur, activity_norm, cfg->delay);
+  std::vector<int> wf;
   auto window_features = [&](int p, std::vector<int>& qs, std::vector<float>& feats) {
     qs.clear();
     feats.clear();
-    int b0, klo, khi;
-    if (!c->window(p, cfg->locality_window, b0, klo, khi)) return;
+    c->window_list(p, cfg->locality_window, wf);
+    if (wf.empty()) return;
     double mx = 0.0;
     std::vector<std::pair<double, int>> order;
-    for (int kk = klo; kk <= khi; ++kk) {
-      const int q = b0 + kk;
+    for (int q : wf) {
       mx = std::max(mx, fx.fire(q));
       order.push_back({fx.fire(q), q});
     }
@@ -335,8 +510,7 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
     std::vector<double> rank(c->n, 0.0);
     for (size_t r = 0; r < order.size(); ++r)
       rank[order[r].second] = order.size() > 1 ? (double)r / (order.size() - 1) : 0.0;
-    for (int kk = klo; kk <= khi; ++kk) {
-      const int q = b0 + kk;
+    for (int q : wf) {
       qs.push_back(q);
       feats.resize(feats.size() + kFeat);
       fx.features(q, p, mx, rank[q], feats.data() + feats.size() - kFeat);
@@ -360,17 +534,28 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
     std::vector<int> offs(freed.size() + 1, 0), all_q;
     std::vector<float> all_f, qs_feat;
     std::vector<int> qs;
-    for (size_t u = 0; u < freed.size(); ++u) {
-      window_features(freed[u], qs, qs_feat);
-      all_q.insert(all_q.end(), qs.begin(), qs.end());
-      all_f.insert(all_f.end(), qs_feat.begin(), qs_feat.end());
-      offs[u + 1] = (int)all_q.size();
-    }
-    std::vector<float> score(all_q.size());
+    std::vector<float> score;
     double ms = 0.0;
-    if (!all_q.empty()) {
-      const int rc = snc_hip_mlp_score(model, all_f.data(), (int64_t)all_q.size(), score.data(), &ms);
+    if (c->sheet && cfg->on_device) {
+      // Candidates, features and scores on the device; back come the
+      // candidates' pres, scores and features (the features for the log).
+      const ConnSheetView sv = sheet_view(c, fx);
+      const int rc = conn_gpu_candidates(sv, freed, cfg->locality_window, model, offs, all_q,
+                                         all_f, score, &ms);
       if (rc) return rc;
+    } else {
+      for (size_t u = 0; u < freed.size(); ++u) {
+        window_features(freed[u], qs, qs_feat);
+        all_q.insert(all_q.end(), qs.begin(), qs.end());
+        all_f.insert(all_f.end(), qs_feat.begin(), qs_feat.end());
+        offs[u + 1] = (int)all_q.size();
+      }
+      score.assign(all_q.size(), 0.0f);
+      if (!all_q.empty()) {
+        const int rc =
+            snc_hip_mlp_score(model, all_f.data(), (int64_t)all_q.size(), score.data(), &ms);
+        if (rc) return rc;
+      }
     }
     rep->candidates_scored = (int64_t)all_q.size();
     rep->score_ms = ms;
@@ -401,11 +586,10 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
       if (grown >= pruned) break;
       int q = grow_local(p);
       if (q >= 0 && explore && u01(c->explore_rng) < cfg->explore) {
-        int b0, klo, khi;
         std::vector<int> open;
-        if (c->window(p, cfg->locality_window, b0, klo, khi))
-          for (int kk = klo; kk <= khi; ++kk)
-            if (!c->connected(b0 + kk, p)) open.push_back(b0 + kk);
+        c->window_list(p, cfg->locality_window, wl);
+        for (int cq : wl)
+          if (!c->connected(cq, p)) open.push_back(cq);
         if (!open.empty())
           q = open[std::uniform_int_distribution<size_t>(0, open.size() - 1)(c->explore_rng)];
       }

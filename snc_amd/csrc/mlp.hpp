@@ -38,3 +38,13 @@ int mlp_launch_quantize(const float* w, const MlpDev& p, hipStream_t st);
 uint16_t mlp_bf16(float f);
 
 }  // namespace snc_amd
+
+struct snc_hip_mlp;
+
+namespace snc_amd {
+
+// Scores M candidates whose features are already on the device (X [M][16])
+// into out [M], on the model's stream, and waits for them.
+int mlp_score_device(snc_hip_mlp* m, const float* X, int64_t M, float* out, double* kernel_ms);
+
+}  // namespace snc_amd

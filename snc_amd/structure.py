@@ -20,7 +20,7 @@ class StructConfigC(C.Structure):
                 ("locality_window", C.c_float), ("protect_rounds", C.c_int32),
                 ("max_attempts_mult", C.c_int32), ("reward_modulated", C.c_int32),
                 ("reward_chance", C.c_float), ("reward_floor", C.c_float), ("seed", C.c_uint64),
-                ("explore", C.c_float)]
+                ("explore", C.c_float), ("on_device", C.c_int32)]
 
 
 class StructReportC(C.Structure):
@@ -42,17 +42,22 @@ class StructConfig:
     reward_floor: float = 0.1
     seed: int = 1
     explore: float = 0.0   # logging runs: share of non-greedy (uniform) grow choices
+    on_device: bool = False  # sheet connectomes: prune selection and grow candidates on the GPU
 
     def c(self):
         return StructConfigC(self.grow_per_epoch, self.delay, self.w_grow_init,
                              self.locality_window, self.protect_rounds, self.max_attempts_mult,
                              1 if self.reward_modulated else 0, self.reward_chance,
-                             self.reward_floor, self.seed, self.explore)
+                             self.reward_floor, self.seed, self.explore,
+                             1 if self.on_device else 0)
 
 
 _SIGS = {
     "snc_conn_layered_local": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32,
                                          C.c_float, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "snc_conn_sheet_recurrent": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32,
+                                           C.c_float, C.c_int32, C.c_float, C.c_uint64,
+                                           C.POINTER(C.c_void_p)]),
     "snc_conn_free": (None, [C.c_void_p]),
     "snc_conn_num_synapses": (C.c_int32, [C.c_void_p]),
     "snc_conn_to_graph": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -81,6 +86,22 @@ class Connectome:
         L.check(L.lib().snc_conn_layered_local(L.ptr(lay, C.c_int32), lay.size, synapse_budget,
                                                delay, w_init, seed, C.byref(self._h)))
         self.layers = list(layers)
+
+    @classmethod
+    def sheet_recurrent(cls, layers, fan_in, rec_fan_in=0, radius=0.1, delay=1, w_init=0.02,
+                        seed=1) -> "Connectome":
+        """Layers on 2-D grids with recurrent hidden layers
+        (snc_conn_sheet_recurrent): each neuron of layer l >= 1 draws fan_in
+        partners from layer l-1 and, if hidden, rec_fan_in from its own layer,
+        p ~ exp(-d / radius) in unit-square coordinates."""
+        lay = np.ascontiguousarray(layers, np.int32)
+        obj = cls.__new__(cls)
+        obj._h = C.c_void_p()
+        L.check(L.lib().snc_conn_sheet_recurrent(L.ptr(lay, C.c_int32), lay.size, fan_in,
+                                                 rec_fan_in, radius, delay, w_init, seed,
+                                                 C.byref(obj._h)))
+        obj.layers = list(layers)
+        return obj
 
     @property
     def num_synapses(self) -> int:

@@ -74,3 +74,87 @@ def test_candidate_features_are_finite_and_local():
     assert np.allclose(F[:, 0], fires[pre] / 100.0)
     assert np.allclose(F[:, 1], fires[post] / 100.0)
     assert np.all(F[:, 3] >= 0) and np.all(F[:, 3] <= 1)
+
+
+# ---- sheet connectome (2-D layers, recurrent hidden layers) ----
+
+def _sheet_xy(layers):
+    """Unit-square grid positions of every neuron, as the sheet connectome lays them out."""
+    xs, ys, lay = [], [], []
+    for l, cnt in enumerate(layers):
+        side = int(np.ceil(np.sqrt(cnt)))
+        k = np.arange(cnt)
+        if side > 1:
+            xs.append((k % side).astype(np.float32) / np.float32(side - 1))
+            ys.append((k // side).astype(np.float32) / np.float32(side - 1))
+        else:
+            xs.append(np.full(cnt, 0.5, np.float32))
+            ys.append(np.full(cnt, 0.5, np.float32))
+        lay.append(np.full(cnt, l))
+    return np.concatenate(xs), np.concatenate(ys), np.concatenate(lay)
+
+
+def _dist(x, y, q, p):
+    return np.sqrt((x[q].astype(np.float64) - x[p]) ** 2 + (y[q].astype(np.float64) - y[p]) ** 2)
+
+
+def test_sheet_connectome_structure():
+    layers = [100, 64, 10]
+    con = Connectome.sheet_recurrent(layers, fan_in=12, rec_fan_in=6, radius=0.15, seed=3)
+    g = con.to_graph()
+    x, y, lay = _sheet_xy(layers)
+    pre, post = g.pre_ids(), g.post_ids
+    assert np.all(pre != post)
+    # hidden posts: 12 from the previous layer, 6 from their own; outputs: 12
+    same = lay[pre] == lay[post]
+    assert np.all((lay[pre] == lay[post] - 1) | (same & (lay[post] == 1)))
+    for p in range(100, 174):
+        fw = int(((post == p) & ~same).sum())
+        rc = int(((post == p) & same).sum())
+        assert fw == 12 and rc == (6 if p < 164 else 0)
+    # locality: partners are much closer than random pairs of the same layers
+    d = _dist(x, y, pre, post)
+    rng = np.random.default_rng(0)
+    q = rng.integers(0, 100, 4000)
+    p = rng.integers(100, 164, 4000)
+    assert d[lay[post] == 1].mean() < 0.6 * _dist(x, y, q, p).mean()
+    again = Connectome.sheet_recurrent(layers, fan_in=12, rec_fan_in=6, radius=0.15, seed=3).to_graph()
+    assert np.array_equal(g.post_ids, again.post_ids) and np.array_equal(g.row_ptr, again.row_ptr)
+    assert list(np.unique(g.role)) == [0, 1, 2]
+
+
+def test_sheet_prune_takes_the_smallest_abs_weights_and_grows_in_window():
+    layers = [81, 49, 9]
+    con = Connectome.sheet_recurrent(layers, fan_in=10, rec_fan_in=5, radius=0.2, seed=5)
+    x, y, lay = _sheet_xy(layers)
+    g0 = con.to_graph()
+    m = g0.num_synapses
+    rng = np.random.default_rng(1)
+    w = rng.standard_normal(m).astype(np.float32) * 0.3
+    w[::7] = w[3]  # ties, broken by CSR position
+    con.set_edge_weights(w)
+    fires = rng.integers(0, 40, g0.num_neurons).astype(np.int64)
+    cfg = StructConfig(grow_per_epoch=25, protect_rounds=0, locality_window=0.3)
+    rep = con.structural_update(np.zeros(m, np.int64), fires, cfg, mode=GROW_HEURISTIC)
+    order = np.lexsort((np.arange(m), np.abs(w)))  # |w| ascending, then position
+    gone = set(order[:25].tolist())
+    pre0 = g0.pre_ids()
+    kept = [(int(pre0[e]), int(g0.post_ids[e])) for e in range(m) if e not in gone]
+    g1 = con.to_graph()
+    now = list(zip(g1.pre_ids().tolist(), g1.post_ids.tolist()))
+    assert rep["pruned"] == 25
+    new = [e for e in now if e not in set(kept)]
+    assert len(new) == rep["grown"] > 0
+    for q, p in new:  # every grown synapse: a window member of an allowed layer
+        assert _dist(x, y, q, p) <= np.float64(np.float32(0.3))
+        assert lay[q] == lay[p] - 1 or (lay[q] == lay[p] == 1)
+
+
+def test_sheet_candidate_distance_feature():
+    layers = [64, 36, 4]
+    con = Connectome.sheet_recurrent(layers, fan_in=8, rec_fan_in=4, radius=0.2, seed=2)
+    x, y, _ = _sheet_xy(layers)
+    pre = np.array([0, 5, 63, 70], np.int32)
+    post = np.array([64, 80, 99, 71], np.int32)
+    F = con.candidate_features(pre, post, np.zeros(104, np.int64))
+    assert np.array_equal(F[:, 3], _dist(x, y, pre, post).astype(np.float32))
