@@ -519,6 +519,7 @@ def main():
         from snc_amd.growlab import ab_compare
 
         out["grow_ab"] = ab_compare(log=log)
+        out["grow_ab_sheet"] = ab_compare(log=log, kind="sheet")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_repeats)
     print(json.dumps(out))

@@ -11,6 +11,9 @@ leaky readout; prototype images with 20 % pixel noise, Poisson p = pixel/255
 prune = 200 per structural epoch, 12 structural epochs of one training epoch
 each. Logging runs explore: a share of freed posts regrows from a uniformly
 drawn window candidate, so the log holds outcomes of non-greedy choices too.
+
+kind="sheet" runs the same loop on the 2-D recurrent sheet connectome
+(SHEET below) with the structural step on the GPU.
 """
 from __future__ import annotations
 
@@ -34,18 +37,38 @@ def workload(n_train=2048, n_test=512, seed=1):
     return w, cfg
 
 
+# The sheet variant: 784 -> 512 (recurrent) -> 10 on 2-D grids, 32 partners
+# from the previous layer and 8 recurrent per hidden neuron (p ~ exp(-d/0.2)),
+# signed N(0, 0.2) start weights, grow = prune = 300 per round within discs of
+# radius 0.1, structural step on the GPU.
+SHEET = dict(fan_in=32, rec_fan_in=8, radius=0.2, w_std=0.2, grow=300, window=0.1)
+
+
 def run_cotrain(seed, mode, w, cfg, model=None, log=False, explore=0.0, outer=12, budget=3000,
-                grow=200, on_round=None):
-    """One co-training run: returns (connectome, per-round records)."""
+                grow=200, on_round=None, kind="layered"):
+    """One co-training run: returns (connectome, per-round records). kind
+    "layered" is the reference's Connectome; "sheet" the 2-D recurrent one."""
+    if kind == "sheet":
+        con = Connectome.sheet_recurrent(LAYERS, fan_in=SHEET["fan_in"],
+                                         rec_fan_in=SHEET["rec_fan_in"], radius=SHEET["radius"],
+                                         seed=seed)
+        con.set_edge_weights((np.random.default_rng(seed + 1000).standard_normal(
+            con.num_synapses) * SHEET["w_std"]).astype(np.float32))
+        grow, window, on_device = SHEET["grow"], SHEET["window"], True
+    else:
+        con = Connectome(LAYERS, budget, 1, 0.02, seed)
+        window, on_device = 0.2, False
+
     def make(g):
         s = HipTrainSession(g, dataclasses.replace(cfg, seed=seed), batch=w.batch,
                             max_steps=w.max_steps)
         s.upload_features(w.X, w.labels, w.lengths)
+        if kind == "sheet":
+            s.set_weights(con.edge_weights())  # signed start weights from round 0
         return s
 
-    con = Connectome(LAYERS, budget, 1, 0.02, seed)
-    scfg = StructConfig(grow_per_epoch=grow, delay=1, w_grow_init=0.02, locality_window=0.2,
-                        protect_rounds=2, seed=seed, explore=explore)
+    scfg = StructConfig(grow_per_epoch=grow, delay=1, w_grow_init=0.02, locality_window=window,
+                        protect_rounds=2, seed=seed, explore=explore, on_device=on_device)
     n = w.labels.size
     train_idx = np.arange(w.n_train, dtype=np.int32)
     test_idx = np.arange(w.n_train, n, dtype=np.int32)
@@ -56,11 +79,12 @@ def run_cotrain(seed, mode, w, cfg, model=None, log=False, explore=0.0, outer=12
     return con, rounds
 
 
-def collect_log(seeds, w, cfg, explore=0.5, outer=12):
+def collect_log(seeds, w, cfg, explore=0.5, outer=12, kind="layered"):
     """Labelled grow events (features, survived) from heuristic runs."""
     Xs, ys = [], []
     for s in seeds:
-        con, _ = run_cotrain(s, GROW_HEURISTIC, w, cfg, log=True, explore=explore, outer=outer)
+        con, _ = run_cotrain(s, GROW_HEURISTIC, w, cfg, log=True, explore=explore, outer=outer,
+                             kind=kind)
         X, y = con.grow_log()
         keep = y >= 0
         Xs.append(X[keep])
@@ -69,13 +93,13 @@ def collect_log(seeds, w, cfg, explore=0.5, outer=12):
 
 
 def ab_compare(seeds=(1, 2, 3, 4, 5), train_seeds=(101, 102, 103), outer=12, mlp_epochs=40,
-               explore=0.5, log=print):
+               explore=0.5, log=print, kind="layered"):
     """Learned vs heuristic grow at equal budget. Returns a dict with the
     per-seed final test accuracies and synapse counts, the MLP's loss before
     and after training, and its scoring rate on one million candidates."""
     t0 = time.time()
     w, cfg = workload()
-    Xl, yl = collect_log(train_seeds, w, cfg, explore=explore, outer=outer)
+    Xl, yl = collect_log(train_seeds, w, cfg, explore=explore, outer=outer, kind=kind)
     log(f"grow log: {Xl.shape[0]} labelled events, survival rate {yl.mean():.3f} "
         f"({time.time() - t0:.1f}s)")
     mlp = GrowMLP(seed=11)
@@ -86,8 +110,8 @@ def ab_compare(seeds=(1, 2, 3, 4, 5), train_seeds=(101, 102, 103), outer=12, mlp
            "mlp_loss_before": float(loss0), "mlp_loss_after": float(loss),
            "heuristic": [], "learned": [], "synapses": [], "candidates_scored_per_step": []}
     for s in seeds:
-        _, rh = run_cotrain(s, GROW_HEURISTIC, w, cfg, outer=outer)
-        _, rl = run_cotrain(s, GROW_LEARNED, w, cfg, model=mlp, outer=outer)
+        _, rh = run_cotrain(s, GROW_HEURISTIC, w, cfg, outer=outer, kind=kind)
+        _, rl = run_cotrain(s, GROW_LEARNED, w, cfg, model=mlp, outer=outer, kind=kind)
         res["heuristic"].append(rh[-1]["test_acc"])
         res["learned"].append(rl[-1]["test_acc"])
         res["synapses"].append([rh[-1]["synapses"], rl[-1]["synapses"]])
@@ -119,8 +143,9 @@ if __name__ == "__main__":
     ap.add_argument("--outer", type=int, default=12)
     ap.add_argument("--mlp-epochs", type=int, default=40)
     ap.add_argument("--explore", type=float, default=0.5)
+    ap.add_argument("--kind", choices=["layered", "sheet"], default="layered")
     a = ap.parse_args()
     out = ab_compare(tuple(int(x) for x in a.seeds.split(",")),
                      tuple(int(x) for x in a.train_seeds.split(",")), a.outer, a.mlp_epochs,
-                     a.explore, log=lambda m: print(m, file=sys.stderr, flush=True))
+                     a.explore, log=lambda m: print(m, file=sys.stderr, flush=True), kind=a.kind)
     print(json.dumps(out))
