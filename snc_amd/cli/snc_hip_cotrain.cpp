@@ -6,8 +6,13 @@
 // with the MLP, trained first on the GPU from grow outcomes logged by
 // heuristic runs (--log-runs of them, exploring a share --explore of grows).
 //
+// --structure sheet runs the 2-D recurrent sheet connectome instead of the
+// layered one (--fan-in, --rec-fan-in, --radius; signed N(0, --w-std) start
+// weights), with the structural step on the GPU unless --host-step.
+//
 //   snc_hip_cotrain --grow heuristic
 //   snc_hip_cotrain --grow learned --log-runs 3 --explore 0.5
+//   snc_hip_cotrain --structure sheet --grow-n 300 --locality 0.1
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -21,7 +26,9 @@ using namespace snc_cli;
 namespace {
 
 struct Options {
-  std::string grow = "heuristic";
+  std::string grow = "heuristic", structure = "layered";
+  int fan_in = 32, rec_fan_in = 8, host_step = 0;
+  float radius = 0.2f, w_std = 0.2f;
   int outer = 12, inner = 1, batch = 256, num_steps = 200, min_steps = 50, num_train = 2048,
       num_test = 512, hidden = 512, dim = 784, classes = 10, budget = 3000, grow_n = 200,
       protect = 2, log_runs = 3, mlp_epochs = 40;
@@ -37,7 +44,9 @@ void usage() {
       "  --protect R  --locality F  --w-grow F  --batch N  --num-steps T  --min-steps T\n"
       "  --num-train N  --num-test N  --hidden N  --dim N  --classes N  --lr F  --w-init F\n"
       "  --noise F  --seed S\n"
-      "  learned: --log-runs N  --explore F  --mlp-epochs N  --mlp-lr F\n");
+      "  learned: --log-runs N  --explore F  --mlp-epochs N  --mlp-lr F\n"
+      "  --structure layered|sheet; sheet: --fan-in N  --rec-fan-in N  --radius F  --w-std F\n"
+      "  --host-step\n");
 }
 
 Options parse(int argc, char** argv) {
@@ -47,6 +56,10 @@ Options parse(int argc, char** argv) {
     if (a == "-h" || a == "--help") {
       usage();
       std::exit(0);
+    }
+    if (a == "--host-step") {
+      o.host_step = 1;
+      continue;
     }
     if (take(i, argc, argv, "--grow", o.grow) || take(i, argc, argv, "--outer", o.outer) ||
         take(i, argc, argv, "--inner", o.inner) || take(i, argc, argv, "--batch", o.batch) ||
@@ -62,10 +75,18 @@ Options parse(int argc, char** argv) {
         take(i, argc, argv, "--w-init", o.w_init) || take(i, argc, argv, "--w-grow", o.w_grow) ||
         take(i, argc, argv, "--locality", o.locality) || take(i, argc, argv, "--noise", o.noise) ||
         take(i, argc, argv, "--explore", o.explore) || take(i, argc, argv, "--mlp-lr", o.mlp_lr) ||
-        take(i, argc, argv, "--seed", o.seed))
+        take(i, argc, argv, "--seed", o.seed) ||
+        take(i, argc, argv, "--structure", o.structure) ||
+        take(i, argc, argv, "--fan-in", o.fan_in) ||
+        take(i, argc, argv, "--rec-fan-in", o.rec_fan_in) ||
+        take(i, argc, argv, "--radius", o.radius) || take(i, argc, argv, "--w-std", o.w_std))
       continue;
     std::fprintf(stderr, "unknown option: %s\n", a.c_str());
     usage();
+    std::exit(2);
+  }
+  if (o.structure != "layered" && o.structure != "sheet") {
+    std::fprintf(stderr, "unknown structure: %s\n", o.structure.c_str());
     std::exit(2);
   }
   return o;
@@ -107,9 +128,21 @@ RunResult run(const Options& o, const Data& d, uint64_t seed, int grow_mode, snc
               bool log, float explore, snc_connectome** keep, bool verbose) {
   const int32_t layers[3] = {o.dim, o.hidden, o.classes};
   snc_connectome* c = nullptr;
-  check(snc_conn_layered_local(layers, 3, o.budget, 1, 0.02f, seed, &c), "connectome");
+  const bool sheet = o.structure == "sheet";
+  if (sheet) {
+    check(snc_conn_sheet_recurrent(layers, 3, o.fan_in, o.rec_fan_in, o.radius, 1, o.w_grow, seed,
+                                   &c),
+          "sheet connectome");
+    std::vector<float> w0(snc_conn_num_synapses(c));
+    std::mt19937_64 wr(seed + 1000);
+    std::normal_distribution<float> nd(0.0f, o.w_std);
+    for (float& x : w0) x = nd(wr);
+    check(snc_conn_set_edge_weights(c, w0.data(), (int64_t)w0.size()), "start weights");
+  } else {
+    check(snc_conn_layered_local(layers, 3, o.budget, 1, 0.02f, seed, &c), "connectome");
+  }
   const snc_struct_config sc{o.grow_n, 1, o.w_grow, o.locality, o.protect, 8, 0, 0.1f, 0.1f,
-                             seed, explore, 0};
+                             seed, explore, sheet && !o.host_step ? 1 : 0};
   const snc_train_config cfg{o.num_steps, {1.0f, 0.95122942f, 0.0f, 1, SNC_RESET_SUBTRACT},
                              o.lr, o.w_init, 4.0f, 0.3f, 1.0f, 1, SNC_READOUT_LEAKY, 1.0f,
                              SNC_ENC_POISSON, 1.0f, 0.2f, 0.0f, seed};
@@ -122,7 +155,7 @@ RunResult run(const Options& o, const Data& d, uint64_t seed, int grow_mode, snc
     snc_hip_train* s = nullptr;
     check(snc_hip_train_create(&g.view, &cfg, o.batch, o.num_steps, SNC_ELIG_F64, &s), "session");
     std::vector<float> w(g.view.num_synapses);
-    if (r > 0) {
+    if (r > 0 || sheet) {  // the sheet's signed start weights from round 0
       check(snc_conn_edge_weights(c, w.data(), (int64_t)w.size()), "edge weights");
       check(snc_hip_train_set_weights(s, w.data(), (int64_t)w.size()), "set weights");
     }

@@ -83,3 +83,15 @@ def test_cotrain_driver_runs(grow):
               "--batch", "64", "--structural-budget", "1500", "--grow-n", "60", "--log-runs", "1"])
     assert r.returncode == 0 and "final: grow=" + grow in r.stdout
     assert r.stdout.count("round ") == 4
+
+
+def test_cotrain_driver_sheet_structure_learns():
+    r = _run([os.path.join(BIN, "snc_hip_cotrain"), "--structure", "sheet", "--grow", "learned",
+              "--outer", "4", "--inner", "2", "--num-train", "1024", "--num-test", "256",
+              "--grow-n", "300", "--locality", "0.1", "--log-runs", "1"])
+    assert r.returncode == 0 and "final: grow=learned" in r.stdout
+    acc = [float(l.split("test_acc=")[1].split()[0]) for l in r.stdout.splitlines()
+           if l.startswith("round")]
+    assert len(acc) == 4 and acc[-1] > 0.2 and acc[-1] > acc[0] + 0.1  # ten classes
+    assert all("scored=" in l and "scored=0 " not in l for l in r.stdout.splitlines()
+               if l.startswith("round") and "pruned=" in l)
