@@ -84,6 +84,29 @@ def scatter_floor(g, cfg, events_per_step, kernel_ms):
             "floor_ops_per_s": ops / (floor_us * 1e-6) if floor_us > 0 else None}
 
 
+def expansion_floor(g, hg, plan, num_steps, kernel_ms):
+    """The hip-atomic step's delivery alone: the fired rows of one step (the
+    step whose event count is nearest the run's mean) replayed with the step
+    kernel's own expansion placement and nothing else
+    (snc_hip_measure_expansion). frac = floor / kernel time: the share of the
+    step that its atomics alone take; the rest is the state, injection and
+    row-pointer round trips before them."""
+    import numpy as np
+
+    r = plan.run(capture_raster=True)
+    n = g.num_neurons
+    bits = np.unpackbits(r.raster.view(np.uint8), axis=1, bitorder="little")[:, :n]
+    outdeg = np.diff(g.row_ptr).astype(np.int64)
+    ev = bits.astype(np.int64) @ outdeg
+    t = int(np.argmin(np.abs(ev - ev.mean())))
+    fired = np.nonzero(bits[t])[0].astype(np.int32)
+    us = hg.measure_expansion(fired)
+    k_us = kernel_ms * 1e3
+    return {"step": t, "fired_rows": int(fired.size), "events": int(ev[t]),
+            "mean_events_per_step": float(ev.mean()), "floor_us": us, "kernel_us": k_us,
+            "frac": us / k_us if k_us > 0 else None}
+
+
 def measured_traffic(kernel_symbol):
     """HBM bytes per dispatch of a kernel from the committed counter summaries
     (profiles/*pmc_traffic*.md: calibrated FETCH_SIZE + WRITE_SIZE), or None."""
@@ -473,7 +496,10 @@ def main():
                               "frac": step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS}}
     if dom == "k_step":
         roofline["scatter_floor"] = scatter_floor(g, cfgp, events / w.num_steps, dk["avg_ms"])
-        log(f"scatter floor: {roofline['scatter_floor']}")
+        log(f"scatter floor (random addresses): {roofline['scatter_floor']}")
+        if cfgp.variant == parse_backend("hip-atomic"):
+            roofline["expansion_floor"] = expansion_floor(g, hg, plan, w.num_steps, dk["avg_ms"])
+            log(f"expansion floor (this step's own deposits): {roofline['expansion_floor']}")
 
     variants = {}
     for name in [v for v in args.compare.split(",") if v]:

@@ -251,6 +251,16 @@ int snc_hip_plan_memory(const snc_hip_plan* p, int64_t* bytes);
 int snc_hip_measure_scatter(int32_t kind, int64_t target_bytes, int64_t ops, int32_t blocks,
                             int32_t reps, double* us_per_launch);
 
+/* The hip-atomic step kernel's delivery alone, for one step's fired rows (no
+ * counterpart in the reference; a benchmark measurement): replays the
+ * expansion of rows `fired` of graph g exactly as the fused step kernel
+ * places it (the rows of each 512-neuron block on that block's waves, one
+ * wave per row, lanes striding the row) into a scratch ring with float
+ * atomics, graph-launched `reps` times; returns the best average microseconds
+ * per launch -- the time floor of a step with these events. */
+int snc_hip_measure_expansion(const snc_hip_graph* g, const int32_t* fired, int32_t n_fired,
+                              int32_t reps, double* us_per_launch);
+
 /* One-shot forward, the shape of cuda::forward: upload, run, read back, free. */
 int snc_hip_forward(const snc_graph_view* g, const snc_event* events, int64_t n_events,
                     int32_t num_steps, const snc_lif* lif, int32_t variant,

@@ -219,6 +219,8 @@ _SIGS = {
                                   C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
     "snc_hip_graph_memory": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "snc_hip_plan_memory": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "snc_hip_measure_expansion": (C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                            C.POINTER(C.c_double)]),
     "snc_hip_measure_scatter": (C.c_int, [C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
                                           C.POINTER(C.c_double)]),
     # SNCG files (snc_gen.h)

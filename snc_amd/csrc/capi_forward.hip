@@ -294,12 +294,39 @@ int snc_hip_graph_upload(const snc_graph_view* gv, snc_hip_graph** out) {
   g.n_heavy = (int)heavy.size();
   g.heavy_host = heavy;
 
+  // Device row order: each row's synapses stably sorted by post. The lanes
+  // of a wave expanding a row then deposit into neighbouring words, so one
+  // atomic instruction touches few cache lines; scattered device atomics
+  // retire per line, not per lane. Results do not depend on this order: the
+  // ordered folds follow column positions (cpos), and a row's synapses to one
+  // post keep their CSR order.
+  std::vector<int32_t> dpost(S), dcpos(S);
+  std::vector<float> dwsig(S);
+  std::vector<uint8_t> ddel(S);
+  {
+    std::vector<int32_t> ord;
+    for (int i = 0; i < n; ++i) {
+      const int rb = gv->row_ptr[i], re = gv->row_ptr[i + 1];
+      ord.resize(re - rb);
+      for (int e = rb; e < re; ++e) ord[e - rb] = e;
+      std::stable_sort(ord.begin(), ord.end(),
+                       [&](int32_t x, int32_t y) { return gv->post_ids[x] < gv->post_ids[y]; });
+      for (int k = 0; k < re - rb; ++k) {
+        const int32_t e = ord[k];
+        dpost[rb + k] = gv->post_ids[e];
+        dwsig[rb + k] = wsig[e];
+        ddel[rb + k] = del8[e];
+        dcpos[rb + k] = cpos[e];
+      }
+    }
+  }
+
   int rc = SNC_OK;
   if (!rc) rc = dupload(&g.row_ptr, gv->row_ptr, (size_t)n + 1);
-  if (!rc) rc = dupload(&g.post, gv->post_ids, (size_t)S);
-  if (!rc) rc = dupload(&g.wsig, wsig.data(), (size_t)S);
-  if (!rc) rc = dupload(&g.delay, del8.data(), (size_t)S);
-  if (!rc) rc = dupload(&g.cpos, cpos.data(), (size_t)S);
+  if (!rc) rc = dupload(&g.post, dpost.data(), (size_t)S);
+  if (!rc) rc = dupload(&g.wsig, dwsig.data(), (size_t)S);
+  if (!rc) rc = dupload(&g.delay, ddel.data(), (size_t)S);
+  if (!rc) rc = dupload(&g.cpos, dcpos.data(), (size_t)S);
   if (!rc) rc = dupload(&g.ccol, ccol.data(), (size_t)n + 1);
   if (!rc) rc = dupload(&g.cval, cval.data(), (size_t)P);
   if (!rc) rc = dupload(&g.cdel, cdel.data(), (size_t)P);

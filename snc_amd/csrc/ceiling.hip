@@ -12,8 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "common.hpp"
+#include "device_graph.hpp"
 #include "hip_err.hpp"
 
 namespace snc_amd {
@@ -56,10 +58,118 @@ void launch_probe(float* buf, uint32_t words, int64_t ops, int blocks, uint32_t 
                      ops, per, rep, sink);
 }
 
+// The fused step kernel's expansion alone: block b's waves take the fired
+// rows of its 512 neurons (rows[bp[b] .. bp[b+1])), one wave per row, lanes
+// striding the row four edges at a time, float atomics into ring slot
+// (tm + d) % L -- the same code path and placement as the step kernel.
+__global__ __launch_bounds__(512) void k_expand_probe(const int32_t* __restrict__ row_ptr,
+                                                      const int32_t* __restrict__ post,
+                                                      const float* __restrict__ wsig,
+                                                      const uint8_t* __restrict__ delay,
+                                                      const int32_t* __restrict__ bp,
+                                                      const int32_t* __restrict__ rows, int n,
+                                                      int L, int tm, float* __restrict__ ring) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = bp[blockIdx.x] + wave; k < bp[blockIdx.x + 1]; k += 8) {
+    const int i = rows[k];
+    const int rb = row_ptr[i], re = row_ptr[i + 1];
+    for (int g = rb; g < re; g += 4 * 64) {
+      int d[4], tgt[4];
+      float w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = min(g + lane + u * 64, re - 1);
+        d[u] = delay[e];
+        tgt[u] = post[e];
+        w[u] = wsig[e];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (g + lane + u * 64 >= re) continue;
+        int s = tm + d[u];
+        if (s >= L) s -= L;
+        atomicAdd(ring + (size_t)s * n + tgt[u], w[u]);
+      }
+    }
+  }
+}
+
 }  // namespace
 }  // namespace snc_amd
 
 using namespace snc_amd;
+
+extern "C" int snc_hip_measure_expansion(const snc_hip_graph* G, const int32_t* fired,
+                                         int32_t n_fired, int32_t reps, double* us_per_launch) {
+  if (!G || !us_per_launch || n_fired < 0 || (n_fired > 0 && !fired) || reps < 1)
+    return fail(SNC_E_INVALID, "bad expansion probe arguments");
+  const DevGraph& g = G->g;
+  const int n = g.n, R = std::max(1, (n + 511) / 512);
+  std::vector<int32_t> bp((size_t)R + 1, 0), rows;
+  for (int k = 0; k < n_fired; ++k) {
+    if (fired[k] < 0 || fired[k] >= n) return fail(SNC_E_INVALID, "fired neuron out of range");
+    bp[fired[k] / 512 + 1]++;
+  }
+  for (int b = 0; b < R; ++b) bp[b + 1] += bp[b];
+  rows.resize((size_t)n_fired);
+  {
+    std::vector<int32_t> at(bp.begin(), bp.end() - 1);
+    for (int k = 0; k < n_fired; ++k) rows[at[fired[k] / 512]++] = fired[k];
+  }
+  hip_clear();
+  int32_t *d_bp = nullptr, *d_rows = nullptr;
+  float* ring = nullptr;
+  hipStream_t st = nullptr;
+  hipGraphExec_t ge = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SNC_OK;
+  hipError_t e = hipMalloc((void**)&d_bp, bp.size() * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_rows, std::max<size_t>(rows.size(), 1) * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&ring, (size_t)g.L * n * 4);
+  if (e == hipSuccess) e = hipMemcpy(d_bp, bp.data(), bp.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !rows.empty())
+    e = hipMemcpy(d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(ring, 0, (size_t)g.L * n * 4);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) rc = hip_fail(e, "expansion probe setup");
+  if (!rc) {
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_expand_probe, dim3(R), dim3(512), 0, st, g.row_ptr, g.post, g.wsig,
+                         g.delay, d_bp, d_rows, n, g.L, r % g.L, ring);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(st, &graph);
+    if (e == hipSuccess && graph) {
+      e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+    }
+    if (e != hipSuccess) rc = hip_fail(e, "expansion probe capture");
+  }
+  if (!rc) {
+    float best = 1e30f;
+    for (int it = 0; it < 4 && !rc; ++it) {
+      e = hipEventRecord(e0, st);
+      if (e == hipSuccess) e = hipGraphLaunch(ge, st);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) rc = hip_fail(e, "expansion probe run");
+      else if (it > 0) best = std::min(best, ms);
+    }
+    if (!rc) *us_per_launch = (double)best * 1e3 / reps;
+  }
+  if (ge) (void)hipGraphExecDestroy(ge);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (d_bp) (void)hipFree(d_bp);
+  if (d_rows) (void)hipFree(d_rows);
+  if (ring) (void)hipFree(ring);
+  return rc;
+}
 
 extern "C" int snc_hip_measure_scatter(int32_t kind, int64_t target_bytes, int64_t ops,
                                        int32_t blocks, int32_t reps, double* us_per_launch) {

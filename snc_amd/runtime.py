@@ -158,6 +158,16 @@ class HipGraph:
         L.check(L.lib().snc_hip_graph_memory(self._h, C.byref(b)))
         return int(b.value)
 
+    def measure_expansion(self, fired, reps: int = 200) -> float:
+        """Best average microseconds per launch of the hip-atomic step
+        kernel's delivery alone for these fired rows
+        (snc_hip_measure_expansion)."""
+        f = np.ascontiguousarray(fired, np.int32)
+        us = C.c_double()
+        L.check(L.lib().snc_hip_measure_expansion(self._h, L.ptr(f, C.c_int32), f.size, int(reps),
+                                                  C.byref(us)))
+        return us.value
+
     def tune_report(self, exact_only=False, batched=False) -> dict:
         """The autotuner's search for this graph (after a hip-auto plan):
         candidates with their measured ms per step, and the chosen one."""
