@@ -294,6 +294,14 @@ typedef struct snc_train_config {
   float enc_max_rate;
   float enc_threshold;
   uint64_t seed;
+  /* Three-factor reward rule (trainer.cpp:165-179): per trained sample, in
+   * minibatch order, an action a ~ softmax(logits) drawn with one
+   * uniform_real_distribution<float> value of mt19937_64(seed ^ 0xA5A5C0FFEE),
+   * reward R = 1[a == label], delta = (R - baseline) * (p - onehot(a)), then
+   * baseline <- decay * baseline + (1 - decay) * R (baseline starts at
+   * 1 / classes). 0 = the supervised delta p - onehot(label). */
+  int32_t reward_mode;
+  float reward_baseline_decay;
 } snc_train_config;
 
 #define SNC_TRAIN_PLAIN 0   /* one sample at a time, per-step kernel launches       */
@@ -342,6 +350,9 @@ typedef struct snc_train_stats {
  * current weights; when apply is set the weights then move by the
  * minibatch-mean e-prop update (cuda_trainer.cu:163-172, with lr / T_b per
  * sample): w <- clamp(w - sum_b (lr / T_b * L_b[post]) * E_b / B, +-w_max).
+ * In reward mode the samples draw their actions and advance the baseline in
+ * idx order, so B = 1 is the CPU Trainer's per-sample step exactly; steps
+ * without apply draw nothing (the reference draws only in apply_update).
  * logits (optional) receives [B][classes]; raster (optional) [B][T_max][ceil(N/32)]. */
 int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t epoch,
                        int32_t mode, int32_t apply, float* logits, uint32_t* raster,

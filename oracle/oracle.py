@@ -93,7 +93,8 @@ def ref_lib():
             "ref_encode": (I64, [I32, V, I32, I32, F, F, F, U64, I32, V, I64]),
             "ref_make_synthetic": (None, [I32, I32, I32, F, U64, V, V]),
             "ref_trainer_create": (V, [I32, I32] + [V] * 7 + [I32, F, F, F, I32, I32, F, F, F,
-                                                             F, F, F, F, F, I32, U64]),
+                                                             F, F, F, F, F, I32, U64, I32,
+                                                             F]),
             "ref_trainer_free": (None, [V]),
             "ref_trainer_get_weights": (None, [V, V]),
             "ref_trainer_set_weights": (None, [V, V, I64]),
@@ -303,16 +304,19 @@ class TrainConfig(C.Structure):
                 ("feedback_scale", C.c_float), ("train_hidden", C.c_int32),
                 ("readout_mode", C.c_int32), ("readout_psi", C.c_float), ("encoder", C.c_int32),
                 ("enc_gain", C.c_float), ("enc_max_rate", C.c_float),
-                ("enc_threshold", C.c_float), ("seed", C.c_uint64)]
+                ("enc_threshold", C.c_float), ("seed", C.c_uint64),
+                ("reward_mode", C.c_int32), ("reward_baseline_decay", C.c_float)]
 
 
 def train_config(num_steps=30, lif=None, lr=0.02, w_init=0.3, w_max=4.0, gamma=0.3,
                  feedback=1.0, train_hidden=True, readout_mode=0, readout_psi=1.0, encoder=1,
-                 gain=1.0, max_rate=0.5, enc_threshold=0.0, seed=1):
+                 gain=1.0, max_rate=0.5, enc_threshold=0.0, seed=1, reward_mode=False,
+                 reward_baseline_decay=0.99):
     lc = _lif(lif) if lif is not None else _Lif(1.0, 0.9, 0.0, 1, 0)
     return TrainConfig(num_steps, lc, lr, w_init, w_max, gamma, feedback,
                        1 if train_hidden else 0, readout_mode, readout_psi, encoder, gain,
-                       max_rate, enc_threshold, seed)
+                       max_rate, enc_threshold, seed, 1 if reward_mode else 0,
+                       reward_baseline_decay)
 
 
 def _bind_train(lib):
@@ -439,7 +443,8 @@ class RefTrainer:
                                                L.decay, L.reset, L.refractory, cfg.encoder,
                                                cfg.enc_gain, cfg.enc_max_rate, cfg.enc_threshold,
                                                cfg.lr, cfg.w_init, cfg.w_max, cfg.surrogate_scale,
-                                               cfg.feedback_scale, cfg.train_hidden, cfg.seed)
+                                               cfg.feedback_scale, cfg.train_hidden, cfg.seed,
+                                               cfg.reward_mode, cfg.reward_baseline_decay)
 
     def __del__(self):
         try:

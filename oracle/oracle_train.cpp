@@ -8,8 +8,10 @@
 //   run_sample    src/training/trainer.cpp:52-152  (inject, LIF + surrogate,
 //                                                   trace, eligibility in double,
 //                                                   delivery of signed weights)
-//   apply_update  src/training/trainer.cpp:154-208 (softmax, delta, learning
-//                                                   signal, w -= lr/T * L * E)
+//   apply_update  src/training/trainer.cpp:154-208 (softmax, delta -- supervised
+//                                                   or reward advantage --,
+//                                                   learning signal,
+//                                                   w -= lr/T * L * E)
 //   train_epoch   src/training/trainer.cpp:210-249 (shuffle, sample seeds)
 // Compiled with -ffp-contract=off so each float expression keeps the
 // reference's evaluation order.
@@ -43,6 +45,8 @@ struct Trainer {
   std::vector<int> internal_idx;
   std::vector<std::vector<int>> chan_to_neurons;
   std::vector<int64_t> syn_deliv, neur_fire;
+  std::mt19937_64 action_rng;  // reward mode (trainer.cpp:39-40)
+  float baseline = 0.0f;
 
   bool leaky_out(int j) const {
     return cfg.readout_mode == SNC_READOUT_LEAKY && role[j] == SNC_ROLE_OUTPUT;
@@ -152,8 +156,11 @@ struct Trainer {
     return by_step;
   }
 
-  // delta = softmax(logits) - onehot(label) (trainer.cpp:158-180, supervised).
-  std::vector<float> delta_of(const std::vector<float>& logits, int label) const {
+  // delta = softmax(logits) - onehot(label), or in reward mode
+  // (R - baseline) * (softmax - onehot(a)) with a ~ softmax drawn from the
+  // action stream, R = 1[a == label], then the baseline's EMA step
+  // (trainer.cpp:158-180).
+  std::vector<float> delta_of(const std::vector<float>& logits, int label) {
     std::vector<float> p(classes);
     const float mx = *std::max_element(logits.begin(), logits.end());
     double sum = 0.0;
@@ -163,6 +170,24 @@ struct Trainer {
     }
     for (int k = 0; k < classes; ++k) p[k] /= (float)sum;
     std::vector<float> d(classes);
+    if (cfg.reward_mode) {
+      std::uniform_real_distribution<float> u(0.0f, 1.0f);
+      const float r = u(action_rng);
+      float cdf = 0.0f;
+      int a = classes - 1;
+      for (int k = 0; k < classes; ++k) {
+        cdf += p[k];
+        if (r <= cdf) {
+          a = k;
+          break;
+        }
+      }
+      const float R = a == label ? 1.0f : 0.0f;
+      const float adv = R - baseline;
+      for (int k = 0; k < classes; ++k) d[k] = adv * (p[k] - (k == a ? 1.0f : 0.0f));
+      baseline = cfg.reward_baseline_decay * baseline + (1.0f - cfg.reward_baseline_decay) * R;
+      return d;
+    }
     for (int k = 0; k < classes; ++k) d[k] = p[k] - (k == label ? 1.0f : 0.0f);
     return d;
   }
@@ -237,6 +262,8 @@ void* oracle_trainer_create(const snc_graph_view* g, const snc_train_config* cfg
   std::normal_distribution<float> bn(0.0f, cfg->feedback_scale);
   t->B.resize((size_t)n_int * t->classes);
   for (float& b : t->B) b = bn(rng);
+  t->action_rng.seed(cfg->seed ^ 0xA5A5C0FFEEULL);
+  t->baseline = 1.0f / t->classes;
   return t;
 }
 
@@ -331,6 +358,8 @@ double oracle_trainer_train_epoch_ref(void* h, const float* X, const int32_t* y,
 // its events (sample_id == b) for lengths[b] steps; then
 //   c_{b,s} = (lr / T_b * L_b[post_s]) * float(E_{b,s})
 //   w_s     = clamp(w_s - (sum_b c_{b,s}) / B, +-w_max)   (b ascending, float).
+// In reward mode L_b comes from the reward delta, drawn and baselined in
+// sample order; without apply no delta (and no action draw) is made.
 // logits_out [B][classes]; raster_out [B][T_max][words] when non-null.
 void oracle_trainer_minibatch(void* h, const snc_event* ev, int64_t n_ev, const int32_t* lengths,
                               const int32_t* labels, int32_t Bn, int32_t apply, float* logits_out,
@@ -349,6 +378,7 @@ void oracle_trainer_minibatch(void* h, const snc_event* ev, int64_t n_ev, const 
     sp += o.spikes;
     evn += o.events;
     if (logits_out) std::copy(o.logits.begin(), o.logits.end(), logits_out + (size_t)b * t->classes);
+    if (!apply) continue;  // no update, no action draw
     const std::vector<float> Ls = t->signal_of(t->delta_of(o.logits, labels[b]));
     const float scale = t->cfg.lr / (float)std::max(1, T);
     for (int s = 0; s < t->S; ++s) acc[s] += (scale * Ls[t->post[s]]) * (float)E[s];

@@ -162,7 +162,8 @@ void* ref_trainer_create(int32_t n, int32_t m, const int32_t* row_ptr, const int
                          float thr, float decay, float reset, int32_t refractory,
                          int32_t enc, float gain, float max_rate, float enc_thr, float lr,
                          float w_init, float w_max, float gamma, float feedback,
-                         int32_t train_hidden, uint64_t seed) {
+                         int32_t train_hidden, uint64_t seed, int32_t reward_mode,
+                         float reward_baseline_decay) {
   auto* r = new RefTrainer{graph_from_arrays(n, m, row_ptr, post, w, d, sign, role, channel),
                            nullptr};
   snc::TrainConfig cfg;
@@ -182,6 +183,8 @@ void* ref_trainer_create(int32_t n, int32_t m, const int32_t* row_ptr, const int
   cfg.feedback_scale = feedback;
   cfg.train_hidden = train_hidden != 0;
   cfg.seed = seed;
+  cfg.reward_mode = reward_mode != 0;
+  cfg.reward_baseline_decay = reward_baseline_decay;
   r->tr = new snc::Trainer(r->g, cfg);
   return r;
 }

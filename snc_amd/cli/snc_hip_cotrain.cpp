@@ -27,7 +27,7 @@ namespace {
 
 struct Options {
   std::string grow = "heuristic", structure = "layered";
-  int fan_in = 32, rec_fan_in = 8, host_step = 0;
+  int fan_in = 32, rec_fan_in = 8, host_step = 0, reward = 0, reward_struct = 0;
   float radius = 0.2f, w_std = 0.2f;
   int outer = 12, inner = 1, batch = 256, num_steps = 200, min_steps = 50, num_train = 2048,
       num_test = 512, hidden = 512, dim = 784, classes = 10, budget = 3000, grow_n = 200,
@@ -46,7 +46,7 @@ void usage() {
       "  --noise F  --seed S\n"
       "  learned: --log-runs N  --explore F  --mlp-epochs N  --mlp-lr F\n"
       "  --structure layered|sheet; sheet: --fan-in N  --rec-fan-in N  --radius F  --w-std F\n"
-      "  --host-step\n");
+      "  --host-step  --reward 0|1 (three-factor weight rule)  --reward-struct 0|1\n");
 }
 
 Options parse(int argc, char** argv) {
@@ -79,7 +79,9 @@ Options parse(int argc, char** argv) {
         take(i, argc, argv, "--structure", o.structure) ||
         take(i, argc, argv, "--fan-in", o.fan_in) ||
         take(i, argc, argv, "--rec-fan-in", o.rec_fan_in) ||
-        take(i, argc, argv, "--radius", o.radius) || take(i, argc, argv, "--w-std", o.w_std))
+        take(i, argc, argv, "--radius", o.radius) || take(i, argc, argv, "--w-std", o.w_std) ||
+        take(i, argc, argv, "--reward", o.reward) ||
+        take(i, argc, argv, "--reward-struct", o.reward_struct))
       continue;
     std::fprintf(stderr, "unknown option: %s\n", a.c_str());
     usage();
@@ -141,11 +143,15 @@ RunResult run(const Options& o, const Data& d, uint64_t seed, int grow_mode, snc
   } else {
     check(snc_conn_layered_local(layers, 3, o.budget, 1, 0.02f, seed, &c), "connectome");
   }
-  const snc_struct_config sc{o.grow_n, 1, o.w_grow, o.locality, o.protect, 8, 0, 0.1f, 0.1f,
-                             seed, explore, sheet && !o.host_step ? 1 : 0};
+  // reward_chance is chance accuracy, as cotrain_main.cpp:217-218 sets it.
+  const snc_struct_config sc{o.grow_n, 1, o.w_grow, o.locality, o.protect, 8,
+                             o.reward_struct != 0 ? 1 : 0,
+                             o.classes ? 1.0f / (float)o.classes : 0.1f, 0.1f, seed, explore,
+                             sheet && !o.host_step ? 1 : 0};
   const snc_train_config cfg{o.num_steps, {1.0f, 0.95122942f, 0.0f, 1, SNC_RESET_SUBTRACT},
                              o.lr, o.w_init, 4.0f, 0.3f, 1.0f, 1, SNC_READOUT_LEAKY, 1.0f,
-                             SNC_ENC_POISSON, 1.0f, 0.2f, 0.0f, seed};
+                             SNC_ENC_POISSON, 1.0f, 0.2f, 0.0f, seed, o.reward != 0 ? 1 : 0,
+                             0.99f};
   RunResult res;
   int epoch = 0;
   for (int r = 0; r < o.outer; ++r) {

@@ -24,7 +24,7 @@ struct Options {
   std::string readout = "leaky", reset = "subtract", encoder = "poisson", elig = "f64", log_csv;
   int epochs = 3, batch = 256, num_steps = 200, min_steps = 50, num_train = 2048,
       num_test = 512, hidden = 512, dim = 784, classes = 10, refractory = 1, fan_in = 32,
-      rec_fan_in = 56;
+      rec_fan_in = 56, reward_mode = 0;
   float lr = 1.0f, w_init = 0.0f, w_max = 4.0f, gamma = 0.3f, feedback = 1.0f, threshold = 1.0f,
         decay = 0.95122942f, gain = 1.0f, max_rate = 0.2f, noise = 0.2f, w_std = 0.2f;
   uint64_t seed = 1;
@@ -40,7 +40,8 @@ void usage() {
       "  --lr F  --w-init F (0: the graph's weights)  --w-max F  --w-std F  --gamma F\n"
       "  --feedback F  --threshold F  --decay F  --refractory R  --reset value|subtract\n"
       "  --readout spiking|leaky  --encoder direct|poisson  --gain F  --max-rate F\n"
-      "  --noise F  --elig f64|f32  --seed S  --log-csv FILE\n");
+      "  --noise F  --elig f64|f32  --seed S  --log-csv FILE\n"
+      "  --reward-mode 0|1   (three-factor reward rule instead of the supervised delta)\n");
 }
 
 Options parse(int argc, char** argv) {
@@ -71,7 +72,8 @@ Options parse(int argc, char** argv) {
         take(i, argc, argv, "--threshold", o.threshold) ||
         take(i, argc, argv, "--decay", o.decay) || take(i, argc, argv, "--gain", o.gain) ||
         take(i, argc, argv, "--max-rate", o.max_rate) || take(i, argc, argv, "--noise", o.noise) ||
-        take(i, argc, argv, "--seed", o.seed))
+        take(i, argc, argv, "--seed", o.seed) ||
+        take(i, argc, argv, "--reward-mode", o.reward_mode))
       continue;
     std::fprintf(stderr, "unknown option: %s\n", a.c_str());
     usage();
@@ -141,7 +143,8 @@ int main(int argc, char** argv) {
                              o.lr, o.w_init, o.w_max, o.gamma, o.feedback, 1,
                              o.readout == "leaky" ? SNC_READOUT_LEAKY : SNC_READOUT_SPIKING,
                              1.0f, o.encoder == "direct" ? SNC_ENC_DIRECT : SNC_ENC_POISSON,
-                             o.gain, o.max_rate, 0.0f, o.seed};
+                             o.gain, o.max_rate, 0.0f, o.seed, o.reward_mode != 0 ? 1 : 0,
+                             0.99f};
   snc_hip_train* s = nullptr;
   check(snc_hip_train_create(&g.view, &cfg, o.batch, o.num_steps,
                              o.elig == "f32" ? SNC_ELIG_F32 : SNC_ELIG_F64, &s),

@@ -94,6 +94,10 @@ struct snc_hip_train {
   void* p_E = nullptr;
   unsigned long long *p_spikes = nullptr, *p_events = nullptr;
   int32_t* p_cur = nullptr;
+  // reward mode: the action stream (trainer.cpp:39) and device state
+  std::mt19937_64 action_rng;
+  float *r_u = nullptr, *r_base = nullptr, *r_delta = nullptr, *r_L = nullptr, *r_adv = nullptr;
+  int32_t* r_act = nullptr;
   // captured plain-path chains, keyed by (sample length, update on/off)
   std::map<std::pair<int, int>, hipGraphExec_t> plain_graphs;
 
@@ -116,7 +120,8 @@ struct snc_hip_train {
                     (void*)neur_fire, (void*)raster, (void*)p_v, (void*)p_tr, (void*)p_psi,
                     (void*)p_ext, (void*)p_ring, (void*)p_logit, (void*)p_L, (void*)p_refr,
                     (void*)p_fired, (void*)p_nf, (void*)p_fw, (void*)p_bits, p_E,
-                    (void*)p_spikes, (void*)p_events, (void*)p_cur})
+                    (void*)p_spikes, (void*)p_events, (void*)p_cur, (void*)r_u,
+                    (void*)r_base, (void*)r_delta, (void*)r_L, (void*)r_adv, (void*)r_act})
       if (p) (void)hipFree(p);
   }
 };
@@ -261,6 +266,17 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   if (!rc) rc = talloc(&s->p_spikes, 1);
   if (!rc) rc = talloc(&s->p_events, 1);
   if (!rc) rc = talloc(&s->p_cur, 4);
+  if (!rc && cfg->reward_mode) {
+    // The baseline starts at chance, 1 / classes in float (trainer.cpp:40).
+    const float base0 = 1.0f / (float)s->classes;
+    rc = talloc(&s->r_u, (size_t)batch);
+    if (!rc) rc = tupload(&s->r_base, &base0, 1);
+    if (!rc) rc = talloc(&s->r_delta, (size_t)batch * s->classes);
+    if (!rc) rc = talloc(&s->r_L, (size_t)batch * n);
+    if (!rc) rc = talloc(&s->r_adv, (size_t)batch);
+    if (!rc) rc = talloc(&s->r_act, (size_t)batch);
+    s->action_rng.seed(cfg->seed ^ 0xA5A5C0FFEEULL);
+  }
   if (rc) return rc;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
@@ -285,6 +301,7 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   a.lr = cfg->lr;
   a.w_max = cfg->w_max;
   a.train_hidden = cfg->train_hidden;
+  a.reward = cfg->reward_mode ? 1 : 0;
   a.encoder = cfg->encoder;
   a.gain = cfg->enc_gain;
   a.max_rate = cfg->enc_max_rate;
@@ -480,6 +497,17 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
   int rc = SNC_OK;
   if (hipMemcpyAsync(s->idx, idx, sizeof(int32_t) * B, hipMemcpyHostToDevice, sm) != hipSuccess)
     return fail(SNC_E_HIP, "index upload failed");
+  const bool reward = apply && s->cfg.reward_mode;
+  if (reward) {
+    // One action draw per trained sample, in minibatch order (trainer.cpp:171-173).
+    std::vector<float> u(B);
+    std::uniform_real_distribution<float> ud(0.0f, 1.0f);
+    for (float& x : u) x = ud(s->action_rng);
+    if (hipMemcpyAsync(s->r_u, u.data(), sizeof(float) * B, hipMemcpyHostToDevice, sm) !=
+            hipSuccess ||
+        hipStreamSynchronize(sm) != hipSuccess)
+      return fail(SNC_E_HIP, "action draw upload failed");
+  }
   TrainArgs a = s->a;
   a.epoch = epoch;
   a.C = apply ? s->C : nullptr;
@@ -555,6 +583,10 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
   } else {
     return fail(SNC_E_INVALID, "unknown training mode");
   }
+  if (!rc && reward)
+    rc = train_launch_reward(a, {s->r_u, s->r_base, s->r_delta, s->r_L, s->r_act, s->r_adv,
+                                 s->cfg.reward_baseline_decay},
+                             B, sm);
   if (!rc && apply) rc = train_launch_apply(a, s->w, B, s->csc_e, s->wc, s->P, sm);
   if (rc) return rc;
   (void)hipEventRecord(s->ev1, sm);

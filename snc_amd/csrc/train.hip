@@ -16,6 +16,11 @@
 // C_{b,s} = (lr / T_b * L_post) * float(E_{b,s}). A separate kernel sums C over
 // the minibatch in sample order and applies w <- clamp(w - sum / B), the
 // minibatch-mean rule of the reference GPU trainer (cuda_trainer.cu:163-172).
+// In reward mode (trainer.cpp:165-179) a sample's delta depends on the
+// baseline left by the samples before it, so the sample kernels store
+// float(E_{b,s}) in C and three small kernels finish the minibatch: the
+// ordered action/advantage/baseline scan, the learning signals, and the
+// contributions.
 //
 // Batched (SNC_TRAIN_BATCHED): one workgroup per sample runs all of its T_b
 // steps in a single launch -- neuron state, fired bitmap, delivery buckets and
@@ -122,6 +127,22 @@ __device__ __forceinline__ void elig_add<float>(float* E, size_t k, float psi, f
   E[k] = __fmaf_rn(psi, tr, E[k]);
 }
 
+// Learning signal of neuron j from the output deltas (trainer.cpp:182-195):
+// an output neuron takes its channel's delta, a hidden one sum_k B[j,k] delta_k
+// (k ascending, float), every other neuron 0.
+__device__ __forceinline__ float learning_signal(const TrainArgs& a, const float* delta, int j) {
+  const int K = a.classes;
+  const int c = a.channel[j];
+  if (a.role[j] == SNC_ROLE_OUTPUT && c >= 0 && c < K) return delta[c];
+  if (a.train_hidden && a.internal_row[j] >= 0) {
+    const float* br = a.Bfb + (size_t)a.internal_row[j] * K;
+    float s = 0.0f;
+    for (int k = 0; k < K; ++k) s = __fadd_rn(s, __fmul_rn(br[k], delta[k]));
+    return s;
+  }
+  return 0.0f;
+}
+
 // End of sample: delta, loss, prediction, learning signal into Ls[n].
 // Runs on the whole block; s_delta holds `classes` floats.
 __device__ void sample_epilogue(const TrainArgs& a, int b, const float* logits_in, float* Ls,
@@ -146,19 +167,7 @@ __device__ void sample_epilogue(const TrainArgs& a, int b, const float* logits_i
     for (int k = 0; k < K; ++k) a.logits[(size_t)b * K + k] = logits_in[k];
   }
   __syncthreads();
-  for (int j = tid; j < n; j += nthreads) {
-    float l = 0.0f;
-    const int c = a.channel[j];
-    if (a.role[j] == SNC_ROLE_OUTPUT && c >= 0 && c < K) {
-      l = s_delta[c];
-    } else if (a.train_hidden && a.internal_row[j] >= 0) {
-      const float* br = a.Bfb + (size_t)a.internal_row[j] * K;
-      float s = 0.0f;
-      for (int k = 0; k < K; ++k) s = __fadd_rn(s, __fmul_rn(br[k], s_delta[k]));
-      l = s;
-    }
-    Ls[j] = l;
-  }
+  for (int j = tid; j < n; j += nthreads) Ls[j] = learning_signal(a, s_delta, j);
 }
 
 // ---------------------------------------------------------------------------
@@ -208,7 +217,8 @@ __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, cons
     for (int k = 0; k < kEligPerThread; ++k) {
       const int sy = s0 + k * kTrainThreads + tid;
       if (sy < a.S && a.C)
-        a.C[(size_t)b * a.S + sy] = __fmul_rn(__fmul_rn(scale, Ls[post[k]]), (float)E[k]);
+        a.C[(size_t)b * a.S + sy] =
+            a.reward ? (float)E[k] : __fmul_rn(__fmul_rn(scale, Ls[post[k]]), (float)E[k]);
     }
   }
 }
@@ -551,7 +561,71 @@ __global__ void k_p_contrib(TrainArgs a, PlainState ps, int T) {
   if (s >= a.S) return;
   const float scale = __fdiv_rn(a.lr, (float)max(1, T));
   const float e = (float)((const ET*)ps.E)[s];
-  a.C[(size_t)ps.cur[0] * a.S + s] = __fmul_rn(__fmul_rn(scale, ps.L[a.post[s]]), e);
+  a.C[(size_t)ps.cur[0] * a.S + s] = a.reward ? e : __fmul_rn(__fmul_rn(scale, ps.L[a.post[s]]), e);
+}
+
+// ---------------------------------------------------------------------------
+// reward mode (trainer.cpp:165-179)
+// ---------------------------------------------------------------------------
+
+// One block. Each sample's softmax p (the epilogue's arithmetic), its action
+// a ~ p by the first k with u_b <= cdf_k (cdf summed in float, the last class
+// when none), and its reward R = 1[a == label]; then one lane walks the
+// minibatch in order: adv_b = R_b - baseline, baseline <- decay * baseline +
+// (1 - decay) * R_b; then delta_b[k] = adv_b * (p_b[k] - onehot(a_b)[k]).
+__global__ __launch_bounds__(256) void k_reward_delta(TrainArgs a, RewardArgs r, int B) {
+  const int K = a.classes, tid = threadIdx.x;
+  for (int b = tid; b < B; b += blockDim.x) {
+    const float* lg = a.logits + (size_t)b * K;
+    float* p = r.delta + (size_t)b * K;  // p until the last phase
+    float mx = lg[0];
+    for (int k = 1; k < K; ++k) mx = fmaxf(mx, lg[k]);
+    double sum = 0.0;
+    for (int k = 0; k < K; ++k) sum += (double)expf(__fsub_rn(lg[k], mx));
+    const float u = r.u[b];
+    float cdf = 0.0f;
+    int act = -1;
+    for (int k = 0; k < K; ++k) {
+      p[k] = __fdiv_rn(expf(__fsub_rn(lg[k], mx)), (float)sum);
+      cdf = __fadd_rn(cdf, p[k]);
+      if (act < 0 && u <= cdf) act = k;
+    }
+    r.act[b] = act < 0 ? K - 1 : act;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    float base = *r.base;
+    const float keep = __fsub_rn(1.0f, r.decay);
+    for (int b = 0; b < B; ++b) {
+      const float R = r.act[b] == a.labels[a.idx[b]] ? 1.0f : 0.0f;
+      r.adv[b] = __fsub_rn(R, base);
+      base = __fadd_rn(__fmul_rn(r.decay, base), __fmul_rn(keep, R));
+    }
+    *r.base = base;
+  }
+  __syncthreads();
+  for (int q = tid; q < B * K; q += blockDim.x) {
+    const int b = q / K, k = q - b * K;
+    const float p = r.delta[q];
+    r.delta[q] = __fmul_rn(r.adv[b], __fsub_rn(p, k == r.act[b] ? 1.0f : 0.0f));
+  }
+}
+
+// Learning signals of every sample: Ls[b][j] from delta_b.
+__global__ void k_reward_signal(TrainArgs a, RewardArgs r) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (j >= a.n) return;
+  r.Ls[(size_t)b * a.n + j] = learning_signal(a, r.delta + (size_t)b * a.classes, j);
+}
+
+// C_{b,s} = (lr / T_b * L_b[post_s]) * float(E_{b,s}), in place over float(E).
+__global__ void k_reward_contrib(TrainArgs a, RewardArgs r) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (s >= a.S) return;
+  const int T = a.lengths[a.idx[b]];
+  const float scale = __fdiv_rn(a.lr, (float)max(1, T));
+  float* c = a.C + (size_t)b * a.S + s;
+  *c = __fmul_rn(__fmul_rn(scale, r.Ls[(size_t)b * a.n + a.post[s]]), *c);
 }
 
 // ---------------------------------------------------------------------------
@@ -656,6 +730,13 @@ int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int T, b
       hipLaunchKernelGGL(k_p_contrib<float>, dim3(gs), dim3(256), 0, st, a, ps, T);
   }
   return hip_last("plain trainer launch failed");
+}
+
+int train_launch_reward(const TrainArgs& a, const RewardArgs& r, int B, hipStream_t st) {
+  hipLaunchKernelGGL(k_reward_delta, dim3(1), dim3(256), 0, st, a, r, B);
+  hipLaunchKernelGGL(k_reward_signal, dim3(blocks_for(a.n, 256), B), dim3(256), 0, st, a, r);
+  hipLaunchKernelGGL(k_reward_contrib, dim3(blocks_for(a.S, 256), B), dim3(256), 0, st, a, r);
+  return hip_last("reward launch failed");
 }
 
 int train_launch_apply(const TrainArgs& a, float* w, int B, const int32_t* csc_e, float* wc,

@@ -17,6 +17,7 @@ struct TrainArgs {
   LifK lif;
   float gamma, lr, w_max;
   int train_hidden;
+  int reward;  // reward mode: C receives float(E) and the reward kernels scale it
   int encoder;
   float gain, max_rate, enc_thr;
   uint32_t seed_lo, seed_hi;
@@ -90,12 +91,27 @@ __host__ __device__ inline size_t train_const_bytes(int n, int n_inputs) {
   return (b + 15) / 16 * 16;
 }
 
+// Reward-mode state of a session (device buffers).
+struct RewardArgs {
+  const float* u;  // [B] action draws of this minibatch, in minibatch order
+  float* base;     // [1] reward baseline, carried across minibatches
+  float* delta;    // [B][classes]
+  float* Ls;       // [B][n] learning signals
+  int32_t* act;    // [B] sampled action
+  float* adv;      // [B] reward advantage
+  float decay;
+};
+
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds);
 int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
                          hipStream_t st);
 int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, hipStream_t st);
 int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int T, bool f64,
                               hipStream_t st);
+// Reward mode, after every sample of the minibatch has run: actions,
+// advantages and the baseline (one ordered scan), learning signals, and the
+// contributions C_{b,s} = (lr / T_b * L_b[post_s]) * float(E_{b,s}).
+int train_launch_reward(const TrainArgs& a, const RewardArgs& r, int B, hipStream_t st);
 int train_launch_apply(const TrainArgs& a, float* w, int B, const int32_t* csc_e, float* wc,
                        int64_t P, hipStream_t st);
 int train_launch_gather(const float* w, const int32_t* csc_e, float* wc, int64_t P,

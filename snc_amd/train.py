@@ -29,7 +29,8 @@ class TrainConfigC(C.Structure):
                 ("feedback_scale", C.c_float), ("train_hidden", C.c_int32),
                 ("readout_mode", C.c_int32), ("readout_psi", C.c_float), ("encoder", C.c_int32),
                 ("enc_gain", C.c_float), ("enc_max_rate", C.c_float),
-                ("enc_threshold", C.c_float), ("seed", C.c_uint64)]
+                ("enc_threshold", C.c_float), ("seed", C.c_uint64),
+                ("reward_mode", C.c_int32), ("reward_baseline_decay", C.c_float)]
 
 
 class TrainStatsC(C.Structure):
@@ -55,6 +56,8 @@ class TrainConfig:
     enc_max_rate: float = 0.5
     enc_threshold: float = 0.0
     seed: int = 1
+    reward_mode: bool = False           # three-factor reward rule (trainer.cpp:165-179)
+    reward_baseline_decay: float = 0.99
 
     def c(self) -> TrainConfigC:
         lif = self.lif or LIFParams()
@@ -62,7 +65,7 @@ class TrainConfig:
                             self.surrogate_scale, self.feedback_scale,
                             1 if self.train_hidden else 0, self.readout_mode, self.readout_psi,
                             self.encoder, self.enc_gain, self.enc_max_rate, self.enc_threshold,
-                            self.seed)
+                            self.seed, 1 if self.reward_mode else 0, self.reward_baseline_decay)
 
 
 @dataclass

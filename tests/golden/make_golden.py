@@ -9,7 +9,8 @@ Every fixture holds inputs and the reference's outputs only:
   encode.npz     encode() outputs for direct / poisson / latency
   synthetic.npz  make_synthetic() output
   train_*.npz    one-sample e-prop steps of snc::Trainer: weights before and
-                 after, neuron fires and synapse deliveries
+                 after, neuron fires and synapse deliveries (train_reward.npz:
+                 the three-factor reward rule, reward_mode = 1)
 """
 from __future__ import annotations
 
@@ -97,18 +98,27 @@ def main():
     write_train()
 
 
-def write_train():
+TRAIN_CASES = [("train_spiking.npz", 0.3, 0.08, 1, 0),
+               ("train_direct.npz", 0.25, 0.05, 0, 0),
+               ("train_reward.npz", 0.3, 0.5, 1, 1)]
+
+
+def write_train(names=None):
     """snc::Trainer epochs on a small static-snc graph: weights and stats."""
-    for name, w_init, lr, enc in [("train_spiking.npz", 0.3, 0.08, 1),
-                                  ("train_direct.npz", 0.25, 0.05, 0)]:
+    for name, w_init, lr, enc, reward in TRAIN_CASES:
+        if names and name not in names:
+            continue
         g = G.static_snc([64, 48, 8], 1500, 0.5, 1, 0.2, 4)
         X, y = O.ref_make_synthetic(12, 64, 8, 0.35, 3)
         cfg = O.train_config(num_steps=20, lr=lr, w_init=w_init, w_max=4.0, gamma=0.3,
-                             feedback=1.0, encoder=enc, max_rate=0.5, seed=5)
+                             feedback=1.0, encoder=enc, max_rate=0.5, seed=5,
+                             reward_mode=bool(reward), reward_baseline_decay=0.9)
         r = O.RefTrainer(g, cfg)
         out = graph_arrays(g)
         out.update(X=X, y=y, num_steps=np.int32(20), lr=np.float32(lr), w_init=np.float32(w_init),
                    encoder=np.int32(enc), seed=np.uint64(5), w0=r.weights)
+        if reward:
+            out.update(reward_mode=np.int32(1), reward_baseline_decay=np.float32(0.9))
         for ep in (1, 2):
             st = r.train_epoch(X, y, ep, 8)
             syn, neur = r.stats()
@@ -122,4 +132,7 @@ def write_train():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1:  # e.g. make_golden.py train_reward.npz: only those train fixtures
+        write_train(sys.argv[1:])
+    else:
+        main()

@@ -76,6 +76,20 @@ def test_train_driver_learns():
     assert len(acc) == 5 and acc[-1] >= 0.4 and acc[-1] > acc[0] + 0.2
 
 
+def test_train_driver_reward_mode_learns():
+    """--reward-mode 1 (train_main.cpp:119): the three-factor rule sees only
+    right/wrong per sample, so it climbs slower than the supervised delta;
+    eight epochs of this deterministic run reach about three times chance."""
+    r = _run([os.path.join(BIN, "snc_hip_train"), "--dim", "196", "--hidden", "96",
+              "--num-train", "400", "--num-test", "200", "--num-steps", "40", "--min-steps", "40",
+              "--batch", "32", "--epochs", "8", "--w-std", "0.3", "--max-rate", "0.3",
+              "--fan-in", "24", "--rec-fan-in", "16", "--reward-mode", "1"])
+    assert r.returncode == 0
+    acc = [float(l.split("test_acc=")[1].split()[0]) for l in r.stdout.splitlines()
+           if l.startswith("epoch")]
+    assert len(acc) == 9 and acc[-1] > 0.12 and acc[-1] > acc[0] + 0.05, acc
+
+
 @pytest.mark.parametrize("grow", ["heuristic", "learned"])
 def test_cotrain_driver_runs(grow):
     r = _run([os.path.join(BIN, "snc_hip_cotrain"), "--grow", grow, "--outer", "4",

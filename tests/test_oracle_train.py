@@ -24,13 +24,17 @@ def _fixture_trainer(name):
     z = dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))
     g = SNNGraph(z["row_ptr"], z["post_ids"], z["weights"], z["delays"], z["sign"], z["role"],
                  z["channel"])
+    reward = "reward_mode" in z
     cfg = O.train_config(num_steps=int(z["num_steps"]), lr=float(z["lr"]),
                          w_init=float(z["w_init"]), w_max=4.0, gamma=0.3, feedback=1.0,
-                         encoder=int(z["encoder"]), max_rate=0.5, seed=int(z["seed"]))
+                         encoder=int(z["encoder"]), max_rate=0.5, seed=int(z["seed"]),
+                         reward_mode=reward,
+                         reward_baseline_decay=float(z["reward_baseline_decay"]) if reward
+                         else 0.99)
     return g, cfg, z
 
 
-@pytest.mark.parametrize("name", ["train_spiking.npz", "train_direct.npz"])
+@pytest.mark.parametrize("name", ["train_spiking.npz", "train_direct.npz", "train_reward.npz"])
 def test_trainer_matches_reference_goldens(name):
     g, cfg, z = _fixture_trainer(name)
     t = O.Trainer(g, cfg)
@@ -61,6 +65,46 @@ def test_trainer_differential_vs_compiled_reference(seed):
         assert sa == sr
         assert np.array_equal(a.weights, r.weights)
         assert all(np.array_equal(p, q) for p, q in zip(a.stats(), r.stats()))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not present")
+@pytest.mark.parametrize("seed", [1, 2])
+def test_reward_mode_differential_vs_compiled_reference(seed):
+    """Three-factor reward rule (trainer.cpp:165-179) over three epochs: the
+    action stream and the baseline persist across epochs."""
+    from snc_amd import graph as G
+    g = G.static_snc([40, 30, 20, 5], 900, 0.5, 2, 0.2, seed)
+    X, y = O.ref_make_synthetic(12, 40, 5, 0.3, seed)
+    cfg = O.train_config(num_steps=18, lr=0.6, w_init=0.35, gamma=0.3, feedback=0.7,
+                         encoder=seed % 2, max_rate=0.6, seed=seed, reward_mode=True,
+                         reward_baseline_decay=0.7 + 0.1 * seed)
+    a, r = O.Trainer(g, cfg), O.RefTrainer(g, cfg)
+    w0 = a.weights
+    for ep in (1, 2, 3):
+        sa, sr = a.train_epoch_ref(X, y, ep), r.train_epoch(X, y, ep, 5)
+        assert sa == sr
+        assert np.array_equal(a.weights, r.weights)
+    assert not np.array_equal(a.weights, w0)
+
+
+def test_reward_minibatch_of_one_equals_reward_epoch_step():
+    """B = 1 minibatches in reward mode follow the per-sample rule of the
+    reference epoch: one-sample epochs (order trivially [0], Direct encoding,
+    so no seed enters) and B = 1 minibatches over the same samples from the
+    same start agree bit for bit -- same action draws, same baseline sequence,
+    same update."""
+    from snc_amd import graph as G
+    g = G.static_snc([30, 24, 4], 500, 0.5, 1, 0.2, 3)
+    X, y = O.ref_make_synthetic(6, 30, 4, 0.3, 5)
+    cfg = O.train_config(num_steps=16, lr=0.7, w_init=0.4, encoder=0, seed=9, reward_mode=True,
+                         reward_baseline_decay=0.5)
+    a, b = O.Trainer(g, cfg), O.Trainer(g, cfg)
+    for i in range(6):
+        a.train_epoch_ref(X[i:i + 1], y[i:i + 1], i + 1)
+        ev = O.encode(0, X[i], 16)
+        ev["sample_id"] = 0
+        b.minibatch(ev, [16], [int(y[i])])
+        assert np.array_equal(a.weights, b.weights)
 
 
 def test_eligibility_of_one_synapse_chain():

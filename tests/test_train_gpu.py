@@ -23,14 +23,16 @@ W_RTOL, W_ATOL = 1e-5, 6e-8
 
 
 def _cfgs(T, readout=READOUT_SPIKING, reset_mode=0, encoder=0, lr=0.2, w_init=0.4, seed=3,
-          decay=0.9, max_rate=0.5):
+          decay=0.9, max_rate=0.5, reward=False, reward_decay=0.99):
     lif = LIFParams(1.0, decay, 0.0, 1, reset_mode)
     gpu = TrainConfig(num_steps=T, lif=lif, lr=lr, w_init=w_init, w_max=4.0, surrogate_scale=0.3,
                       feedback_scale=1.0, readout_mode=readout, readout_psi=1.0, encoder=encoder,
-                      enc_max_rate=max_rate, seed=seed)
+                      enc_max_rate=max_rate, seed=seed, reward_mode=reward,
+                      reward_baseline_decay=reward_decay)
     orc = O.train_config(num_steps=T, lif=lif, lr=lr, w_init=w_init, w_max=4.0, gamma=0.3,
                          feedback=1.0, readout_mode=readout, readout_psi=1.0, encoder=encoder,
-                         max_rate=max_rate, seed=seed)
+                         max_rate=max_rate, seed=seed, reward_mode=reward,
+                         reward_baseline_decay=reward_decay)
     return gpu, orc
 
 
@@ -170,6 +172,45 @@ def test_device_latency_matches_oracle(mode):
     ev = _events_for(X, lengths, enc=2)
     assert ev.size == sum(int((X[b] > 0).sum()) for b in range(B))  # one spike per lit channel
     _compare(sess, orc, w0, np.arange(B), ev, lengths, y, mode)
+
+
+@pytest.mark.parametrize("mode", [BATCHED, PLAIN])
+@pytest.mark.parametrize("readout", [READOUT_SPIKING, READOUT_LEAKY])
+def test_reward_mode_matches_oracle_over_minibatches(mode, readout):
+    """Three-factor reward rule (trainer.cpp:165-179): actions drawn in
+    minibatch order from the session's action stream, the baseline carried
+    across minibatches, and a step without update drawing nothing. Each
+    minibatch starts both sides from the same weights; logits and rasters are
+    bit-exact, weight deltas within the tolerance above."""
+    g = G.static_snc([64, 48, 8], 1500, 0.5, 1, 0.2, 4)
+    B, rounds = 6, 4
+    X, y = O.ref_make_synthetic(B * rounds, 64, 8, 0.35, 7)
+    lengths = np.random.default_rng(3).integers(12, 30, B * rounds).astype(np.int32)
+    gcfg, ocfg = _cfgs(30, readout=readout, reset_mode=1, lr=0.8, w_init=0.4, reward=True,
+                       reward_decay=0.8)
+    sess = HipTrainSession(g, gcfg, batch=B, max_steps=30)
+    orc = O.Trainer(g, ocfg)
+    # the dataset keeps its own indices; the oracle takes each minibatch as samples 0..B-1
+    sess.upload_events(_events_for(X, lengths), B * rounds, y, lengths)
+    moved = 0.0
+    for r in range(rounds):
+        sl = slice(B * r, B * (r + 1))
+        ev = _events_for(X[sl], lengths[sl])
+        w0 = sess.get_weights()
+        orc.weights = w0
+        apply = r != 1  # round 1 runs without an update: no action is drawn
+        st, logits, raster = sess.step(np.arange(B * r, B * (r + 1)), mode=mode, apply=apply,
+                                       want_logits=True, want_raster=True)
+        ref = orc.minibatch(ev, lengths[sl], y[sl], apply=apply, raster=True)
+        assert np.array_equal(logits, ref["logits"])
+        Tm = int(lengths[sl].max())
+        assert np.array_equal(raster[:, :Tm], ref["raster"])
+        wg = sess.get_weights()
+        np.testing.assert_allclose(wg - w0, orc.weights - w0, rtol=W_RTOL, atol=W_ATOL)
+        if not apply:
+            assert np.array_equal(wg, w0)
+        moved = max(moved, float(np.abs(wg - w0).max()))
+    assert moved > 1e-3  # the rule did move the weights
 
 
 def test_delay_rich_recurrent_ring_outside_lds():
