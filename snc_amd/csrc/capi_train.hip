@@ -66,7 +66,7 @@ struct snc_hip_train {
   // device graph
   int32_t *row_ptr = nullptr, *post = nullptr, *src = nullptr, *cpos = nullptr, *ccol = nullptr,
           *csc_e = nullptr, *channel = nullptr, *internal_row = nullptr, *inputs = nullptr;
-  uint8_t *cdel = nullptr, *role = nullptr;
+  uint8_t *cdel = nullptr, *sdel = nullptr, *role = nullptr;
   float *w = nullptr, *wc = nullptr, *Bfb = nullptr;
   // dataset
   int n_data = 0, dim = 0;
@@ -113,6 +113,7 @@ struct snc_hip_train {
     if (stream) (void)hipStreamDestroy(stream);
     for (void* p : {(void*)row_ptr, (void*)post, (void*)src, (void*)cpos, (void*)ccol,
                     (void*)csc_e, (void*)channel, (void*)internal_row, (void*)inputs, (void*)cdel,
+                    (void*)sdel,
                     (void*)role, (void*)w, (void*)wc, (void*)Bfb, (void*)X, (void*)labels,
                     (void*)lengths, (void*)ev_base, (void*)ev_step_ptr, (void*)ev_nid,
                     (void*)ev_amp, (void*)idx, (void*)hist, (void*)C, (void*)ring_g, (void*)logits,
@@ -209,15 +210,29 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   if (P > 0x7fffff00LL) return fail(SNC_E_UNSUPPORTED, "graph too large");
   ccol[n] = (int32_t)P;
   s->P = P;
+  // Column order: delay descending, then pre, then CSR position. One step's
+  // arrivals at a target with one delay share one ring slot; with each
+  // delay's entries contiguous, every slot is one run of the column, in the
+  // (pre, CSR position) order the CPU ring adds them in.
   std::vector<int32_t> cpos(S), csc_e((size_t)P, -1), fillc(n, 0);
-  std::vector<uint8_t> cdel((size_t)P, 0), role(g->role, g->role + n);
-  for (int e = 0; e < S; ++e) {
-    const int j = g->post_ids[e];
-    const int32_t p = ccol[j] + fillc[j]++;
-    cpos[e] = p;
-    csc_e[p] = e;
-    cdel[p] = (uint8_t)g->delays[e];
+  std::vector<uint8_t> cdel((size_t)P, 0), sdel(S), role(g->role, g->role + n);
+  {
+    std::vector<int32_t> by_col(S);
+    std::iota(by_col.begin(), by_col.end(), 0);
+    std::stable_sort(by_col.begin(), by_col.end(), [&](int32_t x, int32_t y) {
+      const int jx = g->post_ids[x], jy = g->post_ids[y];
+      if (jx != jy) return jx < jy;
+      return g->delays[x] > g->delays[y];
+    });
+    for (const int32_t e : by_col) {
+      const int j = g->post_ids[e];
+      const int32_t p = ccol[j] + fillc[j]++;
+      cpos[e] = p;
+      csc_e[p] = e;
+      cdel[p] = (uint8_t)g->delays[e];
+    }
   }
+  for (int e = 0; e < S; ++e) sdel[e] = (uint8_t)g->delays[e];
   for (int i = 0; i < n; ++i)
     if (g->role[i] == SNC_ROLE_INPUT) s->h_inputs.push_back(i);
   s->h_src = src;
@@ -230,6 +245,7 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   if (!rc) rc = tupload(&s->ccol, ccol.data(), ccol.size());
   if (!rc) rc = tupload(&s->csc_e, csc_e.data(), csc_e.size());
   if (!rc) rc = tupload(&s->cdel, cdel.data(), cdel.size());
+  if (!rc) rc = tupload(&s->sdel, sdel.data(), sdel.size());
   if (!rc) rc = tupload(&s->role, role.data(), role.size());
   if (!rc) rc = tupload(&s->channel, g->channel, (size_t)n);
   if (!rc) rc = tupload(&s->internal_row, internal_row.data(), internal_row.size());
@@ -316,6 +332,8 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   a.ccol = s->ccol;
   a.cdel = s->cdel;
   a.wc = s->wc;
+  a.w_csr = s->w;
+  a.sdel = s->sdel;
   a.role = s->role;
   a.channel = s->channel;
   a.internal_row = s->internal_row;

@@ -5,10 +5,13 @@
 //   inject -> LIF + surrogate psi -> trace -> eligibility E_s += psi_post tr_pre
 //   -> delivery of fired rows with the trainer's signed weights.
 // Delivery is the ordered bucket fold of the forward path: fired rows set
-// presence bits over each target's incoming column (ordered pre, CSR
-// position), then every target adds its bits in column order, so rasters are
-// the CPU's bit for bit. Eligibility is accumulated per (sample, synapse) in
-// double (the CPU's exact sums) or float.
+// presence bits over each target's incoming column (delay descending, pre,
+// CSR position), then every target adds its bits in column order, so rasters
+// are the CPU's bit for bit. The batched kernel also stages each arrival's
+// weight and delay in LDS at its rank among the step's set bits (one
+// popcount scan of the bitmap), so the fold reads LDS instead of gathering
+// column entries from HBM. Eligibility is accumulated per (sample, synapse)
+// in double (the CPU's exact sums) or float.
 //
 // After a sample's T_b steps: delta = softmax(logits) - onehot(label)
 // (trainer.cpp:158-180), learning signal L (outputs: delta, hidden: B delta,
@@ -175,14 +178,81 @@ __device__ void sample_epilogue(const TrainArgs& a, int b, const float* logits_i
 // ---------------------------------------------------------------------------
 
 constexpr int kEligPerThread = 16;  // synapses whose eligibility a lane keeps in registers
+constexpr int kFoldBatch = 8;       // staged arrivals a fold round loads together
 
-// Eligibility and contributions of sample b after its T steps, from the psi
-// and trace histories the step loop wrote:
-//   E_s = sum_t psi_t[post_s] * tr_t[pre_s],   t ascending, one fma per step,
-// exactly the per-step accumulation of trainer.cpp:132-139 (the product of
-// two floats is exact in double, so fma == add of the exact product; steps
-// where tr_pre = 0 add +0 and change nothing). Lanes keep kEligPerThread
-// sums in registers; the histories stream through LDS in chunks of steps.
+// Ordered fold of target j's arrivals into its ring slots. Columns are
+// ordered delay descending, so one step's arrivals for one ring slot are one
+// contiguous run of the column, added in (pre, CSR position) order onto the
+// slot's previous value -- the CPU's sum bit for bit -- and each slot is read
+// and written once; the ring values of kFoldBatch arrivals are therefore
+// loaded together before their ordered adds.
+//
+// fold_staged: the step's arrivals are staged in LDS in column order (rank
+// among the set bits of the bitmap), so target j's are s_cval[r0, r1).
+__device__ __forceinline__ void fold_staged(const TrainArgs& a, int j, int r0, int r1,
+                                            const uint2* s_cval, float* ring, int n, int tm) {
+  const int L = a.L;
+  int cur = -1;
+  float acc = 0.0f;
+  for (int r = r0; r < r1; r += kFoldBatch) {
+    int sl[kFoldBatch];
+    float val[kFoldBatch], old[kFoldBatch];
+#pragma unroll
+    for (int u = 0; u < kFoldBatch; ++u) {
+      sl[u] = -1;
+      val[u] = 0.0f;
+      if (r + u < r1) {
+        const uint2 cv = s_cval[r + u];
+        val[u] = __uint_as_float(cv.x);
+        const int s = tm + (int)cv.y;
+        sl[u] = s >= L ? s - L : s;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kFoldBatch; ++u)
+      old[u] = sl[u] >= 0 && sl[u] != cur ? ring[(size_t)sl[u] * n + j] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kFoldBatch; ++u) {
+      if (sl[u] < 0) break;
+      if (sl[u] != cur) {
+        if (cur >= 0) ring[(size_t)cur * n + j] = acc;
+        cur = sl[u];
+        acc = old[u];
+      }
+      acc = __fadd_rn(acc, val[u]);
+    }
+  }
+  if (cur >= 0) ring[(size_t)cur * n + j] = acc;
+}
+
+// fold_gather: a step with more arrivals than the staging holds walks the set
+// bits of target j's column words [w0, w1) (clearing them) and gathers each
+// arrival's weight and delay from the column arrays in HBM.
+__device__ __forceinline__ void fold_gather(const TrainArgs& a, int j, int w0, int w1,
+                                            uint32_t* s_bits, float* ring, int n, int tm) {
+  const int L = a.L;
+  int cur = -1;
+  float acc = 0.0f;
+  for (int w = w0; w < w1; ++w) {
+    uint32_t bw = s_bits[w];
+    if (!bw) continue;
+    s_bits[w] = 0u;
+    while (bw) {
+      const int p = (w << 5) | __builtin_ctz(bw);
+      bw &= bw - 1u;
+      int s = tm + (int)a.cdel[p];
+      if (s >= L) s -= L;
+      if (s != cur) {
+        if (cur >= 0) ring[(size_t)cur * n + j] = acc;
+        cur = s;
+        acc = ring[(size_t)s * n + j];
+      }
+      acc = __fadd_rn(acc, a.wc[p]);
+    }
+  }
+  if (cur >= 0) ring[(size_t)cur * n + j] = acc;
+}
+
 template <typename ET>
 __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, const float* Ls,
                              float* s_chunk, int chunk_steps, int tid) {
@@ -245,7 +315,20 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   float* s_inpx = (float*)(s_inp + a.n_inputs);      // [n_inputs] drive value / probability
   uint8_t* s_role = (uint8_t*)(s_inpx + a.n_inputs); // [n] role
   float* s_ring = (float*)(smem + state_bytes + const_bytes);  // [L][n] when RING_LDS
-  __shared__ int s_nf;
+  // Fold staging, in the bytes the history chunks take after the step loop:
+  // the bitmap's exclusive popcount prefix per word, then one {weight, delay}
+  // per arrival of the step, at its rank; `cap` arrivals fit.
+  const size_t fold_base =
+      state_bytes + const_bytes + (RING_LDS ? (sizeof(float) * (size_t)L * n + 15) / 16 * 16 : 0);
+  const size_t pref_bytes = (sizeof(int32_t) * ((size_t)a.bits_words + 1) + 15) / 16 * 16;
+  int32_t* s_pref = (int32_t*)(smem + fold_base);
+  uint2* s_cval = (uint2*)(smem + fold_base + pref_bytes);
+  const int cap = (size_t)a.lds_bytes > fold_base + pref_bytes
+                      ? (int)min(((size_t)a.lds_bytes - fold_base - pref_bytes) / sizeof(uint2),
+                                 (size_t)0x7fffffff)
+                      : 0;
+  __shared__ int s_nf, s_arr;
+  __shared__ int s_wsum[kTrainThreads / kWave];
   __shared__ unsigned long long s_ev, s_sp;
 
   const int b = blockIdx.x;
@@ -292,6 +375,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   for (size_t k = tid; k < (size_t)L * n; k += kTrainThreads) ring[k] = 0.0f;
   if (tid == 0) {
     s_nf = 0;
+    s_arr = 0;
     s_ev = 0;
     s_sp = 0;
   }
@@ -385,36 +469,61 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
       }
       ev += (unsigned long long)(re - rb);
     }
-    if (lane == 0 && ev) atomicAdd(&s_ev, ev);
+    if (lane == 0 && ev) {
+      atomicAdd(&s_ev, ev);
+      atomicAdd(&s_arr, (int)ev);
+    }
     __syncthreads();
-    // (C) ordered fold into the ring; next step's injections
-    for (int j = tid; j < n; j += kTrainThreads) {
-      const int w0 = s_ccol[j] >> 5, w1 = s_ccol[j + 1] >> 5;
-      int cur = -1;
-      float acc = 0.0f;
-      for (int w = w0; w < w1; ++w) {
-        uint32_t bw = s_bits[w];
-        if (!bw) continue;
-        s_bits[w] = 0u;
-        while (bw) {
-          const int p = (w << 5) | __builtin_ctz(bw);
-          bw &= bw - 1u;
-          int s = tm + (int)a.cdel[p];
-          if (s >= L) s -= L;
-          if (s != cur) {
-            if (cur >= 0) ring[(size_t)cur * n + j] = acc;
-            cur = s;
-            acc = ring[(size_t)s * n + j];
-          }
-          acc = __fadd_rn(acc, a.wc[p]);
+    const bool staged = s_arr <= cap;  // block-uniform
+    if (staged) {
+      // (B2) exclusive popcount prefix of the bitmap words (block scan)
+      const int W = a.bits_words;
+      const int per = (W + kTrainThreads - 1) / kTrainThreads;
+      const int lo = min(W, tid * per), hi = min(W, lo + per);
+      int cnt = 0;
+      for (int w = lo; w < hi; ++w) cnt += __popc(s_bits[w]);
+      const int incl = wave_inclusive_scan(cnt);
+      if (lane == kWave - 1) s_wsum[wave] = incl;
+      __syncthreads();
+      int run = incl - cnt;
+      for (int k = 0; k < wave; ++k) run += s_wsum[k];
+      for (int w = lo; w < hi; ++w) {
+        s_pref[w] = run;
+        run += __popc(s_bits[w]);
+      }
+      if (hi == W) s_pref[W] = run;  // the total (every such thread writes the same)
+      __syncthreads();
+      // (B3) every arrival's weight and delay at its rank, from the fired
+      // rows read in CSR order (coalesced)
+      for (int k = wave; k < nf; k += kTrainThreads / kWave) {
+        const int i = s_fired[k];
+        const int rb = s_rowp[i], re = s_rowp[i + 1];
+        for (int e = rb + lane; e < re; e += kWave) {
+          const int p = a.cpos[e];
+          const float wv = a.w_csr[e];
+          const uint32_t d = a.sdel[e];
+          const int q = p >> 5;
+          const int rk = s_pref[q] + __popc(s_bits[q] & ((1u << (p & 31)) - 1u));
+          s_cval[rk] = make_uint2(__float_as_uint(wv), d);
         }
       }
-      if (cur >= 0) ring[(size_t)cur * n + j] = acc;
+      __syncthreads();
+    }
+    // (C) ordered fold into the ring (staged: the bitmap is only cleared);
+    // next step's injections
+    if (staged) {
+      for (int j = tid; j < n; j += kTrainThreads)
+        fold_staged(a, j, s_pref[s_ccol[j] >> 5], s_pref[s_ccol[j + 1] >> 5], s_cval, ring, n, tm);
+      for (int w = tid; w < a.bits_words; w += kTrainThreads) s_bits[w] = 0u;
+    } else {
+      for (int j = tid; j < n; j += kTrainThreads)
+        fold_gather(a, j, s_ccol[j] >> 5, s_ccol[j + 1] >> 5, s_bits, ring, n, tm);
     }
     if (t + 1 < T) inject(t + 1);
     if (tid == 0) {
       s_sp += (unsigned long long)nf;
       s_nf = 0;
+      s_arr = 0;
     }
     tm = tm + 1 == L ? 0 : tm + 1;
     __syncthreads();
@@ -684,6 +793,12 @@ int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, siz
   const size_t step_bytes = sizeof(float) * 2 * (size_t)a.hn;
   size_t lds = std::max(train_lds_bytes(a, ring_lds), state + step_bytes);
   lds = std::max(lds, std::min(dyn_max, state + kChunkTarget * step_bytes));
+  // Room to stage every arrival of a step when all synapses deliver at once
+  // (the kernel falls back to HBM gathers for a step that does not fit).
+  const size_t fold_full = (train_lds_bytes(a, ring_lds) + 15) / 16 * 16 +
+                           (sizeof(int32_t) * ((size_t)a.bits_words + 1) + 15) / 16 * 16 +
+                           sizeof(uint2) * (size_t)a.S;
+  lds = std::max(lds, std::min(dyn_max, fold_full));
   if (lds > dyn_max) return fail(SNC_E_UNSUPPORTED, "batched trainer state exceeds LDS");
   const hipError_t ea = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (ea != hipSuccess) {

@@ -31,6 +31,8 @@ struct TrainArgs {
   const int32_t* ccol;
   const uint8_t* cdel;
   const float* wc;     // trainer weights in padded column order
+  const float* w_csr;  // trainer weights, CSR order
+  const uint8_t* sdel; // delays, CSR order
   const uint8_t* role;
   const int32_t* channel;
   const int32_t* internal_row;
