@@ -101,6 +101,11 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
  * eligible prune, 0 pruned by it, -1 not yet decided). */
 int64_t snc_conn_grow_log_size(const snc_connectome* c);
 int snc_conn_grow_log(const snc_connectome* c, float* features, int32_t* labels, int64_t n);
+/* Graded outcome of each grow-log entry (no reference counterpart; a
+ * training target for the learned proposal): the percentile of the edge's
+ * |w| among all prune-eligible edges at its first eligible prune (fraction
+ * below it, ties counted half), -1 while undecided. */
+int snc_conn_grow_log_rank(const snc_connectome* c, float* rank, int64_t n);
 
 /* The 16 features of candidate synapses (pre, post) under the given
  * statistics, as the learned grow step computes them. */

@@ -58,6 +58,7 @@ struct snc_connectome {
   std::mt19937_64 explore_rng;  // exploration draws of logging runs only
   std::vector<float> log_feat;
   std::vector<int32_t> log_label;
+  std::vector<float> log_rank;  // |w| percentile among the eligible edges at that prune
 
   bool connected(int pre, int post) const {
     for (const Edge& e : adj[pre])
@@ -452,11 +453,23 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
                        [](const Cand& a, const Cand& b) { return a.score < b.score; });
     for (int j = 0; j < K; ++j) rm[cand[j].i][cand[j].k] = 1;
   }
-  // Grow-log labels: an edge decides its outcome at its first eligible prune.
+  // Grow-log labels: an edge decides its outcome at its first eligible prune:
+  // whether it survived, and where its |w| ranks among the eligible edges
+  // (fraction below it, ties counted half).
+  std::vector<float> sorted_scores;
   for (const Cand& x : cand) {
     const Edge& e = c->adj[x.i][x.k];
-    if (e.log_id >= 0 && c->log_label[e.log_id] < 0)
-      c->log_label[e.log_id] = rm[x.i][x.k] ? 0 : 1;
+    if (e.log_id < 0 || c->log_label[e.log_id] >= 0) continue;
+    c->log_label[e.log_id] = rm[x.i][x.k] ? 0 : 1;
+    if (sorted_scores.empty()) {
+      sorted_scores.reserve(cand.size());
+      for (const Cand& y : cand) sorted_scores.push_back(y.score);
+      std::sort(sorted_scores.begin(), sorted_scores.end());
+    }
+    const auto lo = std::lower_bound(sorted_scores.begin(), sorted_scores.end(), x.score);
+    const auto hi = std::upper_bound(lo, sorted_scores.end(), x.score);
+    const double below = (double)(lo - sorted_scores.begin()) + 0.5 * (double)(hi - lo);
+    c->log_rank[e.log_id] = (float)(below / (double)sorted_scores.size());
   }
   int pruned = 0;
   std::vector<int> freed;
@@ -521,6 +534,7 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
     const int64_t id = (int64_t)c->log_label.size();
     c->log_feat.insert(c->log_feat.end(), f, f + kFeat);
     c->log_label.push_back(-1);
+    c->log_rank.push_back(-1.0f);
     (void)q;
     (void)p;
     return id;
@@ -642,6 +656,12 @@ int snc_conn_grow_log(const snc_connectome* c, float* features, int32_t* labels,
   if (!c || n != (int64_t)c->log_label.size()) return fail(SNC_E_INVALID, "log size mismatch");
   if (features) std::copy(c->log_feat.begin(), c->log_feat.end(), features);
   if (labels) std::copy(c->log_label.begin(), c->log_label.end(), labels);
+  return SNC_OK;
+}
+
+int snc_conn_grow_log_rank(const snc_connectome* c, float* rank, int64_t n) {
+  if (!c || !rank || n != (int64_t)c->log_rank.size()) return fail(SNC_E_INVALID, "log size mismatch");
+  std::copy(c->log_rank.begin(), c->log_rank.end(), rank);
   return SNC_OK;
 }
 

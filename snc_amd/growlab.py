@@ -79,13 +79,15 @@ def run_cotrain(seed, mode, w, cfg, model=None, log=False, explore=0.0, outer=12
     return con, rounds
 
 
-def collect_log(seeds, w, cfg, explore=0.5, outer=12, kind="layered"):
-    """Labelled grow events (features, survived) from heuristic runs."""
+def collect_log(seeds, w, cfg, explore=0.5, outer=12, kind="layered", label="survival"):
+    """Labelled grow events from heuristic runs: (features, survived) or, with
+    label="rank", (features, |w| percentile at the edge's first eligible
+    prune) -- a graded target instead of a mostly-1 binary one."""
     Xs, ys = [], []
     for s in seeds:
         con, _ = run_cotrain(s, GROW_HEURISTIC, w, cfg, log=True, explore=explore, outer=outer,
                              kind=kind)
-        X, y = con.grow_log()
+        X, y = con.grow_log(rank=label == "rank")
         keep = y >= 0
         Xs.append(X[keep])
         ys.append(y[keep])
@@ -93,20 +95,21 @@ def collect_log(seeds, w, cfg, explore=0.5, outer=12, kind="layered"):
 
 
 def ab_compare(seeds=(1, 2, 3, 4, 5), train_seeds=(101, 102, 103), outer=12, mlp_epochs=40,
-               explore=0.5, log=print, kind="layered"):
+               explore=0.5, log=print, kind="layered", label="survival"):
     """Learned vs heuristic grow at equal budget. Returns a dict with the
     per-seed final test accuracies and synapse counts, the MLP's loss before
     and after training, and its scoring rate on one million candidates."""
     t0 = time.time()
     w, cfg = workload()
-    Xl, yl = collect_log(train_seeds, w, cfg, explore=explore, outer=outer, kind=kind)
-    log(f"grow log: {Xl.shape[0]} labelled events, survival rate {yl.mean():.3f} "
+    Xl, yl = collect_log(train_seeds, w, cfg, explore=explore, outer=outer, kind=kind,
+                         label=label)
+    log(f"grow log: {Xl.shape[0]} labelled events, mean {label} label {yl.mean():.3f} "
         f"({time.time() - t0:.1f}s)")
     mlp = GrowMLP(seed=11)
     loss0, _ = mlp.grad(Xl, yl)
     loss = mlp.train(Xl, yl, epochs=mlp_epochs, batch=256, lr=2e-3, seed=5)
     log(f"MLP BCE {loss0:.4f} -> {loss:.4f}")
-    res = {"log_examples": int(Xl.shape[0]), "log_survival_rate": float(yl.mean()),
+    res = {"label": label, "log_examples": int(Xl.shape[0]), "log_label_mean": float(yl.mean()),
            "mlp_loss_before": float(loss0), "mlp_loss_after": float(loss),
            "heuristic": [], "learned": [], "synapses": [], "candidates_scored_per_step": []}
     for s in seeds:
@@ -144,8 +147,10 @@ if __name__ == "__main__":
     ap.add_argument("--mlp-epochs", type=int, default=40)
     ap.add_argument("--explore", type=float, default=0.5)
     ap.add_argument("--kind", choices=["layered", "sheet"], default="layered")
+    ap.add_argument("--label", choices=["survival", "rank"], default="survival")
     a = ap.parse_args()
     out = ab_compare(tuple(int(x) for x in a.seeds.split(",")),
                      tuple(int(x) for x in a.train_seeds.split(",")), a.outer, a.mlp_epochs,
-                     a.explore, log=lambda m: print(m, file=sys.stderr, flush=True), kind=a.kind)
+                     a.explore, log=lambda m: print(m, file=sys.stderr, flush=True), kind=a.kind,
+                     label=a.label)
     print(json.dumps(out))

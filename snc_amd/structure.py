@@ -71,6 +71,7 @@ _SIGS = {
     "snc_conn_grow_log_size": (C.c_int64, [C.c_void_p]),
     "snc_conn_grow_log": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int32),
                                     C.c_int64]),
+    "snc_conn_grow_log_rank": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
     "snc_conn_candidate_features": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32),
                                               C.POINTER(C.c_int32), C.c_int64,
                                               C.POINTER(C.c_int64), C.c_int64, C.c_double,
@@ -133,11 +134,18 @@ class Connectome:
             1 if log else 0, C.byref(rep)))
         return {k: getattr(rep, k) for k, _ in rep._fields_}
 
-    def grow_log(self):
+    def grow_log(self, rank=False):
+        """(features, labels): labels 1/0 survived/pruned at the first eligible
+        prune (-1 undecided); with rank=True the |w| percentile at that prune
+        instead (-1 undecided)."""
         n = int(L.lib().snc_conn_grow_log_size(self._h))
         X = np.zeros((n, 16), np.float32)
         y = np.zeros(n, np.int32)
         L.check(L.lib().snc_conn_grow_log(self._h, L.ptr(X, C.c_float), L.ptr(y, C.c_int32), n))
+        if rank:
+            r = np.zeros(n, np.float32)
+            L.check(L.lib().snc_conn_grow_log_rank(self._h, L.ptr(r, C.c_float), n))
+            return X, r
         return X, y
 
     def candidate_features(self, pre, post, neuron_fires, activity_norm=1.0, delay=1):

@@ -64,6 +64,37 @@ def test_grow_log_labels_follow_first_eligible_prune():
     assert np.all(X[:, 15] == 1.0)  # bias feature
 
 
+def test_grow_log_rank_is_the_abs_weight_percentile_at_the_deciding_prune():
+    """With protect_rounds = 0 every edge is eligible at every prune, so an
+    edge grown (and logged) at update 0 is decided at update 1 among all N
+    edges. With distinct |w| its rank is exactly (k + 0.5) / N for its
+    position k in |w| order, and it was pruned iff k < K."""
+    con = Connectome([50, 30, 6], 300, 1, 0.02, 6)
+    rng = np.random.default_rng(6)
+    K = 20
+    cfg = StructConfig(grow_per_epoch=K, protect_rounds=0)
+    n0 = con.num_synapses
+    con.set_edge_weights((rng.standard_normal(n0) * 0.3).astype(np.float32))
+    con.structural_update(np.zeros(n0, np.int64), rng.integers(0, 30, 86).astype(np.int64), cfg,
+                          log=True)
+    N = con.num_synapses
+    w1 = rng.permutation(N).astype(np.float32) * np.float32(0.001) + np.float32(0.0005)
+    w1 *= np.where(rng.random(N) < 0.5, -1, 1).astype(np.float32)  # distinct |w|, mixed signs
+    con.set_edge_weights(w1)
+    con.structural_update(np.zeros(N, np.int64), rng.integers(0, 30, 86).astype(np.int64), cfg,
+                          log=False)
+    _, y = con.grow_log()
+    _, r = con.grow_log(rank=True)
+    assert y.size == r.size > 0 and np.all(y >= 0)
+    k = r.astype(np.float64) * N - 0.5
+    assert np.allclose(k, np.round(k), atol=1e-3)
+    k = np.round(k).astype(np.int64)
+    assert np.all((k >= 0) & (k < N))
+    assert np.array_equal(y == 0, k < K)
+    # grown edges start at w_grow_init, but w1 replaced every weight: ranks spread
+    assert len(np.unique(k)) == k.size
+
+
 def test_candidate_features_are_finite_and_local():
     con = Connectome([50, 30, 6], 300, 1, 0.02, 4)
     fires = np.arange(86, dtype=np.int64)
