@@ -355,7 +355,7 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   s->lds_max = lds_max;
   // Leave room for the kernel's few bytes of static LDS.
   const size_t lds_budget = lds_max > 256 ? lds_max - 256 : 0;
-  if (train_lds_bytes(a, false) + sizeof(float) * 2 * (size_t)hn > lds_budget)
+  if (std::max(train_lds_bytes(a, false), train_elig_lds_bytes(n, 4)) > lds_budget)
     return fail(SNC_E_UNSUPPORTED, "per-sample state does not fit in LDS");
   s->ring_lds = train_lds_bytes(a, true) <= lds_budget;
   if (!s->ring_lds) {

@@ -253,34 +253,66 @@ __device__ __forceinline__ void fold_gather(const TrainArgs& a, int j, int w0, i
   if (cur >= 0) ring[(size_t)cur * n + j] = acc;
 }
 
-template <typename ET>
+// Eligibility and contributions of sample b after its T steps, from the psi
+// and trace histories the step loop wrote:
+//   E_s = sum_t psi_t[post_s] * tr_t[pre_s],   t ascending, one fma per step,
+// exactly the per-step accumulation of trainer.cpp:132-139 (the product of
+// two floats is exact in double, so fma == add of the exact product). Lanes
+// keep kEligPerThread sums in registers. The histories stream through LDS in
+// chunks of CH steps, transposed: row j holds CH consecutive steps of neuron
+// j's psi (and, in a second array, its trace), so one 16-byte load at a
+// constant offset gives a lane four steps of one neuron. Steps past T in the
+// last chunk are zero and add +0 to a nonnegative sum, changing nothing.
+template <typename ET, int CH>
 __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, const float* Ls,
-                             float* s_chunk, int chunk_steps, int tid) {
-  const int hn = a.hn;
+                             float* s_chunk, int tid) {
+  static_assert(CH % 4 == 0, "chunk rows are read four steps at a time");
+  const int hn = a.hn, n = a.n;
+  float* ps_t = s_chunk;                   // [n][CH]
+  float* tr_t = s_chunk + (size_t)n * CH;  // [n][CH]
   const float* hb = a.hist + (size_t)b * a.T_max * 2 * hn;
   for (int s0 = 0; s0 < a.S; s0 += kEligPerThread * kTrainThreads) {
-    int pre[kEligPerThread], post[kEligPerThread];
+    int pro[kEligPerThread], tro[kEligPerThread];  // row offsets of post and pre
     ET E[kEligPerThread];
 #pragma unroll
     for (int k = 0; k < kEligPerThread; ++k) {
       const int sy = s0 + k * kTrainThreads + tid;
-      pre[k] = sy < a.S ? a.src[sy] : 0;
-      post[k] = sy < a.S ? a.post[sy] : 0;
+      pro[k] = (sy < a.S ? a.post[sy] : 0) * CH;
+      tro[k] = (sy < a.S ? a.src[sy] : 0) * CH;
       E[k] = (ET)0;
     }
-    for (int t0 = 0; t0 < T; t0 += chunk_steps) {
-      const int tc = min(chunk_steps, T - t0);
+    for (int t0 = 0; t0 < T; t0 += CH) {
+      const int tc = min(CH, T - t0);
       __syncthreads();  // previous chunk consumed
-      const float4* src4 = (const float4*)(hb + (size_t)t0 * 2 * hn);
-      float4* dst4 = (float4*)s_chunk;
-      const int quads = tc * 2 * hn / 4;  // hn is a multiple of 4
-      for (int q = tid; q < quads; q += kTrainThreads) dst4[q] = src4[q];
-      __syncthreads();
-      for (int t = 0; t < tc; ++t) {
-        const float* ps = s_chunk + (size_t)t * 2 * hn;
-        const float* tr = ps + hn;
+      // transpose in groups of 4 steps: 8 coalesced loads in flight per lane
+      for (int c = 0; c < CH; c += 4) {
+        for (int j = tid; j < n; j += kTrainThreads) {
+          float pv[4], tv[4];
 #pragma unroll
-        for (int k = 0; k < kEligPerThread; ++k) elig_add<ET>(E, k, ps[post[k]], tr[pre[k]]);
+          for (int u = 0; u < 4; ++u) {
+            const float* row = hb + (size_t)(t0 + c + u) * 2 * hn;
+            pv[u] = c + u < tc ? row[j] : 0.0f;
+            tv[u] = c + u < tc ? row[hn + j] : 0.0f;
+          }
+          *(float4*)(ps_t + (size_t)j * CH + c) = make_float4(pv[0], pv[1], pv[2], pv[3]);
+          *(float4*)(tr_t + (size_t)j * CH + c) = make_float4(tv[0], tv[1], tv[2], tv[3]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kEligPerThread; ++k) {
+        const float4* pp = (const float4*)(ps_t + pro[k]);
+        const float4* rr = (const float4*)(tr_t + tro[k]);
+        ET e[1] = {E[k]};
+#pragma unroll 2
+        for (int q = 0; q < CH / 4; ++q) {
+          const float4 p = pp[q], r = rr[q];
+          elig_add<ET>(e, 0, p.x, r.x);
+          elig_add<ET>(e, 0, p.y, r.y);
+          elig_add<ET>(e, 0, p.z, r.z);
+          elig_add<ET>(e, 0, p.w, r.w);
+        }
+        E[k] = e[0];
       }
     }
 #pragma unroll
@@ -288,9 +320,17 @@ __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, cons
       const int sy = s0 + k * kTrainThreads + tid;
       if (sy < a.S && a.C)
         a.C[(size_t)b * a.S + sy] =
-            a.reward ? (float)E[k] : __fmul_rn(__fmul_rn(scale, Ls[post[k]]), (float)E[k]);
+            a.reward ? (float)E[k]
+                     : __fmul_rn(__fmul_rn(scale, Ls[pro[k] / CH]), (float)E[k]);
     }
   }
+}
+
+// Largest chunk length (steps) whose two transposed arrays fit in `bytes`.
+__host__ __device__ inline int elig_chunk_steps(int n, size_t bytes) {
+  const size_t per_step = sizeof(float) * 2 * (size_t)n;
+  const int fit = per_step ? (int)(bytes / per_step) : 0;
+  return fit >= 28 ? 28 : fit >= 20 ? 20 : fit >= 12 ? 12 : fit >= 8 ? 8 : fit >= 4 ? 4 : 0;
 }
 
 template <typename ET, bool RING_LDS>
@@ -538,9 +578,15 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   sample_epilogue(a, b, s_logit, Ls, s_delta, label, n, tid, kTrainThreads);
   if (a.C) {
     const float scale = __fdiv_rn(a.lr, (float)max(1, T));
-    const int chunk_steps = (int)(((size_t)a.lds_bytes - state_bytes - const_bytes) /
-                                  (sizeof(float) * 2 * (size_t)a.hn));
-    elig_contrib<ET>(a, b, T, scale, Ls, s_ring, max(1, chunk_steps), tid);
+    const size_t ls_bytes = (sizeof(float) * (size_t)n + 15) / 16 * 16;
+    float* s_chunk = (float*)(smem + ls_bytes);
+    switch (elig_chunk_steps(n, (size_t)a.lds_bytes - ls_bytes)) {  // the host checked >= 4
+      case 28: elig_contrib<ET, 28>(a, b, T, scale, Ls, s_chunk, tid); break;
+      case 20: elig_contrib<ET, 20>(a, b, T, scale, Ls, s_chunk, tid); break;
+      case 12: elig_contrib<ET, 12>(a, b, T, scale, Ls, s_chunk, tid); break;
+      case 8: elig_contrib<ET, 8>(a, b, T, scale, Ls, s_chunk, tid); break;
+      default: elig_contrib<ET, 4>(a, b, T, scale, Ls, s_chunk, tid); break;
+    }
   }
   if (tid == 0) {
     a.blk_spikes[b] = s_sp;
@@ -775,9 +821,6 @@ size_t train_lds_bytes(const TrainArgs& a, bool ring_lds) {
 
 int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
                          hipStream_t st) {
-  // State (+ ring), then room for history chunks of up to kChunkTarget steps
-  // in the update phase; never below one step.
-  constexpr size_t kChunkTarget = 32;
   const void* fn = f64 ? (ring_lds ? (const void*)k_train_batched<double, true>
                                    : (const void*)k_train_batched<double, false>)
                        : (ring_lds ? (const void*)k_train_batched<float, true>
@@ -788,11 +831,10 @@ int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, siz
   if (eg != hipSuccess) return hip_fail(eg, "hipFuncGetAttributes of the batched trainer");
   if (attr.sharedSizeBytes >= lds_max) return fail(SNC_E_UNSUPPORTED, "no dynamic LDS left");
   const size_t dyn_max = lds_max - attr.sharedSizeBytes;
-  const size_t state = train_state_bytes(a.n, a.words, a.bits_words, a.classes) +
-                       train_const_bytes(a.n, a.n_inputs);
-  const size_t step_bytes = sizeof(float) * 2 * (size_t)a.hn;
-  size_t lds = std::max(train_lds_bytes(a, ring_lds), state + step_bytes);
-  lds = std::max(lds, std::min(dyn_max, state + kChunkTarget * step_bytes));
+  // The step loop's state (+ ring); after it, the learning signal and two
+  // transposed history chunks of at least 4 and ideally 28 steps.
+  size_t lds = std::max(train_lds_bytes(a, ring_lds), train_elig_lds_bytes(a.n, 4));
+  lds = std::max(lds, std::min(dyn_max, train_elig_lds_bytes(a.n, 28)));
   // Room to stage every arrival of a step when all synapses deliver at once
   // (the kernel falls back to HBM gathers for a step that does not fit).
   const size_t fold_full = (train_lds_bytes(a, ring_lds) + 15) / 16 * 16 +

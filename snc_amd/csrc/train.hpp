@@ -104,6 +104,12 @@ struct RewardArgs {
   float decay;
 };
 
+// LDS of the update phase: the learning signal (n floats, rounded up to 16
+// bytes), then the transposed psi and trace chunks of `steps` steps.
+__host__ __device__ inline size_t train_elig_lds_bytes(int n, int steps) {
+  return (sizeof(float) * (size_t)n + 15) / 16 * 16 + sizeof(float) * 2 * (size_t)n * steps;
+}
+
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds);
 int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
                          hipStream_t st);
