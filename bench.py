@@ -12,6 +12,9 @@ Prints one JSON line (rank 0). Extra objects:
   cpu_baseline  the reference's own CPU forward (oracle/_ref, compiled from
                 the reference sources) or the oracle restatement, timed on
                 this host on the same workload
+  eprop         batched vs plain e-prop on the two training workloads, and
+                (eprop.*.cpu_baseline) the oracle's trainer on one core over
+                one minibatch of the same workload
 
 Byte model (algorithmic, per step): 16 B per neuron (membrane read+write,
 ring slot read+clear), 9 B per synaptic event (post id 4, weight 4, delay 1),
@@ -174,7 +177,7 @@ def _train_session(w, elig):
     return s
 
 
-def eprop_bench(w, epochs, plain_batches, elig):
+def eprop_bench(w, epochs, plain_batches, elig, cpu_samples=0):
     """Batched persistent e-prop against the plain per-sample path on one
     workload. Both start from the same weights; the plain path runs the first
     `plain_batches` minibatches of epoch 1 and its weights are compared with
@@ -274,7 +277,49 @@ def eprop_bench(w, epochs, plain_batches, elig):
             f"(x{bsps / psps:.1f}), identical={identical}, acc {acc0:.3f} -> {curve}")
     else:
         log(f"e-prop {w.name}: batched {bsps:.1f} samples/s, acc {acc0:.3f} -> {curve}")
+    if cpu_samples > 0:
+        out["cpu_baseline"] = eprop_cpu_baseline(w, cpu_samples)
+        out["speedup_batched_vs_cpu"] = bsps / out["cpu_baseline"]["value"]
     return out
+
+
+def eprop_cpu_baseline(w, n_samples):
+    """The oracle's e-prop trainer on one core: one minibatch step over the
+    first n_samples training samples of the same workload (same graph, start
+    weights, per-sample lengths, readout and reset; feature inputs as the
+    device's Philox Poisson spikes, turned into event lists), timed."""
+    import numpy as np
+    import oracle as O
+
+    c = w.cfg
+    ocfg = O.train_config(num_steps=c.num_steps, lif=c.lif, lr=c.lr, w_init=c.w_init,
+                          w_max=c.w_max, gamma=c.surrogate_scale, feedback=c.feedback_scale,
+                          train_hidden=c.train_hidden, readout_mode=c.readout_mode,
+                          readout_psi=c.readout_psi, encoder=c.encoder, gain=c.enc_gain,
+                          max_rate=c.enc_max_rate, enc_threshold=c.enc_threshold, seed=c.seed)
+    tr = O.Trainer(w.graph, ocfg)
+    if w.init_weights is not None:
+        tr.weights = w.init_weights
+    lengths = np.ascontiguousarray(w.lengths[:n_samples], np.int32)
+    if w.kind == "features":
+        evs = []
+        for i in range(n_samples):
+            e = O.encode_philox(w.X[i], int(lengths[i]), gain=c.enc_gain, max_rate=c.enc_max_rate,
+                                threshold=c.enc_threshold, seed=c.seed, sample=i, epoch=1)
+            e["sample_id"] = i
+            evs.append(e)
+        ev = np.concatenate(evs)
+    else:
+        ev = w.events[w.events["sample_id"] < n_samples]
+    t0 = time.perf_counter()
+    tr.minibatch(ev, lengths, w.labels[:n_samples], apply=True)
+    dt = time.perf_counter() - t0
+    steps = int(lengths.sum())
+    log(f"e-prop cpu baseline {w.name}: {n_samples} samples in {dt:.2f} s")
+    return {"value": n_samples / dt, "unit": "samples/s", "cores": 1, "kind": "port",
+            "sample": f"one minibatch step of the oracle trainer over the first {n_samples} "
+                      f"training samples ({steps} sample-steps), eligibility in double",
+            "seconds": dt}
 
 
 def batched_forward(w, hg, B, steps, warmup, variant):
@@ -383,8 +428,11 @@ def main():
     def eprop_all():
         res = {}
         t = time.time()
-        res["c2"] = eprop_bench(K.c2(), args.eprop_epochs, args.plain_batches, 1)
-        res["c3"] = eprop_bench(K.c3(), max(3, args.eprop_epochs), args.plain_batches, 1)
+        # CPU leg: one minibatch of each workload (256 and 128 samples)
+        res["c2"] = eprop_bench(K.c2(), args.eprop_epochs, args.plain_batches, 1,
+                                0 if args.no_cpu else 256)
+        res["c3"] = eprop_bench(K.c3(), max(3, args.eprop_epochs), args.plain_batches, 1,
+                                0 if args.no_cpu else 128)
         log(f"e-prop measurements: {time.time() - t:.1f}s")
         return res
 
