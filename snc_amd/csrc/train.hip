@@ -360,12 +360,13 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   // per arrival of the step, at its rank; `cap` arrivals fit.
   const size_t fold_base =
       state_bytes + const_bytes + (RING_LDS ? (sizeof(float) * (size_t)L * n + 15) / 16 * 16 : 0);
-  const size_t pref_bytes = (sizeof(int32_t) * ((size_t)a.bits_words + 1) + 15) / 16 * 16;
-  int32_t* s_pref = (int32_t*)(smem + fold_base);
+  // Word prefixes are 16-bit: a staged step has at most 65535 arrivals.
+  const size_t pref_bytes = (sizeof(uint16_t) * ((size_t)a.bits_words + 1) + 15) / 16 * 16;
+  uint16_t* s_pref = (uint16_t*)(smem + fold_base);
   uint2* s_cval = (uint2*)(smem + fold_base + pref_bytes);
   const int cap = (size_t)a.lds_bytes > fold_base + pref_bytes
                       ? (int)min(((size_t)a.lds_bytes - fold_base - pref_bytes) / sizeof(uint2),
-                                 (size_t)0x7fffffff)
+                                 (size_t)65535)
                       : 0;
   __shared__ int s_nf, s_arr;
   __shared__ int s_wsum[kTrainThreads / kWave];
@@ -528,10 +529,10 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
       int run = incl - cnt;
       for (int k = 0; k < wave; ++k) run += s_wsum[k];
       for (int w = lo; w < hi; ++w) {
-        s_pref[w] = run;
+        s_pref[w] = (uint16_t)run;
         run += __popc(s_bits[w]);
       }
-      if (hi == W) s_pref[W] = run;  // the total (every such thread writes the same)
+      if (hi == W) s_pref[W] = (uint16_t)run;  // the total (every such thread writes the same)
       __syncthreads();
       // (B3) every arrival's weight and delay at its rank, from the fired
       // rows read in CSR order (coalesced)
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
           const float wv = a.w_csr[e];
           const uint32_t d = a.sdel[e];
           const int q = p >> 5;
-          const int rk = s_pref[q] + __popc(s_bits[q] & ((1u << (p & 31)) - 1u));
+          const int rk = (int)s_pref[q] + __popc(s_bits[q] & ((1u << (p & 31)) - 1u));
           s_cval[rk] = make_uint2(__float_as_uint(wv), d);
         }
       }
@@ -553,7 +554,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     // next step's injections
     if (staged) {
       for (int j = tid; j < n; j += kTrainThreads)
-        fold_staged(a, j, s_pref[s_ccol[j] >> 5], s_pref[s_ccol[j + 1] >> 5], s_cval, ring, n, tm);
+        fold_staged(a, j, (int)s_pref[s_ccol[j] >> 5], (int)s_pref[s_ccol[j + 1] >> 5], s_cval,
+                    ring, n, tm);
       for (int w = tid; w < a.bits_words; w += kTrainThreads) s_bits[w] = 0u;
     } else {
       for (int j = tid; j < n; j += kTrainThreads)
@@ -838,8 +840,8 @@ int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, siz
   // Room to stage every arrival of a step when all synapses deliver at once
   // (the kernel falls back to HBM gathers for a step that does not fit).
   const size_t fold_full = (train_lds_bytes(a, ring_lds) + 15) / 16 * 16 +
-                           (sizeof(int32_t) * ((size_t)a.bits_words + 1) + 15) / 16 * 16 +
-                           sizeof(uint2) * (size_t)a.S;
+                           (sizeof(uint16_t) * ((size_t)a.bits_words + 1) + 15) / 16 * 16 +
+                           sizeof(uint2) * std::min<size_t>((size_t)a.S, 65535);
   lds = std::max(lds, std::min(dyn_max, fold_full));
   if (lds > dyn_max) return fail(SNC_E_UNSUPPORTED, "batched trainer state exceeds LDS");
   const hipError_t ea = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
