@@ -79,14 +79,17 @@ def run_cotrain(seed, mode, w, cfg, model=None, log=False, explore=0.0, outer=12
     return con, rounds
 
 
-def collect_log(seeds, w, cfg, explore=0.5, outer=12, kind="layered", label="survival"):
-    """Labelled grow events from heuristic runs: (features, survived) or, with
+def collect_log(seeds, w, cfg, explore=0.5, outer=12, kind="layered", label="survival",
+                model=None):
+    """Labelled grow events from heuristic runs (or, with a model, from
+    learned-mode runs that use it): (features, survived) or, with
     label="rank", (features, |w| percentile at the edge's first eligible
     prune) -- a graded target instead of a mostly-1 binary one."""
     Xs, ys = [], []
     for s in seeds:
-        con, _ = run_cotrain(s, GROW_HEURISTIC, w, cfg, log=True, explore=explore, outer=outer,
-                             kind=kind)
+        mode = GROW_HEURISTIC if model is None else GROW_LEARNED
+        con, _ = run_cotrain(s, mode, w, cfg, model=model, log=True, explore=explore,
+                             outer=outer, kind=kind)
         X, y = con.grow_log(rank=label == "rank")
         keep = y >= 0
         Xs.append(X[keep])
@@ -95,7 +98,7 @@ def collect_log(seeds, w, cfg, explore=0.5, outer=12, kind="layered", label="sur
 
 
 def ab_compare(seeds=(1, 2, 3, 4, 5), train_seeds=(101, 102, 103), outer=12, mlp_epochs=40,
-               explore=0.5, log=print, kind="layered", label="survival"):
+               explore=0.5, log=print, kind="layered", label="survival", onpolicy=0):
     """Learned vs heuristic grow at equal budget. Returns a dict with the
     per-seed final test accuracies and synapse counts, the MLP's loss before
     and after training, and its scoring rate on one million candidates."""
@@ -109,7 +112,17 @@ def ab_compare(seeds=(1, 2, 3, 4, 5), train_seeds=(101, 102, 103), outer=12, mlp
     loss0, _ = mlp.grad(Xl, yl)
     loss = mlp.train(Xl, yl, epochs=mlp_epochs, batch=256, lr=2e-3, seed=5)
     log(f"MLP BCE {loss0:.4f} -> {loss:.4f}")
-    res = {"label": label, "log_examples": int(Xl.shape[0]), "log_label_mean": float(yl.mean()),
+    # On-policy rounds: log the outcomes of the model's own choices (learned
+    # mode, same exploration) and retrain on everything logged so far.
+    for k in range(onpolicy):
+        Xo, yo = collect_log(train_seeds, w, cfg, explore=explore, outer=outer, kind=kind,
+                             label=label, model=mlp)
+        Xl, yl = np.concatenate([Xl, Xo]), np.concatenate([yl, yo])
+        loss = mlp.train(Xl, yl, epochs=mlp_epochs, batch=256, lr=2e-3, seed=6 + k)
+        log(f"on-policy round {k + 1}: +{Xo.shape[0]} events (mean label {yo.mean():.3f}), "
+            f"MLP BCE {loss:.4f}")
+    res = {"label": label, "onpolicy_rounds": onpolicy, "log_examples": int(Xl.shape[0]),
+           "log_label_mean": float(yl.mean()),
            "mlp_loss_before": float(loss0), "mlp_loss_after": float(loss),
            "heuristic": [], "learned": [], "synapses": [], "candidates_scored_per_step": []}
     for s in seeds:
@@ -148,9 +161,11 @@ if __name__ == "__main__":
     ap.add_argument("--explore", type=float, default=0.5)
     ap.add_argument("--kind", choices=["layered", "sheet"], default="layered")
     ap.add_argument("--label", choices=["survival", "rank"], default="survival")
+    ap.add_argument("--onpolicy", type=int, default=0,
+                    help="rounds of logging the model's own choices and retraining")
     a = ap.parse_args()
     out = ab_compare(tuple(int(x) for x in a.seeds.split(",")),
                      tuple(int(x) for x in a.train_seeds.split(",")), a.outer, a.mlp_epochs,
                      a.explore, log=lambda m: print(m, file=sys.stderr, flush=True), kind=a.kind,
-                     label=a.label)
+                     label=a.label, onpolicy=a.onpolicy)
     print(json.dumps(out))
