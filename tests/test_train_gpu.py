@@ -1,6 +1,8 @@
 """GPU e-prop parity through the C ABI (snc_hip_train_*).
 
-Against the CPU oracle's minibatch step from the same weights and inputs:
+Against the reference's own compiled snc::Trainer (one-sample steps, both
+learning rules), and against the CPU oracle's minibatch step from the same
+weights and inputs:
 rasters and logits bit-exact, per-neuron fires identical, and FP32 weight
 deltas within 1e-5 relative (absolute floor 6e-8, ~2 ulp of the weights:
 softmax uses the device expf). The plain per-sample path and the batched
@@ -211,6 +213,37 @@ def test_reward_mode_matches_oracle_over_minibatches(mode, readout):
             assert np.array_equal(wg, w0)
         moved = max(moved, float(np.abs(wg - w0).max()))
     assert moved > 1e-3  # the rule did move the weights
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not present")
+@pytest.mark.parametrize("reward", [False, True])
+@pytest.mark.parametrize("mode", [BATCHED, PLAIN])
+def test_single_sample_steps_match_the_compiled_reference_trainer(reward, mode):
+    """Batch-1 device steps against the reference's own snc::Trainer (compiled
+    from its sources): Direct encoding (no random draws), spike-count readout,
+    reset to value, fixed T. Each reference epoch holds one sample, so its
+    order is fixed; the device runs that sample as a one-sample minibatch.
+    Weights follow within the tolerance above after every sample, in the
+    supervised rule and in the three-factor reward rule (action stream and
+    baseline carried across samples)."""
+    g = G.static_snc([40, 30, 20, 5], 900, 0.5, 2, 0.2, 3)
+    X, y = O.ref_make_synthetic(8, 40, 5, 0.3, 3)
+    T = 18
+    gcfg, ocfg = _cfgs(T, encoder=0, lr=0.6, w_init=0.35, seed=3, decay=0.85, reward=reward,
+                       reward_decay=0.8)
+    sess = HipTrainSession(g, gcfg, batch=1, max_steps=T)
+    sess.upload_features(X, y, np.full(8, T, np.int32))
+    ref = O.RefTrainer(g, ocfg)
+    start = ref.weights
+    assert np.array_equal(sess.get_weights(), start)
+    for i in range(8):
+        w0 = ref.weights
+        ref.train_epoch(X[i:i + 1], y[i:i + 1], i + 1, 5)
+        sess.step(np.array([i]), epoch=i + 1, mode=mode, apply=True)
+        np.testing.assert_allclose(sess.get_weights() - w0, ref.weights - w0, rtol=W_RTOL,
+                                   atol=W_ATOL)
+        sess.set_weights(ref.weights)  # each sample starts both sides from the same weights
+    assert np.abs(ref.weights - start).max() > 1e-3
 
 
 def test_delay_rich_recurrent_ring_outside_lds():
