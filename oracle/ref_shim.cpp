@@ -239,7 +239,7 @@ double ref_trainer_evaluate(void* h, const float* X, const int32_t* y, int32_t n
   return r->tr->evaluate(d);
 }
 
-// ---- connectome (grow heuristic, the judge of the learned proposal) --------
+// ---- connectome (the grow heuristic the learned proposal is compared with) -
 void* ref_connectome_create(const int32_t* layers, int32_t n_layers, int32_t budget,
                             int32_t delay, float w_init, uint64_t seed) {
   std::vector<int> L(layers, layers + n_layers);

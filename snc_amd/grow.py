@@ -4,7 +4,7 @@ The MLP (16 -> 64 -> 64 -> 1, BF16 weights, FP32 accumulation) scores
 candidate synapses for the grow step of structural co-training; it is
 trained on the GPU on logged grow outcomes (did the grown synapse survive
 the next prune). The reference's grow heuristic remains the fallback and
-the judge (src/structure/connectome.cpp:164-211).
+the baseline it is compared with (src/structure/connectome.cpp:164-211).
 """
 from __future__ import annotations
 

@@ -5,8 +5,8 @@ structural epoch, 12 structural epochs of batched e-prop on the prototype
 image task).
 
 The MLP is trained on the GPU, by its own forward/backward kernels, from grow
-outcomes logged by heuristic co-training runs (seeds 101-103), then judged on
-seeds 1-5 it has not seen. Every part is deterministic (ordered folds and
+outcomes logged by heuristic co-training runs (seeds 101-103), then compared
+on seeds 1-5 it has not seen. Every part is deterministic (ordered folds and
 reductions), so the comparison is reproducible bit for bit.
 """
 import numpy as np

@@ -200,7 +200,7 @@ def test_reward_mode_matches_oracle_over_minibatches(mode, readout):
         ev = _events_for(X[sl], lengths[sl])
         w0 = sess.get_weights()
         orc.weights = w0
-        apply = r != 1  # round 1 runs without an update: no action is drawn
+        apply = r != 1  # minibatch 1 runs without an update: no action is drawn
         st, logits, raster = sess.step(np.arange(B * r, B * (r + 1)), mode=mode, apply=apply,
                                        want_logits=True, want_raster=True)
         ref = orc.minibatch(ev, lengths[sl], y[sl], apply=apply, raster=True)
