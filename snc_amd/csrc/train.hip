@@ -192,6 +192,24 @@ constexpr int kFoldBatch = 8;       // staged arrivals a fold round loads togeth
 __device__ __forceinline__ void fold_staged(const TrainArgs& a, int j, int r0, int r1,
                                             const uint2* s_cval, float* ring, int n, int tm) {
   const int L = a.L;
+  if (r0 == r1) return;
+  // Arrivals with one delay (first and last equal: the column is ordered by
+  // delay) all land in one slot: one ring read, the adds in order, one write.
+  if (s_cval[r0].y == s_cval[r1 - 1].y) {
+    int s = tm + (int)s_cval[r0].y;
+    if (s >= L) s -= L;
+    float* slot = ring + (size_t)s * n + j;
+    float acc = *slot;
+    int r = r0;
+    for (; r + 4 <= r1; r += 4) {
+      const float v0 = __uint_as_float(s_cval[r].x), v1 = __uint_as_float(s_cval[r + 1].x);
+      const float v2 = __uint_as_float(s_cval[r + 2].x), v3 = __uint_as_float(s_cval[r + 3].x);
+      acc = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(acc, v0), v1), v2), v3);
+    }
+    for (; r < r1; ++r) acc = __fadd_rn(acc, __uint_as_float(s_cval[r].x));
+    *slot = acc;
+    return;
+  }
   int cur = -1;
   float acc = 0.0f;
   for (int r = r0; r < r1; r += kFoldBatch) {
