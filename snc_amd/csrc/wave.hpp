@@ -19,14 +19,18 @@ __device__ __forceinline__ int rank_below(uint64_t m) {
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Inclusive prefix sum of x over the wave (Hillis-Steele with DPP-free shuffles).
+// Inclusive prefix sum of x over the wave, every lane active. Data-parallel
+// primitives in registers (no LDS round trip): row_shr 1, 2, 4, 8 scan each
+// row of 16 lanes; row_bcast 15 then adds row 0's total into row 1 (and row
+// 2's into row 3), and row_bcast 31 adds lane 31's prefix into rows 2 and 3.
+// Lanes whose source lies outside the row keep 0 and add nothing.
 __device__ __forceinline__ int wave_inclusive_scan(int x) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int y = __shfl_up(x, d, kWave);
-    if (lane >= d) x += y;
-  }
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 -> rows 1, 3
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 -> rows 2, 3
   return x;
 }
 
