@@ -364,7 +364,9 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   uint32_t* s_bits = s_fw + a.words;              // delivery buckets [bits_words]
   float* s_logit = (float*)(s_bits + a.bits_words);  // [classes]
   float* s_delta = s_logit + a.classes;              // [classes]
-  const size_t state_bytes = train_state_bytes(n, a.words, a.bits_words, a.classes);
+  uint32_t* s_rand = (uint32_t*)(s_delta + a.classes);  // [rand_words] Philox words of one step
+  const int rand_words = train_rand_words(a);
+  const size_t state_bytes = train_state_bytes(n, a.words, a.bits_words, a.classes, rand_words);
   const size_t const_bytes = train_const_bytes(n, a.n_inputs);
   int32_t* s_rowp = (int32_t*)(smem + state_bytes);  // [n + 1] row_ptr
   int32_t* s_ccol = s_rowp + n + 1;                  // [n + 1] column starts
@@ -412,7 +414,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   }
   // Input drive of this sample, from the features once: the direct value,
   // the Poisson spike probability (0: the channel never fires), or the
-  // latency spike step as int bits (-1: never).
+  // latency spike step as int bits (-1: never), or for Poisson the spike
+  // threshold on a Philox word as uint bits (0: never).
   if (!a.ev_base)
     for (int q = tid; q < a.n_inputs; q += kTrainThreads) {
       const int j = a.inputs[q];
@@ -424,7 +427,8 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
         if (x > a.enc_thr)
           val = a.encoder == SNC_ENC_DIRECT ? x * a.gain
                 : lat                       ? __int_as_float(latency_step(x, T))
-                                            : fminf(1.0f, x * a.max_rate);
+                                            : __uint_as_float(prob_to_u32(
+                                                  (double)fminf(1.0f, x * a.max_rate)));
       }
       s_inp[q] = j;
       s_inpx[q] = val;
@@ -453,16 +457,31 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
       for (int q = tid; q < a.n_inputs; q += kTrainThreads)
         if (__float_as_int(s_inpx[q]) == t) s_ext[s_inp[q]] = a.gain;
     } else {
+      // Poisson: the words of step t, staged in s_rand by fill_rand(t)
       for (int q = tid; q < a.n_inputs; q += kTrainThreads) {
-        const float p = s_inpx[q];
-        if (p == 0.0f) continue;
-        const int j = s_inp[q], c = s_chan[j];
-        const Philox4 r = philox4x32_10((uint32_t)(c >> 2), (uint32_t)t, (uint32_t)sample,
-                                        (uint32_t)a.epoch, a.seed_lo, a.seed_hi);
-        if (r.v[c & 3] < prob_to_u32((double)p)) s_ext[j] = a.gain;
+        const uint32_t thr = __float_as_uint(s_inpx[q]);
+        if (thr == 0u) continue;
+        const int j = s_inp[q];
+        if (s_rand[s_chan[j]] < thr) s_ext[j] = a.gain;
       }
     }
   };
+  // The Philox block of channel group g = c / 4 at step t (counter
+  // {g, t, sample, epoch}), computed once per group instead of once per
+  // channel; the highest threads take it, since the LIF phase gives the
+  // lowest ones a second neuron when n > kTrainThreads.
+  auto fill_rand = [&](int t) {
+    for (int g = kTrainThreads - 1 - tid; g < rand_words / 4; g += kTrainThreads) {
+      const Philox4 r = philox4x32_10((uint32_t)g, (uint32_t)t, (uint32_t)sample,
+                                      (uint32_t)a.epoch, a.seed_lo, a.seed_hi);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) s_rand[4 * g + i] = r.v[i];
+    }
+  };
+  if (T > 0 && rand_words) {
+    fill_rand(0);
+    __syncthreads();
+  }
   if (T > 0) inject(0);
   __syncthreads();
 
@@ -511,6 +530,9 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
         if (f) s_fired[pos + rank_below(m)] = j;
       }
     }
+    // next step's Philox words: inject(t) has read this step's (it ran
+    // before the barrier that ended the previous step)
+    if (rand_words && t + 1 < T) fill_rand(t + 1);
     __syncthreads();
     // (B) raster row; fired rows set bucket bits
     if (a.raster)
@@ -833,7 +855,7 @@ __global__ void k_gather_wc(const float* __restrict__ w, const int32_t* __restri
 static inline int blocks_for(int64_t n, int t) { return (int)std::max<int64_t>(1, (n + t - 1) / t); }
 
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds) {
-  size_t b = train_state_bytes(a.n, a.words, a.bits_words, a.classes) +
+  size_t b = train_state_bytes(a.n, a.words, a.bits_words, a.classes, train_rand_words(a)) +
              train_const_bytes(a.n, a.n_inputs);
   if (ring_lds) b += sizeof(float) * (size_t)a.L * a.n;
   return b;

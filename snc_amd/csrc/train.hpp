@@ -77,11 +77,17 @@ struct PlainState {
 // rounded up to 16; the ring (when in LDS) and later the history chunks
 // start there.
 __host__ __device__ inline size_t train_state_bytes(int n, int words, int bits_words,
-                                                    int classes) {
+                                                    int classes, int rand_words) {
   const size_t b = sizeof(float) * 3 * (size_t)n + sizeof(int) * 2 * (size_t)n +
-                   sizeof(uint32_t) * ((size_t)words + bits_words) +
+                   sizeof(uint32_t) * ((size_t)words + bits_words + rand_words) +
                    sizeof(float) * 2 * (size_t)classes;
   return (b + 15) / 16 * 16;
+}
+
+// Philox words the batched kernel stages per step (feature inputs with the
+// Poisson encoder: one word per channel, channels rounded up to 4), else 0.
+__host__ __device__ inline int train_rand_words(const TrainArgs& a) {
+  return !a.ev_base && a.X && a.encoder == SNC_ENC_POISSON ? (a.dim + 3) / 4 * 4 : 0;
 }
 
 // LDS bytes of the per-sample copies of read-only graph data the step loop
