@@ -408,7 +408,8 @@ def main():
     ap.add_argument("--eprop-only", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for counter runs: a 256 MiB device copy (the calibration of the "
-                         "traffic counters), then one run of --variant on the workload")
+                         "traffic counters), then one run of --variant on the workload "
+                         "(with --eprop-only: the e-prop measurements instead)")
     ap.add_argument("--profile-only", action="store_true",
                     help="build the workload and run K steps without extra measurements")
     args = ap.parse_args()
@@ -436,9 +437,6 @@ def main():
         log(f"e-prop measurements: {time.time() - t:.1f}s")
         return res
 
-    if args.eprop_only:
-        print(json.dumps({"eprop": eprop_all()}))
-        return
     if args.pmc_probe:
         # A plain device-to-device copy of 256 MiB through the HIP runtime
         # (its copy kernel): 256 MiB read + 256 MiB written per dispatch.
@@ -458,6 +456,9 @@ def main():
         hip.hipDeviceSynchronize()
         hip.hipFree(src)
         hip.hipFree(dst)
+    if args.eprop_only:
+        print(json.dumps({"eprop": eprop_all()}))
+        return
     t0 = time.time()
     w = K.c1(args.width, args.height, args.timesteps)
     g = w.graph
