@@ -99,6 +99,17 @@ def test_cotrain_driver_runs(grow):
     assert r.stdout.count("round ") == 4
 
 
+def test_cotrain_driver_reward_rules_run():
+    """--reward 1 (three-factor weight rule) and --reward-struct 1 (rewiring
+    rate scaled by the reward), as cotrain_main.cpp:127-128 takes them."""
+    r = _run([os.path.join(BIN, "snc_hip_cotrain"), "--grow", "heuristic", "--outer", "3",
+              "--num-train", "256", "--num-test", "128", "--num-steps", "40", "--min-steps", "30",
+              "--batch", "64", "--structural-budget", "1500", "--grow-n", "60", "--reward", "1",
+              "--reward-struct", "1"])
+    assert r.returncode == 0 and "final: grow=heuristic" in r.stdout
+    assert r.stdout.count("round ") == 3
+
+
 def test_cotrain_driver_sheet_structure_learns():
     r = _run([os.path.join(BIN, "snc_hip_cotrain"), "--structure", "sheet", "--grow", "learned",
               "--outer", "4", "--inner", "2", "--num-train", "1024", "--num-test", "256",
