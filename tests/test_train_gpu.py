@@ -177,8 +177,9 @@ def test_device_latency_matches_oracle(mode):
 
 
 @pytest.mark.parametrize("mode", [BATCHED, PLAIN])
-@pytest.mark.parametrize("readout", [READOUT_SPIKING, READOUT_LEAKY])
-def test_reward_mode_matches_oracle_over_minibatches(mode, readout):
+@pytest.mark.parametrize("readout,elig", [(READOUT_SPIKING, ELIG_F64), (READOUT_LEAKY, ELIG_F64),
+                                          (READOUT_LEAKY, ELIG_F32)])
+def test_reward_mode_matches_oracle_over_minibatches(mode, readout, elig):
     """Three-factor reward rule (trainer.cpp:165-179): actions drawn in
     minibatch order from the session's action stream, the baseline carried
     across minibatches, and a step without update drawing nothing. Each
@@ -190,7 +191,7 @@ def test_reward_mode_matches_oracle_over_minibatches(mode, readout):
     lengths = np.random.default_rng(3).integers(12, 30, B * rounds).astype(np.int32)
     gcfg, ocfg = _cfgs(30, readout=readout, reset_mode=1, lr=0.8, w_init=0.4, reward=True,
                        reward_decay=0.8)
-    sess = HipTrainSession(g, gcfg, batch=B, max_steps=30)
+    sess = HipTrainSession(g, gcfg, batch=B, max_steps=30, elig=elig)
     orc = O.Trainer(g, ocfg)
     # the dataset keeps its own indices; the oracle takes each minibatch as samples 0..B-1
     sess.upload_events(_events_for(X, lengths), B * rounds, y, lengths)
