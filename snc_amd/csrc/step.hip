@@ -59,16 +59,18 @@ __device__ __forceinline__ void raster_or(uint32_t* row, int first, uint64_t m, 
   if ((uint32_t)(m >> 32) && w0 + 1 < words) atomicOr(row + w0 + 1, (uint32_t)(m >> 32));
 }
 
-// Expansion of CSR row [rb, re) by one wave. Each lane loads kExpandUnroll
-// edges before its first deposit, so a row of up to 64 * kExpandUnroll edges
-// costs one round of loads; indexes past the row's end are clamped to its
-// last edge (in bounds, the deposit is skipped). Two edges per lane cover the
-// delivery-stress graph's 100-edge rows in one round with the fewest clamped
-// loads (measured 1.4 % faster per step than four, eight slower still).
-constexpr int kExpandUnroll = 2;
+// Expansion of CSR row [rb, re) by one wave. Each lane loads U edges before
+// its first deposit, so a row of up to 64 * U edges costs one round of
+// loads; indexes past the row's end are clamped to its last edge (in bounds,
+// the deposit is skipped). Measured on the delivery-stress graph (100-edge
+// rows): hip-atomic is 1.4 % faster per step with two edges per lane than
+// with four (eight is slower still); hip-bucket is 1.1 % faster with four.
+template <bool BUCKET>
+constexpr int expand_unroll() { return BUCKET ? 4 : 2; }
 
 template <bool BUCKET>
 __device__ __forceinline__ void expand_row(const StepArgs& a, int rb, int re, int tm, int lane) {
+  constexpr int kExpandUnroll = expand_unroll<BUCKET>();
   for (int g = rb; g < re; g += kExpandUnroll * kWave) {  // wave-uniform trip count
     const int e0 = g + lane;
     int d[kExpandUnroll], tgt[kExpandUnroll];
