@@ -161,7 +161,7 @@ RunResult run(const Options& o, const Data& d, uint64_t seed, int grow_mode, snc
     snc_hip_train* s = nullptr;
     check(snc_hip_train_create(&g.view, &cfg, o.batch, o.num_steps, SNC_ELIG_F64, &s), "session");
     std::vector<float> w(g.view.num_synapses);
-    if (r > 0 || sheet) {  // the sheet's signed start weights from round 0
+    if (r > 0 || sheet) {  // the sheet's signed start weights from structural epoch 0
       check(snc_conn_edge_weights(c, w.data(), (int64_t)w.size()), "edge weights");
       check(snc_hip_train_set_weights(s, w.data(), (int64_t)w.size()), "set weights");
     }

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a0, int t, int tm) {
   const int j = base + tid;
   const int n = a.n;
 
-  // Round 1: every load that depends only on the neuron, the step and the
+  // First batch of loads: everything that depends only on the neuron, the step and the
   // block -- state, row and column bounds, the inline injection entries.
   const int2* tab = a.inj_tab + ((size_t)t * a.n_reg_blocks + b) * kInjInline;
   const bool spill = tab[kInjInline - 1].x == kInjSpill;
@@ -145,8 +145,9 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a0, int t, int tm) {
     s_ev = 0;
   }
   if (BUCKET && mine) {
-    // Rounds 2-3 (registers only, before any barrier): the target's column
-    // words of this step's arrival bitmap, then the values of the set bits.
+    // Second and third batches of loads (registers only, before any
+    // barrier): the target's column words of this step's arrival bitmap,
+    // then the values of the set bits.
     // Neighbouring lanes read neighbouring columns, so the bitmap streams
     // through once per step.
     uint32_t* bnow = a.bits + (size_t)tm * a.bits_slot;

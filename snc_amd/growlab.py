@@ -64,7 +64,7 @@ def run_cotrain(seed, mode, w, cfg, model=None, log=False, explore=0.0, outer=12
                             max_steps=w.max_steps)
         s.upload_features(w.X, w.labels, w.lengths)
         if kind == "sheet":
-            s.set_weights(con.edge_weights())  # signed start weights from round 0
+            s.set_weights(con.edge_weights())  # signed start weights from structural epoch 0
         return s
 
     scfg = StructConfig(grow_per_epoch=grow, delay=1, w_grow_init=0.02, locality_window=window,

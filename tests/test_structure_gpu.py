@@ -73,7 +73,7 @@ def test_device_cotraining_learns():
         s = HipTrainSession(g, dataclasses.replace(w.cfg, seed=2), batch=w.batch,
                             max_steps=w.max_steps)
         s.upload_features(w.X, w.labels, w.lengths)
-        s.set_weights(con.edge_weights())  # the connectome's signed weights from round 0
+        s.set_weights(con.edge_weights())  # the connectome's signed weights from structural epoch 0
         return s
 
     scfg = StructConfig(grow_per_epoch=300, locality_window=0.1, protect_rounds=2, seed=2,
