@@ -412,6 +412,9 @@ def main():
                          "(with --eprop-only: the e-prop measurements instead)")
     ap.add_argument("--profile-only", action="store_true",
                     help="build the workload and run K steps without extra measurements")
+    ap.add_argument("--score-only", action="store_true",
+                    help="time the grow MLP scoring 2^20 candidate descriptors, K calls after "
+                         "W warm-up calls, and print that line alone")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -456,6 +459,28 @@ def main():
         hip.hipDeviceSynchronize()
         hip.hipFree(src)
         hip.hipFree(dst)
+    if args.score_only:
+        import numpy as np
+
+        from snc_amd.grow import N_FEATURES, GrowMLP
+
+        m = 1 << 20
+        mlp = GrowMLP(seed=1)
+        X = np.random.default_rng(0).standard_normal((m, N_FEATURES), dtype=np.float32)
+        for _ in range(args.warmup):
+            mlp.score(X)
+        kms = []
+        for _ in range(args.steps):
+            mlp.score(X)
+            kms.append(mlp.last_kernel_ms)
+        k = sum(kms) / len(kms)
+        nbytes = (4 * N_FEATURES + 4) * m  # fp32 features in, fp32 score out
+        flops = 2 * (16 * 64 + 64 * 64 + 64) * m  # the three layers' multiply-adds
+        print(json.dumps({"grow_score": {
+            "candidates": m, "calls": args.steps, "kernel_ms": k,
+            "candidates_per_s": m / (k * 1e-3), "GB_per_s": nbytes / (k * 1e-3) / 1e9,
+            "TFLOP_per_s": flops / (k * 1e-3) / 1e12}}))
+        return
     if args.eprop_only:
         print(json.dumps({"eprop": eprop_all()}))
         return
