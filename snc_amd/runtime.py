@@ -331,6 +331,34 @@ def resident_graph(g: SNNGraph) -> HipGraph:
     return hg
 
 
+def forward_oneshot(g: SNNGraph, events, num_steps: int, lif: LIFParams, backend: Backend,
+                    capture_raster=False, capture_v=False) -> ForwardResult:
+    """cuda::forward's shape (cuda_backend.cu:186-361) through snc_hip_forward:
+    upload the graph, build a plan, run, read back and free everything in one
+    call. Nothing stays resident, so hip-auto searches on every call."""
+    if not backend_is_hip(backend):
+        raise ValueError(f"{backend_name(backend)}: the hip runtime serves hip-* backends only")
+    n, T = g.num_neurons, int(num_steps)
+    c_out = max(g.num_output_channels, 1)
+    words = (n + 31) // 32
+    logits = np.zeros(c_out, dtype=np.float32)
+    nspk = np.zeros(n, dtype=np.int32)
+    raster = np.zeros((T, words), dtype=np.uint32) if capture_raster else None
+    steps = np.zeros(T, dtype=np.int64) if capture_raster else None
+    fv = np.zeros(n, dtype=np.float32) if capture_v else None
+    cap = L.ForwardCapture(L.ptr(logits, C.c_float), L.ptr(nspk, C.c_int32),
+                           L.ptr(raster, C.c_uint32), L.ptr(fv, C.c_float),
+                           L.ptr(steps, C.c_int64))
+    st = L.ForwardStats()
+    ev, p = event_ptr(events)
+    lc = lif.c()
+    L.check(L.lib().snc_hip_forward(C.byref(g.view()), p, ev.size, T, C.byref(lc), int(backend),
+                                    C.byref(cap), C.byref(st)))
+    return ForwardResult(logits[: g.num_output_channels],
+                         RunStats(st.spikes, st.synaptic_events, st.overflow, st.ms),
+                         Backend(st.used_variant), nspk, raster, fv, steps)
+
+
 def forward(g: SNNGraph, events, num_steps: int, lif: LIFParams, backend: Backend,
             capture_raster=False, capture_v=False) -> ForwardResult:
     """hip::forward: run one sample, read back. The graph stays resident
