@@ -401,6 +401,20 @@ int snc_hip_mlp_score(snc_hip_mlp* m, const float* features, int64_t M, float* s
 int snc_hip_mlp_grad(snc_hip_mlp* m, const float* features, const float* labels, int64_t M,
                      float* grad, double* loss);
 
+/* Allocation-free form of snc_hip_mlp_grad. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight training or serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as snc_hip_mlp_grad for at most
+ * SNC_MLP_NOALLOC_MAX_EXAMPLES examples; a larger M is rejected
+ * (SNC_E_INVALID). It calls no hipMalloc or hipFree: the staged examples and
+ * the per-example gradients and losses live in one fixed device array of
+ * about 21.8 MB per GPU, held from the first model created on that GPU until
+ * the process ends. Every call on a GPU shares that array, so calls that
+ * overlap on one GPU (from several threads or models) race. */
+#define SNC_MLP_NOALLOC_MAX_EXAMPLES 1024
+int snc_hip_mlp_grad_noalloc(snc_hip_mlp* m, const float* features, const float* labels,
+                             int64_t M, float* grad, double* loss);
+
 /* Minibatch Adam on the GPU: `epochs` shuffled passes (SplitMix64(seed)). */
 int snc_hip_mlp_train(snc_hip_mlp* m, const float* features, const float* labels, int64_t M,
                       int32_t epochs, int32_t batch, float lr, uint64_t seed, double* final_loss);

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -24,6 +25,7 @@ static_assert(kMlpIn == SNC_MLP_IN, "feature count mismatch");
 struct snc_hip_mlp {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float* scratch = nullptr;  // this model's device's copy of the fixed scratch array
   float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;  // FP32 master, Adam state, grad
   uint16_t *w1b = nullptr, *w2b = nullptr, *w3b = nullptr;
   float *X = nullptr, *y = nullptr, *out = nullptr, *G = nullptr, *loss = nullptr, *loss1 = nullptr;
@@ -41,6 +43,49 @@ struct snc_hip_mlp {
 };
 
 namespace {
+
+// Fixed scratch of snc_hip_mlp_grad_noalloc: what snc_hip_mlp_grad keeps in
+// buffers it allocates -- the staged examples X [kNoAllocMax][16] and labels,
+// the per-example losses and the per-example gradients G [kNoAllocMax][params]
+// -- in one file-scope device array. Every offset is a multiple of 64 floats,
+// so each part starts 256-byte aligned.
+constexpr int64_t kNoAllocMax = SNC_MLP_NOALLOC_MAX_EXAMPLES;
+constexpr int64_t kScrX = 0;
+constexpr int64_t kScrY = kScrX + kNoAllocMax * kMlpIn;
+constexpr int64_t kScrLoss = kScrY + kNoAllocMax;
+constexpr int64_t kScrG = kScrLoss + kNoAllocMax;
+constexpr int64_t kScrFloats = kScrG + kNoAllocMax * kMlpParams;
+static_assert(kNoAllocMax % 64 == 0, "scratch parts must stay 256-byte aligned");
+
+}  // namespace
+
+static __device__ __align__(256) float scratch[kScrFloats];
+
+namespace {
+
+// The device address of `scratch` on each device, resolved once per device
+// with that device current. In host code the array's name is only a host-side
+// stand-in; kernels and copies get these addresses.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_addr[kMaxDevices] = {};
+
+int scratch_address(float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "scratch: no current device");
+  if (dev < 0 || dev >= kMaxDevices) return fail(SNC_E_UNSUPPORTED, "scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "scratch: hipGetSymbolAddress");
+    if (!p) return fail(SNC_E_HIP, "scratch: hipGetSymbolAddress returned null");
+    g_scratch_addr[dev] = (float*)p;
+  }
+  *out = g_scratch_addr[dev];
+  return SNC_OK;
+}
 
 int grow(void** p, int64_t& cap, int64_t need, size_t elem) {
   if (need <= cap) return SNC_OK;
@@ -149,6 +194,9 @@ int snc_hip_mlp_create(uint64_t seed, snc_hip_mlp** out) {
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess)
     return fail(SNC_E_HIP, "stream/event creation failed");
+  // The allocation-free gradient's scratch on this device (once per device).
+  rc = scratch_address(&m->scratch);
+  if (rc) return rc;
   // He-uniform: U(-sqrt(6 / fan_in), +sqrt(6 / fan_in)); zero biases.
   std::vector<float> p(kMlpParams, 0.0f);
   SplitMix64 r(seed);
@@ -218,6 +266,38 @@ int snc_hip_mlp_grad(snc_hip_mlp* m, const float* features, const float* labels,
   if (rc) return rc;
   std::copy(gacc.begin(), gacc.end(), grad);
   if (loss) *loss = l;
+  return SNC_OK;
+}
+
+int snc_hip_mlp_grad_noalloc(snc_hip_mlp* m, const float* features, const float* labels,
+                             int64_t M, float* grad, double* loss) {
+  if (!m || !features || !labels || !grad || M < 1) return fail(SNC_E_INVALID, "bad arguments");
+  if (M > kNoAllocMax)
+    return fail(SNC_E_INVALID, "snc_hip_mlp_grad_noalloc: M = " + std::to_string(M) +
+                                   " exceeds SNC_MLP_NOALLOC_MAX_EXAMPLES (" +
+                                   std::to_string(kNoAllocMax) + ")");
+  if (!m->scratch) return fail(SNC_E_HIP, "snc_hip_mlp_grad_noalloc: scratch not resolved");
+  hip_clear();
+  // Every G row and loss entry of the M examples is written by the gradient
+  // kernel before the reduction reads it, so nothing left by an earlier call
+  // needs clearing.
+  float* X = m->scratch + kScrX;
+  float* y = m->scratch + kScrY;
+  float* lossv = m->scratch + kScrLoss;
+  float* G = m->scratch + kScrG;
+  if (hipMemcpyAsync(X, features, sizeof(float) * M * kMlpIn, hipMemcpyHostToDevice, m->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(y, labels, sizeof(float) * M, hipMemcpyHostToDevice, m->stream) != hipSuccess)
+    return fail(SNC_E_HIP, "example upload failed");
+  int rc = mlp_launch_grad(m->dev(), X, y, M, G, lossv, m->g, m->loss1, m->stream);
+  if (rc) return rc;
+  float l = 0.0f;
+  if (hipMemcpyAsync(grad, m->g, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(&l, m->loss1, sizeof(float), hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+      hipStreamSynchronize(m->stream) != hipSuccess)
+    return fail(SNC_E_HIP, "MLP gradient readback failed");
+  if (loss) *loss = (double)l;
   return SNC_OK;
 }
 

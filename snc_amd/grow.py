@@ -16,6 +16,7 @@ from . import _lib as L
 
 N_FEATURES = 16
 N_PARAMS = 5313
+NOALLOC_MAX_EXAMPLES = 1024  # SNC_MLP_NOALLOC_MAX_EXAMPLES
 
 _SIGS = {
     "snc_hip_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
@@ -26,6 +27,9 @@ _SIGS = {
                                     C.POINTER(C.c_float), C.POINTER(C.c_double)]),
     "snc_hip_mlp_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                    C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+    "snc_hip_mlp_grad_noalloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                           C.c_int64, C.POINTER(C.c_float),
+                                           C.POINTER(C.c_double)]),
     "snc_hip_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                     C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
                                     C.POINTER(C.c_double)]),
@@ -67,6 +71,18 @@ class GrowMLP:
         loss = C.c_double()
         L.check(L.lib().snc_hip_mlp_grad(self._h, L.ptr(X, C.c_float), L.ptr(y, C.c_float),
                                          X.shape[0], L.ptr(g, C.c_float), C.byref(loss)))
+        return loss.value, g
+
+    def grad_noalloc(self, features, labels):
+        """grad() through snc_hip_mlp_grad_noalloc: no device allocation, at
+        most NOALLOC_MAX_EXAMPLES examples, the GPU's one fixed scratch array."""
+        X = np.ascontiguousarray(features, np.float32).reshape(-1, N_FEATURES)
+        y = np.ascontiguousarray(labels, np.float32)
+        g = np.zeros(N_PARAMS, np.float32)
+        loss = C.c_double()
+        L.check(L.lib().snc_hip_mlp_grad_noalloc(self._h, L.ptr(X, C.c_float),
+                                                 L.ptr(y, C.c_float), X.shape[0],
+                                                 L.ptr(g, C.c_float), C.byref(loss)))
         return loss.value, g
 
     def train(self, features, labels, epochs=20, batch=256, lr=1e-3, seed=1) -> float:
