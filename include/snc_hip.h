@@ -171,13 +171,16 @@ int snc_hip_plan_get_config(const snc_hip_plan* p, snc_plan_config* out);
  * SNC_VARIANT_AUTO_EXACT search, batched the search of batched plans (kept
  * apart from single-sample plans). Fails when no such search has run yet. */
 #define SNC_TUNE_MAX_CANDIDATES 24
+#define SNC_TUNE_ERROR_LEN 120
 typedef struct snc_tune_report {
   int32_t n_candidates;
   int32_t chosen;
   int32_t trial_steps;
   double search_ms; /* wall time of the whole search */
   snc_plan_config cand[SNC_TUNE_MAX_CANDIDATES];
-  double ms_per_step[SNC_TUNE_MAX_CANDIDATES];
+  double ms_per_step[SNC_TUNE_MAX_CANDIDATES]; /* -1: the candidate could not run */
+  int32_t status[SNC_TUNE_MAX_CANDIDATES];     /* SNC_OK or the SNC_E_* it failed with */
+  char error[SNC_TUNE_MAX_CANDIDATES][SNC_TUNE_ERROR_LEN]; /* its error text, or "" */
 } snc_tune_report;
 int snc_hip_graph_tune_report(const snc_hip_graph* g, int32_t exact_only, int32_t batched,
                               snc_tune_report* out);

@@ -6,8 +6,10 @@ raises ``HipBackendMissing``.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsnc_hip.so")
@@ -148,12 +150,15 @@ class PlanConfig(C.Structure):
 
 
 TUNE_MAX_CANDIDATES = 24
+TUNE_ERROR_LEN = 120
 
 
 class TuneReport(C.Structure):
     _fields_ = [("n_candidates", C.c_int32), ("chosen", C.c_int32), ("trial_steps", C.c_int32),
                 ("search_ms", C.c_double), ("cand", PlanConfig * TUNE_MAX_CANDIDATES),
-                ("ms_per_step", C.c_double * TUNE_MAX_CANDIDATES)]
+                ("ms_per_step", C.c_double * TUNE_MAX_CANDIDATES),
+                ("status", C.c_int32 * TUNE_MAX_CANDIDATES),
+                ("error", (C.c_char * TUNE_ERROR_LEN) * TUNE_MAX_CANDIDATES)]
 
 
 _P = C.c_void_p
@@ -234,6 +239,27 @@ _SIGS = {
 }
 
 _lib = None
+
+# Objects holding library handles (graphs, plans, sessions, models). They are
+# closed at interpreter exit, before the HIP runtime's own teardown, instead
+# of by __del__ during module teardown, when the runtime may already be gone.
+_LIVE = weakref.WeakSet()
+
+
+def track(obj):
+    """Register an object with a close() method for closing at exit."""
+    _LIVE.add(obj)
+    return obj
+
+
+@atexit.register
+def _close_live():
+    # plans, sessions and models first, graphs (which plans use) last
+    for obj in sorted(list(_LIVE), key=lambda o: bool(getattr(o, "_close_last", False))):
+        try:
+            obj.close()
+        except Exception:
+            pass
 
 
 def lib():

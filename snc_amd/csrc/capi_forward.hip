@@ -1072,6 +1072,9 @@ static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_
     if (rep.n_candidates >= SNC_TUNE_MAX_CANDIDATES) break;
     rep.cand[rep.n_candidates] = c;
     rep.ms_per_step[rep.n_candidates] = ms;
+    rep.status[rep.n_candidates] = rc;
+    if (rc)
+      std::snprintf(rep.error[rep.n_candidates], SNC_TUNE_ERROR_LEN, "%s", snc_hip_last_error());
     if (ms >= 0.0 && (best < 0 || ms < best_ms)) {
       best = rep.n_candidates;
       best_ms = ms;

@@ -98,7 +98,7 @@ struct snc_hip_train {
   std::mt19937_64 action_rng;
   float *r_u = nullptr, *r_base = nullptr, *r_delta = nullptr, *r_L = nullptr, *r_adv = nullptr;
   int32_t* r_act = nullptr;
-  // captured plain-path chains, keyed by (sample length, update on/off)
+  // captured plain-path chains, keyed by (chain length, update on/off)
   std::map<std::pair<int, int>, hipGraphExec_t> plain_graphs;
 
   void drop_graphs() {
@@ -128,6 +128,10 @@ struct snc_hip_train {
 };
 
 namespace {
+
+// Step granule of the plain path's captured chains: 8 steps, or more when
+// max_steps exceeds 256, so that at most 32 chain lengths exist.
+int plain_granule(int max_steps) { return std::max(8, (max_steps + 31) / 32); }
 
 int fill0(hipStream_t st, void* p, size_t bytes) {
   return p && bytes ? fwd_launch_fill(st, p, 0u, (bytes + 3) / 4) : SNC_OK;
@@ -565,27 +569,32 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
       if (!r) r = train_launch_plain_sample(a, s->ps, T, s->f64, sm);
       return r;
     };
-    // The chain is captured once per (length, update) into a hipGraph and
+    // The chain is captured into a hipGraph per (chain length, update) and
     // replayed; rasters (a per-call buffer) and SNC_HIP_GRAPH=0 run eagerly.
+    // A sample of length T replays the chain of length T rounded up to a
+    // multiple of plain_granule() (its step kernels past T return at once),
+    // so at most 2 * ceil(max_steps / granule) <= 66 graphs are ever live.
     const char* ge = std::getenv("SNC_HIP_GRAPH");
     const bool use_graph = !a.raster && !(ge && ge[0] == '0');
+    const int gran = plain_granule(s->max_steps);
     for (int b = 0; b < B && !rc; ++b) {
       const int sample = idx[b];
       const int T = s->h_lengths[sample];
-      rc = train_launch_plain_begin(s->ps, b, sample, epoch, sm);
+      rc = train_launch_plain_begin(s->ps, b, sample, epoch, T, sm);
       if (rc) break;
       if (!use_graph) {
         rc = chain(T);
         continue;
       }
-      const auto key = std::make_pair(T, apply ? 1 : 0);
+      const int Tc = std::min(s->max_steps, (T + gran - 1) / gran * gran);
+      const auto key = std::make_pair(Tc, apply ? 1 : 0);
       auto it = s->plain_graphs.find(key);
       if (it == s->plain_graphs.end()) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         if (hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal) != hipSuccess)
           return fail(SNC_E_HIP, "hipStreamBeginCapture failed");
-        const int crc = chain(T);
+        const int crc = chain(Tc);
         const hipError_t e = hipStreamEndCapture(sm, &graph);
         if (crc || e != hipSuccess || !graph) {
           if (graph) (void)hipGraphDestroy(graph);

@@ -641,15 +641,18 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
 // ---------------------------------------------------------------------------
 
 // The plain kernels read the sample being run from ps.cur = {position b in
-// the minibatch, dataset index, epoch}, so one captured chain per sample
-// length serves every sample.
-__global__ void k_p_begin(PlainState ps, int b, int sample, int epoch) {
+// the minibatch, dataset index, epoch, length T}, so one captured chain of
+// some length serves every sample that is not longer: the step kernels of
+// steps t >= T return at once.
+__global__ void k_p_begin(PlainState ps, int b, int sample, int epoch, int T) {
   ps.cur[0] = b;
   ps.cur[1] = sample;
   ps.cur[2] = epoch;
+  ps.cur[3] = T;
 }
 
 __global__ void k_p_inject(TrainArgs a, PlainState ps, int t) {
+  if (t >= ps.cur[3]) return;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int sample = ps.cur[1];
   if (a.ev_base) {
@@ -664,6 +667,7 @@ __global__ void k_p_inject(TrainArgs a, PlainState ps, int t) {
 }
 
 __global__ void k_p_lif(TrainArgs a, PlainState ps, int t, int tm) {
+  if (t >= ps.cur[3]) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   bool f = false;
   if (j < a.n) {
@@ -700,6 +704,7 @@ __global__ void k_p_lif(TrainArgs a, PlainState ps, int t, int tm) {
 }
 
 __global__ void k_p_trace(TrainArgs a, PlainState ps, int t) {
+  if (t >= ps.cur[3]) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < a.n) ps.tr[j] = trace_step(a.lif.decay, ps.tr[j], (ps.fw[j >> 5] >> (j & 31)) & 1u);
   if (a.raster && j < a.words)
@@ -707,14 +712,15 @@ __global__ void k_p_trace(TrainArgs a, PlainState ps, int t) {
 }
 
 template <typename ET>
-__global__ void k_p_elig(TrainArgs a, PlainState ps) {
+__global__ void k_p_elig(TrainArgs a, PlainState ps, int t) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.S) return;
+  if (t >= ps.cur[3] || s >= a.S) return;
   const float tp = ps.tr[a.src[s]];
   if (tp != 0.0f) elig_add<ET>((ET*)ps.E, s, ps.psi[a.post[s]], tp);
 }
 
-__global__ void k_p_deliver(TrainArgs a, PlainState ps) {
+__global__ void k_p_deliver(TrainArgs a, PlainState ps, int t) {
+  if (t >= ps.cur[3]) return;
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int wave = gtid >> 6, lane = lane_id(), nw = (gridDim.x * blockDim.x) >> 6;
   const int nf = *ps.nf;
@@ -731,7 +737,8 @@ __global__ void k_p_deliver(TrainArgs a, PlainState ps) {
   if (lane == 0 && ev) atomicAdd(ps.events, ev);
 }
 
-__global__ void k_p_fold(TrainArgs a, PlainState ps, int tm) {
+__global__ void k_p_fold(TrainArgs a, PlainState ps, int t, int tm) {
+  if (t >= ps.cur[3]) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0) {
     *ps.spikes += (unsigned long long)*ps.nf;
@@ -741,11 +748,12 @@ __global__ void k_p_fold(TrainArgs a, PlainState ps, int tm) {
   fold_light(j, a.n, a.L, tm, a.ccol, a.wc, a.cdel, ps.bits, ps.ring);
 }
 
-__global__ __launch_bounds__(kTrainThreads) void k_p_finish(TrainArgs a, PlainState ps, int T) {
+__global__ __launch_bounds__(kTrainThreads) void k_p_finish(TrainArgs a, PlainState ps) {
   __shared__ float s_delta[64];
   __shared__ float s_logit[64];
   const int tid = threadIdx.x;
   const int b = ps.cur[0];
+  const int T = ps.cur[3];
   if (tid < a.classes)
     s_logit[tid] = a.readout_leaky ? __fdiv_rn(ps.logit[tid], (float)max(1, T)) : ps.logit[tid];
   __syncthreads();
@@ -753,10 +761,10 @@ __global__ __launch_bounds__(kTrainThreads) void k_p_finish(TrainArgs a, PlainSt
 }
 
 template <typename ET>
-__global__ void k_p_contrib(TrainArgs a, PlainState ps, int T) {
+__global__ void k_p_contrib(TrainArgs a, PlainState ps) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.S) return;
-  const float scale = __fdiv_rn(a.lr, (float)max(1, T));
+  const float scale = __fdiv_rn(a.lr, (float)max(1, ps.cur[3]));
   const float e = (float)((const ET*)ps.E)[s];
   a.C[(size_t)ps.cur[0] * a.S + s] = a.reward ? e : __fmul_rn(__fmul_rn(scale, ps.L[a.post[s]]), e);
 }
@@ -897,8 +905,9 @@ int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, siz
   return SNC_OK;
 }
 
-int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, hipStream_t st) {
-  hipLaunchKernelGGL(k_p_begin, dim3(1), dim3(1), 0, st, ps, b, sample, epoch);
+int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, int T,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_p_begin, dim3(1), dim3(1), 0, st, ps, b, sample, epoch, T);
   return hip_last("plain begin launch failed");
 }
 
@@ -914,19 +923,19 @@ int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int T, b
     hipLaunchKernelGGL(k_p_trace, dim3(std::max(gn, blocks_for(a.words, 256))), dim3(256), 0, st,
                        a, ps, t);
     if (f64)
-      hipLaunchKernelGGL(k_p_elig<double>, dim3(gs), dim3(256), 0, st, a, ps);
+      hipLaunchKernelGGL(k_p_elig<double>, dim3(gs), dim3(256), 0, st, a, ps, t);
     else
-      hipLaunchKernelGGL(k_p_elig<float>, dim3(gs), dim3(256), 0, st, a, ps);
-    hipLaunchKernelGGL(k_p_deliver, dim3(std::min(gn, 1024)), dim3(256), 0, st, a, ps);
-    hipLaunchKernelGGL(k_p_fold, dim3(gn), dim3(256), 0, st, a, ps, tm);
+      hipLaunchKernelGGL(k_p_elig<float>, dim3(gs), dim3(256), 0, st, a, ps, t);
+    hipLaunchKernelGGL(k_p_deliver, dim3(std::min(gn, 1024)), dim3(256), 0, st, a, ps, t);
+    hipLaunchKernelGGL(k_p_fold, dim3(gn), dim3(256), 0, st, a, ps, t, tm);
     tm = tm + 1 == L ? 0 : tm + 1;
   }
-  hipLaunchKernelGGL(k_p_finish, dim3(1), dim3(kTrainThreads), 0, st, a, ps, T);
+  hipLaunchKernelGGL(k_p_finish, dim3(1), dim3(kTrainThreads), 0, st, a, ps);
   if (a.C) {
     if (f64)
-      hipLaunchKernelGGL(k_p_contrib<double>, dim3(gs), dim3(256), 0, st, a, ps, T);
+      hipLaunchKernelGGL(k_p_contrib<double>, dim3(gs), dim3(256), 0, st, a, ps);
     else
-      hipLaunchKernelGGL(k_p_contrib<float>, dim3(gs), dim3(256), 0, st, a, ps, T);
+      hipLaunchKernelGGL(k_p_contrib<float>, dim3(gs), dim3(256), 0, st, a, ps);
   }
   return hip_last("plain trainer launch failed");
 }

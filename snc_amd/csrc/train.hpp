@@ -66,7 +66,7 @@ struct TrainArgs {
 struct PlainState {
   float *v, *tr, *psi, *ext, *ring, *logit, *L;
   int32_t *refr, *fired, *nf;
-  int32_t* cur;  // [3]: position in the minibatch, dataset index, epoch
+  int32_t* cur;  // [4]: position in the minibatch, dataset index, epoch, length
   uint32_t *fw, *bits;
   void* E;  // [S]
   unsigned long long *spikes, *events;
@@ -119,7 +119,11 @@ __host__ __device__ inline size_t train_elig_lds_bytes(int n, int steps) {
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds);
 int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
                          hipStream_t st);
-int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, hipStream_t st);
+int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, int T,
+                             hipStream_t st);
+// The step kernels of T steps and the end-of-sample kernels. The sample's own
+// length (set by train_launch_plain_begin) may be shorter: its step kernels
+// of later steps return at once.
 int train_launch_plain_sample(const TrainArgs& a, const PlainState& ps, int T, bool f64,
                               hipStream_t st);
 // Reward mode, after every sample of the minibatch has run: actions,

@@ -41,6 +41,7 @@ class GrowMLP:
     def __init__(self, seed: int = 1):
         self._h = C.c_void_p()
         L.check(L.lib().snc_hip_mlp_create(seed, C.byref(self._h)))
+        L.track(self)
         self.last_kernel_ms = 0.0
 
     @property

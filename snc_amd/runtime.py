@@ -142,10 +142,13 @@ class PlanConfig:
 class HipGraph:
     """A graph uploaded once to HBM (snc_hip_graph_upload)."""
 
+    _close_last = True  # plans built on it close before it at exit
+
     def __init__(self, g: SNNGraph):
         self.graph = g
         self._h = C.c_void_p()
         L.check(L.lib().snc_hip_graph_upload(C.byref(g.view()), C.byref(self._h)))
+        L.track(self)
 
     def profile(self) -> dict:
         p = L.GraphProfile()
@@ -175,8 +178,11 @@ class HipGraph:
         L.check(L.lib().snc_hip_graph_tune_report(self._h, 1 if exact_only else 0,
                                                   1 if batched else 0, C.byref(r)))
         cands = [(PlanConfig.from_c(r.cand[i]), r.ms_per_step[i]) for i in range(r.n_candidates)]
+        failed = [(PlanConfig.from_c(r.cand[i]), int(r.status[i]),
+                   r.error[i].value.decode(errors="replace"))
+                  for i in range(r.n_candidates) if r.status[i] != L.SNC_OK]
         return {"candidates": cands, "chosen": r.chosen, "trial_steps": r.trial_steps,
-                "search_ms": r.search_ms}
+                "search_ms": r.search_ms, "failed": failed}
 
     def close(self):
         if self._h:
@@ -226,6 +232,7 @@ class ForwardPlan:
         else:
             L.check(L.lib().snc_hip_plan_create(hg._h, p, ev.size, self.num_steps, C.byref(lc),
                                                 int(backend), C.byref(self._h)))
+        L.track(self)
 
     @property
     def variant(self) -> Backend:

@@ -86,6 +86,7 @@ class Connectome:
         self._h = C.c_void_p()
         L.check(L.lib().snc_conn_layered_local(L.ptr(lay, C.c_int32), lay.size, synapse_budget,
                                                delay, w_init, seed, C.byref(self._h)))
+        L.track(self)
         self.layers = list(layers)
 
     @classmethod
@@ -101,6 +102,7 @@ class Connectome:
         L.check(L.lib().snc_conn_sheet_recurrent(L.ptr(lay, C.c_int32), lay.size, fan_in,
                                                  rec_fan_in, radius, delay, w_init, seed,
                                                  C.byref(obj._h)))
+        L.track(obj)
         obj.layers = list(layers)
         return obj
 

@@ -144,6 +144,7 @@ class HipTrainSession:
         cc = cfg.c()
         L.check(lib.snc_hip_train_create(C.byref(g.view()), C.byref(cc), self.batch,
                                          self.max_steps, elig, C.byref(self._h)))
+        L.track(self)
         self.n_data = 0
 
     # weights ---------------------------------------------------------------
