@@ -322,6 +322,33 @@ def eprop_cpu_baseline(w, n_samples):
             "seconds": dt}
 
 
+def eprop_phases(w, elig, minibatches=3):
+    """Where the batched e-prop kernel's time goes on one workload: the phase
+    split (snc_hip_train_profile) of the first `minibatches` training
+    minibatches of epoch 1, after one untimed warm-up launch."""
+    import numpy as np
+
+    from snc_amd.train import BATCHED, PHASE_NAMES
+
+    order = np.random.default_rng(12345).permutation(np.arange(w.n_train, dtype=np.int32))
+    batches = [order[i:i + w.batch].astype(np.int32) for i in range(0, order.size, w.batch)]
+    s = _train_session(w, elig)
+    s.step(batches[0], epoch=0, mode=BATCHED, apply=False)
+    runs = [s.profile(b, epoch=1) for b in batches[:minibatches]]
+    s.close()
+    out = {"workload": w.name, "batch": w.batch, "eligibility": "f64" if elig == 1 else "f32",
+           "minibatches": len(runs),
+           "kernel_ms": [r["kernel_ms"] for r in runs],
+           "max_steps_per_minibatch": [int(w.lengths[b].max()) for b in batches[:minibatches]],
+           "phases_mean_us": {k: float(np.mean([r["phases"][k]["mean_us"] for r in runs]))
+                              for k in PHASE_NAMES},
+           "phases_max_us": {k: float(np.mean([r["phases"][k]["max_us"] for r in runs]))
+                             for k in PHASE_NAMES}}
+    log(f"e-prop phases {w.name} ({out['eligibility']}): kernel {out['kernel_ms']} ms; mean us "
+        + ", ".join(f"{k} {v:.0f}" for k, v in out["phases_mean_us"].items()))
+    return out
+
+
 def batched_forward(w, hg, B, steps, warmup, variant):
     """The delivery-stress graph with B independent drives in one batched plan
     (one launch per step covers every sample): events/s and the step
@@ -406,6 +433,9 @@ def main():
     ap.add_argument("--no-grow", action="store_true",
                     help="skip the learned-vs-heuristic grow comparison")
     ap.add_argument("--eprop-only", action="store_true")
+    ap.add_argument("--eprop-phases", action="store_true",
+                    help="print the batched e-prop kernel's phase split on both training "
+                         "workloads (f64 and f32 eligibility) and exit")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for counter runs: a 256 MiB device copy (the calibration of the "
                          "traffic counters), then one run of --variant on the workload "
@@ -483,6 +513,10 @@ def main():
         return
     if args.eprop_only:
         print(json.dumps({"eprop": eprop_all()}))
+        return
+    if args.eprop_phases:
+        print(json.dumps({"eprop_phases": [eprop_phases(mk(), el) for mk in (K.c2, K.c3)
+                                           for el in (1, 0)]}))
         return
     t0 = time.time()
     w = K.c1(args.width, args.height, args.timesteps)

@@ -373,6 +373,22 @@ int snc_hip_train_evaluate(snc_hip_train* s, const int32_t* idx, int32_t n, int3
 
 /* cudatrain::reset_stats / collect_stats: per-synapse deliveries and
  * per-neuron fires accumulated over training steps (not evaluation). */
+/* Where the batched kernel's time goes (no counterpart in the reference: an
+ * observability measurement). Runs one batched minibatch of dataset samples
+ * idx[0..B) with a timed build of the kernel, computing the contributions but
+ * applying no update (weights and statistics are unchanged). For each of the
+ * SNC_TRAIN_PHASES phases, mean_us[k] is the time all workgroups together
+ * spent in it divided by B (per sample; eligibility rounds run on whichever
+ * workgroup claims them) and max_us[k] the largest time one workgroup spent in
+ * it: 0 set-up, 1 LIF + trace + histories, 2 delivery bits, 3 bitmap prefix
+ * scan, 4 arrival staging, 5 ordered fold + next injections, 6 end-of-sample
+ * epilogue, 7 eligibility sums, 8 history chunks streamed into LDS,
+ * 9 contribution stores, 10 claiming (and waiting for) eligibility work.
+ * *kernel_ms (may be NULL) is the launch's device time by HIP events. */
+#define SNC_TRAIN_PHASES 11
+int snc_hip_train_profile(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t epoch,
+                          double* mean_us, double* max_us, double* kernel_ms);
+
 int snc_hip_train_reset_stats(snc_hip_train* s);
 int snc_hip_train_collect_stats(const snc_hip_train* s, int64_t* syn_deliveries,
                                 int64_t* neuron_fires);

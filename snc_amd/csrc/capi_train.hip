@@ -68,6 +68,11 @@ struct snc_hip_train {
           *csc_e = nullptr, *channel = nullptr, *internal_row = nullptr, *inputs = nullptr;
   uint8_t *cdel = nullptr, *sdel = nullptr, *role = nullptr;
   float *w = nullptr, *wc = nullptr, *Bfb = nullptr;
+  int2* eunit = nullptr;  // eligibility work units
+  int32_t *ecol_pre = nullptr, *ecol_syn = nullptr;
+  int* wq = nullptr;       // eligibility work queue [2 * batch]
+  float* Ls_g = nullptr;   // learning signals [batch][n]
+  int n_cu = 1;
   // dataset
   int n_data = 0, dim = 0;
   std::vector<int32_t> h_labels, h_lengths;
@@ -114,7 +119,9 @@ struct snc_hip_train {
     for (void* p : {(void*)row_ptr, (void*)post, (void*)src, (void*)cpos, (void*)ccol,
                     (void*)csc_e, (void*)channel, (void*)internal_row, (void*)inputs, (void*)cdel,
                     (void*)sdel,
-                    (void*)role, (void*)w, (void*)wc, (void*)Bfb, (void*)X, (void*)labels,
+                    (void*)role, (void*)w, (void*)wc, (void*)Bfb, (void*)eunit,
+                    (void*)ecol_pre, (void*)ecol_syn, (void*)wq, (void*)Ls_g, (void*)X,
+                    (void*)labels,
                     (void*)lengths, (void*)ev_base, (void*)ev_step_ptr, (void*)ev_nid,
                     (void*)ev_amp, (void*)idx, (void*)hist, (void*)C, (void*)ring_g, (void*)logits,
                     (void*)loss, (void*)correct, (void*)blk_spikes, (void*)blk_events,
@@ -237,6 +244,61 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
     }
   }
   for (int e = 0; e < S; ++e) sdel[e] = (uint8_t)g->delays[e];
+  // Eligibility work units: each post's synapses in column order, cut into
+  // runs of at most kEligPerThread; a lane of the update phase takes one run.
+  if (n >= (1 << 26)) return fail(SNC_E_UNSUPPORTED, "too many neurons for eligibility units");
+  std::vector<int2> eunit;
+  std::vector<int32_t> ecol_pre, ecol_syn;
+  ecol_pre.reserve(S);
+  ecol_syn.reserve(S);
+  // Within a post the synapses are dealt to the unit slots so that slot k of
+  // unit i holds a pre with pre % 8 == (i + k) % 8 where one is left: the
+  // eight lanes one LDS pass serves then read trace rows in different bank
+  // groups. Each synapse's sum is unchanged; only which lane computes it.
+  std::vector<std::vector<int32_t>> by_cls(8);
+  for (int j = 0; j < n; ++j) {
+    const int q0 = (int)ecol_pre.size();
+    for (auto& v : by_cls) v.clear();
+    int cnt = 0;
+    for (int64_t p = ccol[j + 1] - 1; p >= ccol[j]; --p)  // popped back to front below
+      if (csc_e[p] >= 0) {
+        by_cls[src[csc_e[p]] & 7].push_back(csc_e[p]);
+        ++cnt;
+      }
+    const int units = (cnt + kEligPerThread - 1) / kEligPerThread;
+    std::vector<int32_t> slot((size_t)units * kEligPerThread, -1);
+    auto take = [&](int cls) {
+      for (int d = 0; d < 8; ++d) {
+        auto& v = by_cls[(cls + d) & 7];
+        if (!v.empty()) {
+          const int32_t e = v.back();
+          v.pop_back();
+          return e;
+        }
+      }
+      return (int32_t)-1;
+    };
+    int placed = 0;
+    for (int k = 0; k < kEligPerThread && placed < cnt; ++k)
+      for (int i = 0; i < units && placed < cnt; ++i) {
+        // the last unit is the short one: it only takes slots below its length
+        const int len_i = i + 1 < units ? kEligPerThread : cnt - (units - 1) * kEligPerThread;
+        if (k >= len_i) continue;
+        slot[(size_t)i * kEligPerThread + k] = take((i + k) & 7);
+        ++placed;
+      }
+    for (int i = 0; i < units; ++i) {
+      const int len_i = i + 1 < units ? kEligPerThread : cnt - (units - 1) * kEligPerThread;
+      eunit.push_back(make_int2((j << 5) | (len_i - 1), (int)ecol_pre.size()));
+      for (int k = 0; k < len_i; ++k) {
+        const int32_t e = slot[(size_t)i * kEligPerThread + k];
+        ecol_pre.push_back(src[e]);
+        ecol_syn.push_back(e);
+      }
+    }
+    (void)q0;
+  }
+  if ((int64_t)ecol_syn.size() != S) return fail(SNC_E_HIP, "eligibility units miss synapses");
   for (int i = 0; i < n; ++i)
     if (g->role[i] == SNC_ROLE_INPUT) s->h_inputs.push_back(i);
   s->h_src = src;
@@ -255,6 +317,11 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   if (!rc) rc = tupload(&s->internal_row, internal_row.data(), internal_row.size());
   if (!rc) rc = tupload(&s->inputs, s->h_inputs.data(), s->h_inputs.size());
   if (!rc) rc = tupload(&s->w, w.data(), w.size());
+  if (!rc) rc = tupload(&s->eunit, eunit.data(), eunit.size());
+  if (!rc) rc = tupload(&s->ecol_pre, ecol_pre.data(), ecol_pre.size());
+  if (!rc) rc = tupload(&s->ecol_syn, ecol_syn.data(), ecol_syn.size());
+  if (!rc) rc = talloc(&s->wq, 2 * (size_t)batch);
+  if (!rc) rc = talloc(&s->Ls_g, (size_t)batch * n);
   if (!rc) rc = talloc(&s->wc, (size_t)P);
   if (!rc) rc = tupload(&s->Bfb, Bfb.data(), Bfb.size());
   const size_t esz = s->f64 ? sizeof(double) : sizeof(float);
@@ -346,6 +413,13 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   a.idx = s->idx;
   a.hist = s->hist;
   a.hn = hn;
+  a.eunit = s->eunit;
+  a.n_units = (int)eunit.size();
+  a.ecol_pre = s->ecol_pre;
+  a.ecol_syn = s->ecol_syn;
+  a.rounds = (a.n_units + kTrainThreads - 1) / kTrainThreads;
+  a.wq = s->wq;
+  a.Ls_g = s->Ls_g;
   a.logits = s->logits;
   a.loss = s->loss;
   a.correct = s->correct;
@@ -356,6 +430,13 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   hipDeviceProp_t prop;
   (void)hipGetDeviceProperties(&prop, 0);
   const size_t lds_max = prop.sharedMemPerBlock;
+  s->n_cu = std::max(1, prop.multiProcessorCount);
+  // A workgroup polls for unpublished eligibility work for at most 200 ms.
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+    return fail(SNC_E_HIP, "wall clock rate unavailable");
+  a.wait_ticks = (long long)khz * 200;
   s->lds_max = lds_max;
   // Leave room for the kernel's few bytes of static LDS.
   const size_t lds_budget = lds_max > 256 ? lds_max - 256 : 0;
@@ -549,7 +630,10 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
   }
   (void)hipEventRecord(s->ev0, sm);
   if (mode == SNC_TRAIN_BATCHED) {
-    rc = train_launch_batched(a, B, s->f64, s->ring_lds, s->lds_max, sm);
+    if (a.C) rc = fill0(sm, s->wq, sizeof(int) * 2 * (size_t)B);
+    if (!rc)
+      rc = train_launch_batched(a, B, train_batched_grid(a, B, s->n_cu), s->f64, s->ring_lds,
+                                s->lds_max, sm);
   } else if (mode == SNC_TRAIN_PLAIN) {
     const size_t esz = s->f64 ? sizeof(double) : sizeof(float);
     rc = fill0(sm, s->p_spikes, 8);
@@ -711,6 +795,63 @@ int snc_hip_train_evaluate(snc_hip_train* s, const int32_t* idx, int32_t n, int3
     tot.ms += one.ms;
   }
   if (st) *st = tot;
+  return SNC_OK;
+}
+
+int snc_hip_train_profile(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t epoch,
+                          double* mean_us, double* max_us, double* kernel_ms) {
+  if (!s || !idx || B < 1 || B > s->batch || !mean_us || !max_us)
+    return fail(SNC_E_INVALID, "bad arguments");
+  static_assert(kTrainPhases == SNC_TRAIN_PHASES, "phase count mismatch");
+  hip_clear();
+  if (!s->n_data) return fail(SNC_E_INVALID, "no dataset uploaded");
+  for (int b = 0; b < B; ++b)
+    if (idx[b] < 0 || idx[b] >= s->n_data) return fail(SNC_E_INVALID, "sample index out of range");
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+    return fail(SNC_E_HIP, "wall clock rate unavailable");
+  TrainArgs a = s->a;
+  a.epoch = epoch;
+  a.C = s->C;  // contributions are computed but not applied
+  a.neur_fire = nullptr;
+  a.raster = nullptr;
+  const int G = train_batched_grid(a, B, s->n_cu);
+  unsigned long long* prof = nullptr;
+  int rc = talloc(&prof, (size_t)G * kTrainPhases);
+  if (rc) return rc;
+  a.prof = prof;
+  hipStream_t sm = s->stream;
+  if (hipMemcpyAsync(s->idx, idx, sizeof(int32_t) * B, hipMemcpyHostToDevice, sm) != hipSuccess)
+    rc = fail(SNC_E_HIP, "index upload failed");
+  if (!rc) rc = fill0(sm, prof, sizeof(unsigned long long) * G * kTrainPhases);
+  if (!rc) rc = fill0(sm, s->wq, sizeof(int) * 2 * (size_t)B);
+  if (!rc) (void)hipEventRecord(s->ev0, sm);
+  if (!rc) rc = train_launch_batched(a, B, G, s->f64, s->ring_lds, s->lds_max, sm);
+  if (!rc) (void)hipEventRecord(s->ev1, sm);
+  std::vector<unsigned long long> h((size_t)G * kTrainPhases);
+  if (!rc && (hipStreamSynchronize(sm) != hipSuccess ||
+              hipMemcpy(h.data(), prof, sizeof(unsigned long long) * h.size(),
+                        hipMemcpyDeviceToHost) != hipSuccess))
+    rc = fail(SNC_E_HIP, "profiled minibatch failed");
+  tfree(prof);
+  if (rc) return rc;
+  const double us_per_tick = 1e3 / (double)khz;
+  for (int k = 0; k < kTrainPhases; ++k) {
+    double sum = 0.0, mx = 0.0;
+    for (int b = 0; b < G; ++b) {
+      const double us = (double)h[(size_t)b * kTrainPhases + k] * us_per_tick;
+      sum += us;
+      mx = std::max(mx, us);
+    }
+    mean_us[k] = sum / B;  // per sample: eligibility rounds may run on any workgroup
+    max_us[k] = mx;
+  }
+  if (kernel_ms) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    *kernel_ms = ms;
+  }
   return SNC_OK;
 }
 

@@ -52,7 +52,6 @@
 
 namespace snc_amd {
 
-constexpr int kTrainThreads = 1024;
 
 // ---------------------------------------------------------------------------
 // shared device pieces
@@ -177,7 +176,6 @@ __device__ void sample_epilogue(const TrainArgs& a, int b, const float* logits_i
 // batched persistent kernel: blockIdx.x = position in the minibatch
 // ---------------------------------------------------------------------------
 
-constexpr int kEligPerThread = 16;  // synapses whose eligibility a lane keeps in registers
 constexpr int kFoldBatch = 8;       // staged arrivals a fold round loads together
 
 // Ordered fold of target j's arrivals into its ring slots. Columns are
@@ -271,75 +269,124 @@ __device__ __forceinline__ void fold_gather(const TrainArgs& a, int j, int w0, i
   if (cur >= 0) ring[(size_t)cur * n + j] = acc;
 }
 
-// Eligibility and contributions of sample b after its T steps, from the psi
-// and trace histories the step loop wrote:
+// Phase timer of the profiled instantiation (PROF): after each barrier that
+// ends a phase, thread 0 adds the wall-clock ticks since the previous mark to
+// that phase's total; the totals go to a.prof [B][kTrainPhases].
+template <bool PROF>
+__device__ __forceinline__ void phase_mark(unsigned long long* s_ph, long long* s_last, int k,
+                                           int tid) {
+  if constexpr (PROF) {
+    if (tid == 0) {
+      const long long now = wall_clock64();
+      s_ph[k] += (unsigned long long)(now - *s_last);
+      *s_last = now;
+    }
+  }
+}
+
+// Eligibility and contributions of one round of sample b (work units u0 ..
+// u0 + kTrainThreads - 1) after its T steps, from the psi and trace histories
+// the step loop wrote:
 //   E_s = sum_t psi_t[post_s] * tr_t[pre_s],   t ascending, one fma per step,
 // exactly the per-step accumulation of trainer.cpp:132-139 (the product of
-// two floats is exact in double, so fma == add of the exact product). Lanes
-// keep kEligPerThread sums in registers. The histories stream through LDS in
-// chunks of CH steps, transposed: row j holds CH consecutive steps of neuron
-// j's psi (and, in a second array, its trace), so one 16-byte load at a
-// constant offset gives a lane four steps of one neuron. Steps past T in the
-// last chunk are zero and add +0 to a nonnegative sum, changing nothing.
-template <typename ET, int CH>
+// two floats is exact in double, so fma == add of the exact product).
+// The histories stream through LDS in chunks of CH steps, transposed: row j
+// holds CH consecutive steps of neuron j's psi (and, in a second array, its
+// trace), so one 16-byte load at a constant offset gives a lane four steps of
+// one neuron. Each lane takes one work unit: up to kEligPerThread synapses of
+// one post, in column order (a.eunit), with their sums in registers; it reads
+// the post's four psi values once and reuses them for every synapse of the
+// unit, so a fused multiply-add costs about 4 bytes of LDS reads (the
+// synapse's trace) instead of 8. Steps past T in the last chunk are zero and
+// add +0 to a nonnegative sum, changing nothing.
+template <typename ET, int CH, bool PROF>
 __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, const float* Ls,
-                             float* s_chunk, int tid) {
+                             float* s_chunk, int tid, int u0, unsigned long long* s_ph,
+                             long long* s_last) {
   static_assert(CH % 4 == 0, "chunk rows are read four steps at a time");
   const int hn = a.hn, n = a.n;
   float* ps_t = s_chunk;                   // [n][CH]
   float* tr_t = s_chunk + (size_t)n * CH;  // [n][CH]
   const float* hb = a.hist + (size_t)b * a.T_max * 2 * hn;
-  for (int s0 = 0; s0 < a.S; s0 += kEligPerThread * kTrainThreads) {
-    int pro[kEligPerThread], tro[kEligPerThread];  // row offsets of post and pre
+  {
+    const int u = u0 + tid;
+    int post = 0, len = 0, q0 = 0;
+    if (u < a.n_units) {
+      const int2 unit = a.eunit[u];
+      post = unit.x >> 5;
+      len = (unit.x & 31) + 1;
+      q0 = unit.y;
+    }
+    const int pro = post * CH;
+    // row offsets of the unit's pres; a unit shorter than kEligPerThread
+    // repeats its last pre, so the sum loop has no per-synapse branches (the
+    // extra sums are never stored)
+    int tro[kEligPerThread];
     ET E[kEligPerThread];
 #pragma unroll
     for (int k = 0; k < kEligPerThread; ++k) {
-      const int sy = s0 + k * kTrainThreads + tid;
-      pro[k] = (sy < a.S ? a.post[sy] : 0) * CH;
-      tro[k] = (sy < a.S ? a.src[sy] : 0) * CH;
+      tro[k] = (len > 0 ? a.ecol_pre[q0 + min(k, len - 1)] : 0) * CH;
       E[k] = (ET)0;
     }
     for (int t0 = 0; t0 < T; t0 += CH) {
       const int tc = min(CH, T - t0);
       __syncthreads();  // previous chunk consumed
+      phase_mark<PROF>(s_ph, s_last, 7, tid);
       // transpose in groups of 4 steps: 8 coalesced loads in flight per lane
       for (int c = 0; c < CH; c += 4) {
         for (int j = tid; j < n; j += kTrainThreads) {
           float pv[4], tv[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const float* row = hb + (size_t)(t0 + c + u) * 2 * hn;
-            pv[u] = c + u < tc ? row[j] : 0.0f;
-            tv[u] = c + u < tc ? row[hn + j] : 0.0f;
+          for (int v = 0; v < 4; ++v) {
+            const float* row = hb + (size_t)(t0 + c + v) * 2 * hn;
+            pv[v] = c + v < tc ? row[j] : 0.0f;
+            tv[v] = c + v < tc ? row[hn + j] : 0.0f;
           }
           *(float4*)(ps_t + (size_t)j * CH + c) = make_float4(pv[0], pv[1], pv[2], pv[3]);
           *(float4*)(tr_t + (size_t)j * CH + c) = make_float4(tv[0], tv[1], tv[2], tv[3]);
         }
       }
       __syncthreads();
-#pragma unroll
-      for (int k = 0; k < kEligPerThread; ++k) {
-        const float4* pp = (const float4*)(ps_t + pro[k]);
-        const float4* rr = (const float4*)(tr_t + tro[k]);
-        ET e[1] = {E[k]};
-#pragma unroll 2
+      phase_mark<PROF>(s_ph, s_last, 8, tid);
+      if (len > 0) {
+        constexpr int kBatch = 8;  // trace rows loaded together before their sums
+#pragma unroll 1
         for (int q = 0; q < CH / 4; ++q) {
-          const float4 p = pp[q], r = rr[q];
-          elig_add<ET>(e, 0, p.x, r.x);
-          elig_add<ET>(e, 0, p.y, r.y);
-          elig_add<ET>(e, 0, p.z, r.z);
-          elig_add<ET>(e, 0, p.w, r.w);
+          const float4 p = ((const float4*)(ps_t + pro))[q];
+#pragma unroll
+          for (int k0 = 0; k0 < kEligPerThread; k0 += kBatch) {
+            float4 r[kBatch];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) r[k] = ((const float4*)(tr_t + tro[k0 + k]))[q];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+              ET e[1] = {E[k0 + k]};
+              elig_add<ET>(e, 0, p.x, r[k].x);
+              elig_add<ET>(e, 0, p.y, r[k].y);
+              elig_add<ET>(e, 0, p.z, r[k].z);
+              elig_add<ET>(e, 0, p.w, r[k].w);
+              E[k0 + k] = e[0];
+            }
+          }
         }
-        E[k] = e[0];
       }
     }
+    if constexpr (PROF) {
+      __syncthreads();
+      phase_mark<PROF>(s_ph, s_last, 7, tid);
+    }
+    // C is stored in unit order (each post's synapses in column order), so a
+    // lane writes its unit's contributions to consecutive addresses.
+    if (a.C && len > 0) {
+      const float ls = __fmul_rn(scale, Ls[post]);
+      float* cb = a.C + (size_t)b * a.S + q0;
 #pragma unroll
-    for (int k = 0; k < kEligPerThread; ++k) {
-      const int sy = s0 + k * kTrainThreads + tid;
-      if (sy < a.S && a.C)
-        a.C[(size_t)b * a.S + sy] =
-            a.reward ? (float)E[k]
-                     : __fmul_rn(__fmul_rn(scale, Ls[pro[k] / CH]), (float)E[k]);
+      for (int k = 0; k < kEligPerThread; ++k)
+        if (k < len) cb[k] = a.reward ? (float)E[k] : __fmul_rn(ls, (float)E[k]);
+    }
+    if constexpr (PROF) {
+      __syncthreads();
+      phase_mark<PROF>(s_ph, s_last, 9, tid);
     }
   }
 }
@@ -351,10 +398,15 @@ __host__ __device__ inline int elig_chunk_steps(int n, size_t bytes) {
   return fit >= 28 ? 28 : fit >= 20 ? 20 : fit >= 12 ? 12 : fit >= 8 ? 8 : fit >= 4 ? 4 : 0;
 }
 
-template <typename ET, bool RING_LDS>
-__global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// Steps of minibatch sample b (workgroup b): the whole step loop, the
+// end-of-sample epilogue, and -- when contributions are wanted -- the
+// learning signals to HBM and the sample's publication to the eligibility
+// work queue.
+template <typename ET, bool RING_LDS, bool PROF>
+__device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
+                             unsigned long long* s_ph, long long* s_last_p) {
   const int n = a.n, L = a.L, tid = threadIdx.x;
+  long long& s_last = *s_last_p;
   float* s_v = (float*)smem;
   float* s_tr = s_v + n;
   float* s_ext = s_tr + n;
@@ -392,7 +444,6 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   __shared__ int s_wsum[kTrainThreads / kWave];
   __shared__ unsigned long long s_ev, s_sp;
 
-  const int b = blockIdx.x;
   const int sample = a.idx[b];
   const int T = a.lengths[sample];
   const int label = a.labels[sample];
@@ -484,6 +535,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   }
   if (T > 0) inject(0);
   __syncthreads();
+  phase_mark<PROF>(s_ph, &s_last, 0, tid);
 
   int tm = 0;
   for (int t = 0; t < T; ++t) {
@@ -534,6 +586,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     // before the barrier that ended the previous step)
     if (rand_words && t + 1 < T) fill_rand(t + 1);
     __syncthreads();
+    phase_mark<PROF>(s_ph, &s_last, 1, tid);
     // (B) raster row; fired rows set bucket bits
     if (a.raster)
       for (int w = tid; w < a.words; w += kTrainThreads)
@@ -555,6 +608,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
       atomicAdd(&s_arr, (int)ev);
     }
     __syncthreads();
+    phase_mark<PROF>(s_ph, &s_last, 2, tid);
     const bool staged = s_arr <= cap;  // block-uniform
     if (staged) {
       // (B2) exclusive popcount prefix of the bitmap words (block scan)
@@ -574,6 +628,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
       }
       if (hi == W) s_pref[W] = (uint16_t)run;  // the total (every such thread writes the same)
       __syncthreads();
+      phase_mark<PROF>(s_ph, &s_last, 3, tid);
       // (B3) every arrival's weight and delay at its rank, from the fired
       // rows read in CSR order (coalesced)
       for (int k = wave; k < nf; k += kTrainThreads / kWave) {
@@ -589,6 +644,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
         }
       }
       __syncthreads();
+      phase_mark<PROF>(s_ph, &s_last, 4, tid);
     }
     // (C) ordered fold into the ring (staged: the bitmap is only cleared);
     // next step's injections
@@ -609,6 +665,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
     }
     tm = tm + 1 == L ? 0 : tm + 1;
     __syncthreads();
+    phase_mark<PROF>(s_ph, &s_last, 5, tid);
   }
 
   if (a.readout_leaky && tid < a.classes)
@@ -618,21 +675,128 @@ __global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
   // place and the rest of LDS holds history chunks.
   float* Ls = s_v;
   sample_epilogue(a, b, s_logit, Ls, s_delta, label, n, tid, kTrainThreads);
-  if (a.C) {
-    const float scale = __fdiv_rn(a.lr, (float)max(1, T));
-    const size_t ls_bytes = (sizeof(float) * (size_t)n + 15) / 16 * 16;
-    float* s_chunk = (float*)(smem + ls_bytes);
-    switch (elig_chunk_steps(n, (size_t)a.lds_bytes - ls_bytes)) {  // the host checked >= 4
-      case 28: elig_contrib<ET, 28>(a, b, T, scale, Ls, s_chunk, tid); break;
-      case 20: elig_contrib<ET, 20>(a, b, T, scale, Ls, s_chunk, tid); break;
-      case 12: elig_contrib<ET, 12>(a, b, T, scale, Ls, s_chunk, tid); break;
-      case 8: elig_contrib<ET, 8>(a, b, T, scale, Ls, s_chunk, tid); break;
-      default: elig_contrib<ET, 4>(a, b, T, scale, Ls, s_chunk, tid); break;
-    }
+  if constexpr (PROF) {
+    __syncthreads();
+    phase_mark<PROF>(s_ph, &s_last, 6, tid);
   }
   if (tid == 0) {
     a.blk_spikes[b] = s_sp;
     a.blk_events[b] = s_ev;
+  }
+  if (a.C) {
+    for (int j = tid; j < n; j += kTrainThreads) a.Ls_g[(size_t)b * n + j] = Ls[j];
+    // Publish: every thread's history and learning-signal stores are made
+    // visible device-wide (all XCDs) before the sample enters the queue.
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&a.wq[b], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// The eligibility work queue. Sample b's eligibility is `rounds` items
+// (round r: work units r * kTrainThreads ...). wq[b] is 1 once sample b is
+// published (its histories and learning signals are in HBM), wq[B + b]
+// counts the claims of its rounds. Wave 0 of a workgroup scans the samples 64
+// at a time, starting from its own position, and claims a round of a
+// published sample with one atomic add, so claims of different samples never
+// meet on one address; a claim beyond `rounds` is simply ignored. An item is
+// only claimed once its sample is published, so a claimed item never waits.
+// The scan returns -1 when every round of every sample is claimed. While some
+// sample is still stepping it polls, and gives up after `wait_ticks` of wall
+// clock: a workgroup that gives up leaves its work to the others, and each
+// sample's own workgroup claims its sample's rounds right after publishing
+// it, so every item is processed exactly once. The polls are relaxed loads
+// (an acquire on every poll would invalidate this XCD's L2 each time and
+// evict the data of the samples still stepping); one acquire fence follows a
+// successful claim, and the workgroup barrier after it orders the other
+// waves' loads of the item's data behind it.
+__device__ int claim_item(const TrainArgs& a, int start) {
+  const int B = a.n_samples, R = a.rounds, lane = lane_id();
+  const long long t0 = wall_clock64();
+  for (;;) {
+    bool unfinished = false;
+    for (int c0 = 0; c0 < B; c0 += kWave) {
+      const int k = c0 + lane;
+      const int c = k < B ? (start + k) % B : 0;
+      int pub = 0, cl = R;
+      if (k < B) {
+        cl = __hip_atomic_load(&a.wq[B + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pub = __hip_atomic_load(&a.wq[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (ballot(cl < R)) unfinished = true;
+      uint64_t avail = ballot(pub != 0 && cl < R);
+      while (avail) {
+        const int l = __builtin_ctzll(avail);
+        avail &= avail - 1;
+        int r = 0;
+        if (lane == l) r = atomicAdd(&a.wq[B + c], 1);
+        r = __shfl(r, l, kWave);
+        if (r < R) {
+          const int cc = __shfl(c, l, kWave);
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          return cc * R + r;
+        }
+      }
+    }
+    if (!unfinished) return -1;
+    if (wall_clock64() - t0 > a.wait_ticks) return -1;
+    __builtin_amdgcn_s_sleep(127);
+  }
+}
+
+template <typename ET, bool PROF>
+__device__ void elig_queue(const TrainArgs& a, unsigned char* smem, unsigned long long* s_ph,
+                           long long* s_last) {
+  const int n = a.n, tid = threadIdx.x;
+  __shared__ int s_item;
+  const size_t ls_bytes = (sizeof(float) * (size_t)n + 15) / 16 * 16;
+  float* Ls = (float*)smem;
+  float* s_chunk = (float*)(smem + ls_bytes);
+  const int ch = elig_chunk_steps(n, (size_t)a.lds_bytes - ls_bytes);  // the host checked >= 4
+  for (;;) {
+    __syncthreads();  // the previous item's LDS is consumed
+    if (tid < kWave) {
+      const int it = claim_item(a, (int)blockIdx.x % a.n_samples);
+      if (tid == 0) s_item = it;
+    }
+    __syncthreads();
+    phase_mark<PROF>(s_ph, s_last, 10, tid);
+    const int it = s_item;
+    if (it < 0) break;
+    const int b = it / a.rounds, r = it - b * a.rounds;
+    const int T = a.lengths[a.idx[b]];
+    const float scale = __fdiv_rn(a.lr, (float)max(1, T));
+    for (int j = tid; j < n; j += kTrainThreads) Ls[j] = a.Ls_g[(size_t)b * n + j];
+    const int u0 = r * kTrainThreads;
+    switch (ch) {
+      case 28: elig_contrib<ET, 28, PROF>(a, b, T, scale, Ls, s_chunk, tid, u0, s_ph, s_last); break;
+      case 20: elig_contrib<ET, 20, PROF>(a, b, T, scale, Ls, s_chunk, tid, u0, s_ph, s_last); break;
+      case 12: elig_contrib<ET, 12, PROF>(a, b, T, scale, Ls, s_chunk, tid, u0, s_ph, s_last); break;
+      case 8: elig_contrib<ET, 8, PROF>(a, b, T, scale, Ls, s_chunk, tid, u0, s_ph, s_last); break;
+      default: elig_contrib<ET, 4, PROF>(a, b, T, scale, Ls, s_chunk, tid, u0, s_ph, s_last); break;
+    }
+  }
+}
+
+// One launch per minibatch: workgroups 0 .. B-1 each run one sample's steps
+// (train_sample); then every workgroup, including the helpers B .. grid-1,
+// takes eligibility rounds from the queue until it is empty.
+template <typename ET, bool RING_LDS, bool PROF>
+__global__ __launch_bounds__(kTrainThreads) void k_train_batched(TrainArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x;
+  __shared__ unsigned long long s_ph[kTrainPhases];
+  __shared__ long long s_last;
+  if constexpr (PROF) {
+    if (tid < kTrainPhases) s_ph[tid] = 0ull;
+    if (tid == 0) s_last = wall_clock64();
+  }
+  if ((int)blockIdx.x < a.n_samples)
+    train_sample<ET, RING_LDS, PROF>(a, smem, (int)blockIdx.x, s_ph, &s_last);
+  if (a.C) elig_queue<ET, PROF>(a, smem, s_ph, &s_last);
+  if constexpr (PROF) {
+    __syncthreads();
+    if (tid < kTrainPhases) a.prof[(size_t)blockIdx.x * kTrainPhases + tid] = s_ph[tid];
   }
 }
 
@@ -760,13 +924,16 @@ __global__ __launch_bounds__(kTrainThreads) void k_p_finish(TrainArgs a, PlainSt
   sample_epilogue(a, b, s_logit, ps.L, s_delta, a.labels[ps.cur[1]], a.n, tid, kTrainThreads);
 }
 
+// Contributions in unit order (q: each post's synapses in column order; the
+// synapse is ecol_syn[q]), the layout the batched kernel stores.
 template <typename ET>
 __global__ void k_p_contrib(TrainArgs a, PlainState ps) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.S) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.S) return;
+  const int s = a.ecol_syn[q];
   const float scale = __fdiv_rn(a.lr, (float)max(1, ps.cur[3]));
   const float e = (float)((const ET*)ps.E)[s];
-  a.C[(size_t)ps.cur[0] * a.S + s] = a.reward ? e : __fmul_rn(__fmul_rn(scale, ps.L[a.post[s]]), e);
+  a.C[(size_t)ps.cur[0] * a.S + q] = a.reward ? e : __fmul_rn(__fmul_rn(scale, ps.L[a.post[s]]), e);
 }
 
 // ---------------------------------------------------------------------------
@@ -823,26 +990,29 @@ __global__ void k_reward_signal(TrainArgs a, RewardArgs r) {
   r.Ls[(size_t)b * a.n + j] = learning_signal(a, r.delta + (size_t)b * a.classes, j);
 }
 
-// C_{b,s} = (lr / T_b * L_b[post_s]) * float(E_{b,s}), in place over float(E).
+// C_{b,s} = (lr / T_b * L_b[post_s]) * float(E_{b,s}), in place over float(E)
+// (unit order: entry q is synapse ecol_syn[q]).
 __global__ void k_reward_contrib(TrainArgs a, RewardArgs r) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
-  if (s >= a.S) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (q >= a.S) return;
   const int T = a.lengths[a.idx[b]];
   const float scale = __fdiv_rn(a.lr, (float)max(1, T));
-  float* c = a.C + (size_t)b * a.S + s;
-  *c = __fmul_rn(__fmul_rn(scale, r.Ls[(size_t)b * a.n + a.post[s]]), *c);
+  float* c = a.C + (size_t)b * a.S + q;
+  *c = __fmul_rn(__fmul_rn(scale, r.Ls[(size_t)b * a.n + a.post[a.ecol_syn[q]]]), *c);
 }
 
 // ---------------------------------------------------------------------------
 // update
 // ---------------------------------------------------------------------------
 
-// w_s <- clamp(w_s - (sum_b C_{b,s}) / B, +-w_max), b ascending.
+// w_s <- clamp(w_s - (sum_b C_{b,s}) / B, +-w_max), b ascending; C in unit
+// order, so entry q updates synapse ecol_syn[q] (each synapse exactly once).
 __global__ void k_apply(TrainArgs a, float* w, int B) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.S) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.S) return;
   float acc = 0.0f;
-  for (int b = 0; b < B; ++b) acc = __fadd_rn(acc, a.C[(size_t)b * a.S + s]);
+  for (int b = 0; b < B; ++b) acc = __fadd_rn(acc, a.C[(size_t)b * a.S + q]);
+  const int s = a.ecol_syn[q];
   const float nw = __fsub_rn(w[s], __fdiv_rn(acc, (float)B));
   w[s] = fminf(fmaxf(nw, -a.w_max), a.w_max);
 }
@@ -869,12 +1039,18 @@ size_t train_lds_bytes(const TrainArgs& a, bool ring_lds) {
   return b;
 }
 
-int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
-                         hipStream_t st) {
-  const void* fn = f64 ? (ring_lds ? (const void*)k_train_batched<double, true>
-                                   : (const void*)k_train_batched<double, false>)
-                       : (ring_lds ? (const void*)k_train_batched<float, true>
-                                   : (const void*)k_train_batched<float, false>);
+int train_launch_batched(const TrainArgs& a, int B, int grid, bool f64, bool ring_lds,
+                         size_t lds_max, hipStream_t st) {
+  const bool prof = a.prof != nullptr;
+  const void* fn =
+      prof ? (f64 ? (ring_lds ? (const void*)k_train_batched<double, true, true>
+                              : (const void*)k_train_batched<double, false, true>)
+                  : (ring_lds ? (const void*)k_train_batched<float, true, true>
+                              : (const void*)k_train_batched<float, false, true>))
+           : (f64 ? (ring_lds ? (const void*)k_train_batched<double, true, false>
+                              : (const void*)k_train_batched<double, false, false>)
+                  : (ring_lds ? (const void*)k_train_batched<float, true, false>
+                              : (const void*)k_train_batched<float, false, false>));
   // The kernel's own static LDS counts against the same per-block limit.
   hipFuncAttributes attr{};
   const hipError_t eg = hipFuncGetAttributes(&attr, fn);
@@ -899,8 +1075,10 @@ int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, siz
   }
   TrainArgs aa = a;
   aa.lds_bytes = (int)lds;
+  aa.n_samples = B;
   void* args[] = {(void*)&aa};
-  const hipError_t e = hipLaunchKernel(fn, dim3(B), dim3(kTrainThreads), args, lds, st);
+  const hipError_t e = hipLaunchKernel(fn, dim3(std::max(grid, B)), dim3(kTrainThreads), args,
+                                       lds, st);
   if (e != hipSuccess) return hip_fail(e, "batched trainer launch");
   return SNC_OK;
 }

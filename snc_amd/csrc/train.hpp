@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "forward_plan.hpp"
@@ -51,7 +52,7 @@ struct TrainArgs {
   float* hist;         // [B][T_max][2][hn]: psi then trace of every neuron, per step
   int hn;              // history row length: n rounded up to a multiple of 4
   int lds_bytes;       // dynamic LDS of the batched launch
-  float* C;            // [B][S] contributions, or null (no update)
+  float* C;            // [B][S] contributions in unit order (entry q: synapse ecol_syn[q]), or null
   float* ring_g;       // [B][L][n] when the ring is not in LDS
   float* logits;       // [B][classes]
   double* loss;        // [B]
@@ -60,7 +61,33 @@ struct TrainArgs {
   unsigned long long* blk_events;  // [B]
   unsigned long long* neur_fire;   // [n] or null
   uint32_t* raster;                // [B][T_max][words] or null
+  unsigned long long* prof;        // [B][kTrainPhases] wall-clock ticks per phase, or null
+  // eligibility work units: {post << 5 | (count - 1), first index into ecol_*},
+  // each up to kEligPerThread synapses of one post in column order
+  const int2* eunit;
+  int n_units;
+  const int32_t* ecol_pre;  // [S] pre of each synapse, unit order
+  const int32_t* ecol_syn;  // [S] its CSR position
+  // eligibility work queue of the batched launch (see claim_item)
+  int n_samples;            // B: workgroups 0 .. B-1 run one sample each
+  int rounds;               // work items per sample: ceil(n_units / kTrainThreads)
+  long long wait_ticks;     // wall-clock ticks a workgroup polls for unpublished work
+  int* wq;                  // [2B]: published flag, then claimed rounds, per sample
+  float* Ls_g;              // [B][n] learning signals of each sample
 };
+
+// Threads of a batched-trainer workgroup (one sample per workgroup).
+constexpr int kTrainThreads = 1024;
+
+// Synapses of one post that one lane's eligibility unit holds in registers.
+constexpr int kEligPerThread = 16;
+
+// Phases of the batched kernel timed by its profiled instantiation:
+// 0 set-up, 1 LIF + trace + histories, 2 delivery bits, 3 bitmap prefix scan,
+// 4 arrival staging, 5 ordered fold + next injections, 6 end-of-sample
+// epilogue, 7 eligibility sums, 8 history chunks streamed into LDS,
+// 9 contribution stores, 10 claiming (and waiting for) eligibility work.
+constexpr int kTrainPhases = 11;
 
 // Device state of the plain path (one sample at a time).
 struct PlainState {
@@ -117,8 +144,17 @@ __host__ __device__ inline size_t train_elig_lds_bytes(int n, int steps) {
 }
 
 size_t train_lds_bytes(const TrainArgs& a, bool ring_lds);
-int train_launch_batched(const TrainArgs& a, int B, bool f64, bool ring_lds, size_t lds_max,
-                         hipStream_t st);
+// Workgroups of a batched launch of B samples: B, plus (when contributions
+// are computed) helpers on the CUs the samples leave free, no more than there
+// are eligibility items beyond one per sample.
+inline int train_batched_grid(const TrainArgs& a, int B, int n_cu) {
+  if (!a.C) return B;
+  const int helpers = std::max(0, std::min(n_cu - B, B * a.rounds - B));
+  return B + helpers;
+}
+// a.wq must be zero (2 * B ints) when a.C is set.
+int train_launch_batched(const TrainArgs& a, int B, int grid, bool f64, bool ring_lds,
+                         size_t lds_max, hipStream_t st);
 int train_launch_plain_begin(const PlainState& ps, int b, int sample, int epoch, int T,
                              hipStream_t st);
 // The step kernels of T steps and the end-of-sample kernels. The sample's own

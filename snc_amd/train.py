@@ -100,6 +100,9 @@ _SIGS = {
                                       C.c_int32, C.POINTER(TrainStatsC)]),
     "snc_hip_train_evaluate": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                          C.POINTER(TrainStatsC)]),
+    "snc_hip_train_profile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]),
     "snc_hip_train_reset_stats": (C.c_int, [C.c_void_p]),
     "snc_hip_train_collect_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64),
                                               C.POINTER(C.c_int64)]),
@@ -126,6 +129,13 @@ def _stats(st: TrainStatsC) -> EpochStats:
                       synaptic_events=st.synaptic_events,
                       energy=1.0 * st.spikes + 0.2 * st.synaptic_events, ms=st.ms,
                       samples=st.samples)
+
+
+# Phases of the batched kernel (SNC_TRAIN_PHASES, snc_hip_train_profile).
+PHASE_NAMES = ("setup", "lif_trace_history", "delivery_bits", "prefix_scan", "arrival_staging",
+               "fold_inject", "epilogue", "eligibility", "history_stream", "contrib_store",
+               "queue_wait")
+PHASES = len(PHASE_NAMES)
 
 
 class HipTrainSession:
@@ -213,6 +223,21 @@ class HipTrainSession:
     # structural statistics -----------------------------------------------------
     def reset_stats(self):
         L.check(_bind().snc_hip_train_reset_stats(self._h))
+
+    def profile(self, idx, epoch=0) -> dict:
+        """Phase split of one batched minibatch (snc_hip_train_profile): per
+        phase, the mean and largest time (us) over the minibatch's workgroups,
+        and the launch's device time; no update is applied."""
+        i = np.ascontiguousarray(idx, np.int32)
+        mean = np.zeros(PHASES, np.float64)
+        mx = np.zeros(PHASES, np.float64)
+        ms = C.c_double()
+        L.check(_bind().snc_hip_train_profile(self._h, L.ptr(i, C.c_int32), i.size, int(epoch),
+                                              L.ptr(mean, C.c_double), L.ptr(mx, C.c_double),
+                                              C.byref(ms)))
+        return {"kernel_ms": ms.value,
+                "phases": {name: {"mean_us": float(mean[k]), "max_us": float(mx[k])}
+                           for k, name in enumerate(PHASE_NAMES)}}
 
     def collect_stats(self):
         syn = np.zeros(self.graph.num_synapses, np.int64)

@@ -284,3 +284,26 @@ def test_epochs_learn_prototype_classes():
         assert st.samples == 400
     acc1 = sess.evaluate(te)
     assert acc1 >= 0.4 and acc1 >= acc0 + 0.2, (acc0, acc1)
+
+
+def test_phase_profile_applies_nothing_and_accounts_for_the_launch():
+    """snc_hip_train_profile: the timed build of the batched kernel leaves the
+    weights as they were, every phase is non-negative, and the slowest
+    workgroup's phases together fit inside the launch's device time."""
+    g = G.sheet_graph([100, 64, 10], 16, recurrent_fan_in=8, radius=3.0, sheet=10.0, w_std=0.3,
+                      max_delay=5, seed=2)
+    B = 16
+    X, y = I.prototype_images(B, 100, 10, 0.6, 0.2, seed=4)
+    Xf = X.astype(np.float32) / np.float32(255.0)
+    lengths = np.random.default_rng(1).integers(10, 40, B).astype(np.int32)
+    gcfg, _ = _cfgs(40, readout=READOUT_LEAKY, reset_mode=1, encoder=1, max_rate=0.4)
+    s = HipTrainSession(g, gcfg, batch=B, max_steps=40)
+    s.upload_features(Xf, y, lengths)
+    w0 = s.get_weights()
+    p = s.profile(np.arange(B), epoch=1)
+    assert np.array_equal(s.get_weights(), w0)
+    ph = p["phases"]
+    assert all(v["mean_us"] >= 0 and v["max_us"] >= v["mean_us"] for v in ph.values())
+    assert ph["lif_trace_history"]["mean_us"] > 0 and ph["eligibility"]["mean_us"] > 0
+    assert sum(v["mean_us"] for v in ph.values()) <= p["kernel_ms"] * 1e3 * 1.05
+    s.close()
