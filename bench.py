@@ -240,13 +240,10 @@ def eprop_bench(w, epochs, plain_batches, elig, cpu_samples=0):
         spk_b, ev_b, st_b = spk_b + r[3], ev_b + r[4], st_b + r[5]
         nmb += len(bl)
         curve.append(sb.evaluate(te))
-    # Algorithmic bytes of the batched steps (per sample-step 28 N + 8 S, per
-    # event 9, per spike 8, per minibatch update 4 S B + 8 S) over their
-    # device time, against the 8 TB/s HBM peak.
     N, S = w.graph.num_neurons, w.graph.num_synapses
-    model = (28.0 * N + 8.0 * S) * st_b + 9.0 * ev_b + 8.0 * spk_b + (4.0 * S * w.batch + 8.0 * S) * nmb
-    gbs = model / (dev_b / 1e3) / 1e9
+    phases = eprop_phases_of(sb, w, batches, elig, 2)
     sb.close()
+    roofline = eprop_roofline(w, elig, dev_b, nmb, st_b, ev_b, spk_b, phases)
     bsps = n_b / wall_b
     out = {
         "workload": w.name, "neurons": w.graph.num_neurons, "synapses": w.graph.num_synapses,
@@ -257,13 +254,8 @@ def eprop_bench(w, epochs, plain_batches, elig, cpu_samples=0):
                     "samples": n_b, "epochs": epochs, "GSynOps/s": ev_b / wall_b / 1e9,
                     "test_acc_per_epoch": curve},
         "test_acc_initial": acc0,
-        "roofline": {"bound": "hbm", "kernel": "k_train_batched + k_apply",
-                     "bytes_model": "(28N + 8S) per sample-step + 9 per event + 8 per spike "
-                                    "+ (4SB + 8S) per minibatch",
-                     "bytes": model, "device_ms": dev_b, "achieved": gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "note": "eligibility stays in registers: the 8S per sample-step of the "
-                             "model is never moved, so frac can exceed what HBM carries"},
+        "roofline": roofline,
+        "phases": phases,
     }
     if nb > 0:
         psps = n_p / wall_p
@@ -322,30 +314,114 @@ def eprop_cpu_baseline(w, n_samples):
             "seconds": dt}
 
 
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 vector / matrix
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32
+MI355X_CUS = 256
+
+
+def eprop_roofline(w, elig, dev_ms, minibatches, sample_steps, events, spikes, phases):
+    """The batched e-prop kernel against its ceilings. Its eligibility sums
+    stay on chip (registers and LDS), so HBM is not what binds it:
+      hbm            counted HBM bytes per training dispatch of k_train_batched
+                     (calibrated FETCH_SIZE + WRITE_SIZE from the committed
+                     counter summary) over the measured mean device time
+      eligibility    2 flops per synapse per sample-step (one fused multiply-add
+                     of E_s += psi_post * tr_pre, f64 or f32) against the dense
+                     peak: over the whole minibatch's device time, and over the
+                     CU time the sums phase itself took (phase split)
+      model_bytes    the algorithmic byte model (28 N + 8 S per sample-step, ...),
+                     which charges 8 bytes per synapse-step of eligibility that
+                     never leaves the chip -- reported, not the bound"""
+    N, S = w.graph.num_neurons, w.graph.num_synapses
+    f64 = elig == 1
+    peak = FP64_PEAK_TFLOPS if f64 else FP32_PEAK_TFLOPS
+    ms_mb = dev_ms / max(1, minibatches)
+    flops = 2.0 * S * sample_steps
+    tflops = flops / (dev_ms / 1e3) / 1e12
+    # the sums phase: flops of the profiled minibatches over the CU time the
+    # phase took in them, scaled to the whole chip (MI355X_CUS CUs)
+    sums_cu_ms = phases["phases_mean_us"]["eligibility"] * w.batch / 1e3 * phases["minibatches"]
+    prof_flops = 2.0 * S * phases["sample_steps"]
+    phase_tflops = (prof_flops / (sums_cu_ms / 1e3) / 1e12 * MI355X_CUS
+                    if sums_cu_ms > 0 else None)
+    ring = "true" if w.name.startswith("c2") else "false"
+    sym = f"snc_amd::k_train_batched<{'double' if f64 else 'float'}, {ring}, false>"
+    traffic = measured_traffic(sym)
+    model = ((28.0 * N + 8.0 * S) * sample_steps + 9.0 * events + 8.0 * spikes
+             + (4.0 * S * w.batch + 8.0 * S) * minibatches)
+    out = {"kernel": sym, "device_ms_per_minibatch": ms_mb,
+           "eligibility": {"unit": "TFLOP/s", "flops": flops, "achieved": tflops, "peak": peak,
+                           "frac": tflops / peak, "precision": "f64" if f64 else "f32",
+                           "sums_phase_cu_ms": sums_cu_ms,
+                           "sums_phase_achieved": phase_tflops,
+                           "sums_phase_frac": phase_tflops / peak if phase_tflops else None},
+           "model_bytes": {"bytes": model, "gbs": model / (dev_ms / 1e3) / 1e9,
+                           "frac": model / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                           "note": "byte model of the reference's design (eligibility in HBM); "
+                                   "this kernel keeps eligibility on chip, so this is not its "
+                                   "bound"}}
+    if traffic:
+        kern = [v for v in phases["kernel_ms"] if v > 0]
+        k_ms = sum(kern) / len(kern) if kern else ms_mb
+        gbs = traffic["bytes_per_launch"] / (k_ms / 1e3) / 1e9
+        out["hbm"] = {"unit": "GB/s", "traffic_bytes_per_dispatch": traffic["bytes_per_launch"],
+                      "source": traffic["source"], "kernel_ms": k_ms, "achieved": gbs,
+                      "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
+def eprop_phases_of(s, w, batches, elig, minibatches):
+    """Phase split (snc_hip_train_profile) of the first minibatches on an open
+    session: per phase, the mean time per sample and the largest time one
+    workgroup spent in it, and each launch's device time."""
+    import numpy as np
+
+    from snc_amd.train import PHASE_NAMES
+
+    runs = [s.profile(b, epoch=1) for b in batches[:minibatches]]
+    out = {"minibatches": len(runs), "kernel_ms": [r["kernel_ms"] for r in runs],
+           "sample_steps": int(sum(int(w.lengths[b].sum()) for b in batches[:minibatches])),
+           "max_steps_per_minibatch": [int(w.lengths[b].max()) for b in batches[:minibatches]],
+           "phases_mean_us": {k: float(np.mean([r["phases"][k]["mean_us"] for r in runs]))
+                              for k in PHASE_NAMES},
+           "phases_max_us": {k: float(np.mean([r["phases"][k]["max_us"] for r in runs]))
+                             for k in PHASE_NAMES}}
+    log(f"e-prop phases {w.name} ({'f64' if elig == 1 else 'f32'}): kernel {out['kernel_ms']} "
+        "ms; mean us " + ", ".join(f"{k} {v:.0f}" for k, v in out["phases_mean_us"].items()))
+    return out
+
+
+def eprop_train_only(w, elig, minibatches):
+    """Only training minibatches (no evaluation, no warm-up launch without
+    update), for counter runs: every k_train_batched dispatch is a training
+    dispatch of the workload's minibatch size."""
+    import numpy as np
+
+    from snc_amd.train import BATCHED
+
+    order = np.random.default_rng(12345).permutation(np.arange(w.n_train, dtype=np.int32))
+    s = _train_session(w, elig)
+    for i in range(minibatches):
+        s.step(order[i * w.batch:(i + 1) * w.batch], epoch=1, mode=BATCHED, apply=True)
+    s.close()
+    log(f"e-prop {w.name}: {minibatches} training minibatches")
+
+
 def eprop_phases(w, elig, minibatches=3):
     """Where the batched e-prop kernel's time goes on one workload: the phase
     split (snc_hip_train_profile) of the first `minibatches` training
     minibatches of epoch 1, after one untimed warm-up launch."""
     import numpy as np
 
-    from snc_amd.train import BATCHED, PHASE_NAMES
+    from snc_amd.train import BATCHED
 
     order = np.random.default_rng(12345).permutation(np.arange(w.n_train, dtype=np.int32))
     batches = [order[i:i + w.batch].astype(np.int32) for i in range(0, order.size, w.batch)]
     s = _train_session(w, elig)
     s.step(batches[0], epoch=0, mode=BATCHED, apply=False)
-    runs = [s.profile(b, epoch=1) for b in batches[:minibatches]]
+    out = {"workload": w.name, "batch": w.batch, "eligibility": "f64" if elig == 1 else "f32"}
+    out.update(eprop_phases_of(s, w, batches, elig, minibatches))
     s.close()
-    out = {"workload": w.name, "batch": w.batch, "eligibility": "f64" if elig == 1 else "f32",
-           "minibatches": len(runs),
-           "kernel_ms": [r["kernel_ms"] for r in runs],
-           "max_steps_per_minibatch": [int(w.lengths[b].max()) for b in batches[:minibatches]],
-           "phases_mean_us": {k: float(np.mean([r["phases"][k]["mean_us"] for r in runs]))
-                              for k in PHASE_NAMES},
-           "phases_max_us": {k: float(np.mean([r["phases"][k]["max_us"] for r in runs]))
-                             for k in PHASE_NAMES}}
-    log(f"e-prop phases {w.name} ({out['eligibility']}): kernel {out['kernel_ms']} ms; mean us "
-        + ", ".join(f"{k} {v:.0f}" for k, v in out["phases_mean_us"].items()))
     return out
 
 
@@ -433,6 +509,9 @@ def main():
     ap.add_argument("--no-grow", action="store_true",
                     help="skip the learned-vs-heuristic grow comparison")
     ap.add_argument("--eprop-only", action="store_true")
+    ap.add_argument("--eprop-train-only", type=int, default=0, metavar="K",
+                    help="for counter runs: K training minibatches of each training workload "
+                         "(f64 eligibility), nothing else, then exit")
     ap.add_argument("--eprop-phases", action="store_true",
                     help="print the batched e-prop kernel's phase split on both training "
                          "workloads (f64 and f32 eligibility) and exit")
@@ -513,6 +592,10 @@ def main():
         return
     if args.eprop_only:
         print(json.dumps({"eprop": eprop_all()}))
+        return
+    if args.eprop_train_only:
+        for mk in (K.c2, K.c3):
+            eprop_train_only(mk(), 1, args.eprop_train_only)
         return
     if args.eprop_phases:
         print(json.dumps({"eprop_phases": [eprop_phases(mk(), el) for mk in (K.c2, K.c3)
