@@ -12,8 +12,10 @@
 //     measured by the trial rather than modelled.
 //   * hip-sort is tried once (its cost follows the events, not the shape),
 //     for single-sample plans.
-//   * persistent variants only when cooperative launch is available, for
-//     single-sample plans.
+//   * no persistent (cooperative whole-run) variant: its grid-wide barrier
+//     per step cost more than a graph-launched step on every measured graph
+//     (delivery stress: 29 vs 7.3 us per step with float atomics, 55 vs 12.1
+//     with buckets), so it runs only when a plan asks for it explicitly.
 //   * hip-pull for batched plans of up to 32 samples, at the 99th-percentile
 //     in-degree cap and at twice it.
 #include "autotune.hpp"
@@ -46,7 +48,7 @@ snc_plan_config default_plan_config(const DevGraph& g, int variant) {
   return c;
 }
 
-std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only, bool coop,
+std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only,
                                                  bool allow_sort, int num_samples) {
   std::vector<snc_plan_config> out;
   auto add = [&](int variant, int block, int thr, int persistent) {
@@ -77,10 +79,6 @@ std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_o
     for (int blk : {128, 256, 512}) add(SNC_VARIANT_ATOMIC, blk, kLightLimit, 0);
   if (allow_sort && (uint64_t)g.n * (uint64_t)g.L < (1ull << 32))
     add(SNC_VARIANT_SORT, 256, kLightLimit, 0);
-  if (coop) {
-    add(SNC_VARIANT_BUCKET, 256, kLightLimit, 1);
-    if (!exact_only) add(SNC_VARIANT_ATOMIC, 256, kLightLimit, 1);
-  }
   if (!exact_only && num_samples > 1 && num_samples <= 32) {
     const int cap = pull_default_cap(g);
     add(SNC_VARIANT_PULL, 256, cap, 0);

@@ -1049,8 +1049,7 @@ static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_
   }
   const auto t0 = std::chrono::steady_clock::now();
   const std::vector<snc_plan_config> cands =
-      autotune_candidates(G->g, exact_only, num_samples == 1 && persist_supported(),
-                          num_samples == 1, num_samples);
+      autotune_candidates(G->g, exact_only, num_samples == 1, num_samples);
   snc_tune_report rep{};
   rep.trial_steps = autotune_trial_steps(num_steps);
   int best = -1;

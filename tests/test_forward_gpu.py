@@ -270,6 +270,8 @@ def test_autotuner_search_picks_fastest_and_stays_exact():
     ms = [m for _, m in rep["candidates"]]
     assert len(ms) >= 4 and rep["trial_steps"] == 128
     assert all(c.variant != Backend.HipAtomic for c, _ in rep["candidates"])
+    assert not any(c.persistent for c, _ in rep["candidates"])  # forced plans only
+    assert rep["failed"] == []
     ran = [m for m in ms if m >= 0]
     assert ms[rep["chosen"]] == min(ran)
     assert plan.config == rep["candidates"][rep["chosen"]][0]
