@@ -77,8 +77,7 @@ std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_o
 
   if (!exact_only)
     for (int blk : {128, 256, 512}) add(SNC_VARIANT_ATOMIC, blk, kLightLimit, 0);
-  if (allow_sort && (uint64_t)g.n * (uint64_t)g.L < (1ull << 32))
-    add(SNC_VARIANT_SORT, 256, kLightLimit, 0);
+  if (allow_sort) add(SNC_VARIANT_SORT, 256, kLightLimit, 0);
   if (!exact_only && num_samples > 1 && num_samples <= 32) {
     const int cap = pull_default_cap(g);
     add(SNC_VARIANT_PULL, 256, cap, 0);

@@ -87,7 +87,8 @@ DevGraph::~DevGraph() {
 using namespace snc_amd;
 
 struct snc_hip_plan;
-static int sort_alloc(snc_hip_plan* P);
+static int sort_alloc(snc_hip_plan* P, const std::vector<int32_t>& inj_ptr,
+                      const std::vector<int32_t>& inj_nid);
 static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_events,
                            int num_samples, int num_steps, const snc_lif* lif, bool exact_only,
                            snc_plan_config* out);
@@ -131,9 +132,10 @@ struct snc_hip_plan {
   size_t dirty_stride = 0;
   size_t bits_total = 0, dirty_total = 0, ring_total = 0;  // elements allocated
   std::vector<int64_t> last_spikes, last_events;           // per sample, last run
-  uint32_t *keys = nullptr, *keys_alt = nullptr;
-  int32_t *vals = nullptr, *vals_alt = nullptr, *hist = nullptr, *ev_ptr = nullptr;
-  size_t sort_capacity = 0;
+  int4* sseg = nullptr;  // hip-sort segments, each synapse's segment, lists, counters
+  int32_t *sseg_e = nullptr, *slist = nullptr, *scnt = nullptr, *sinj_ptr = nullptr;
+  int n_seg = 0;
+  std::vector<int32_t> scnt_init;  // scnt at reset: each counter at its segment's first column position
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // [raster off, on]
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -173,32 +175,46 @@ struct snc_hip_plan {
     dfree(inj_heavy);
     dfree(heavy_ids);
     dfree(is_heavy);
-    dfree(keys);
-    dfree(keys_alt);
-    dfree(vals);
-    dfree(vals_alt);
-    dfree(hist);
-    dfree(ev_ptr);
+    dfree(sseg);
+    dfree(sseg_e);
+    dfree(slist);
+    dfree(scnt);
+    dfree(sinj_ptr);
   }
 };
 
-// Scratch of the hip-sort strategy, sized for the worst step (every neuron
-// fires): S event keys and values, double-buffered, plus offsets.
-static int sort_alloc(snc_hip_plan* P) {
+// State of the hip-sort strategy: the target segments, their event lists
+// and counters per step parity, and each step's injections per segment. A
+// segment's list holds at most one event per column position of the segment
+// per step (a pre fires at most once per step), so the lists of one parity
+// share one P-sized array, segment s at its first column position.
+static int sort_alloc(snc_hip_plan* P, const std::vector<int32_t>& inj_ptr,
+                      const std::vector<int32_t>& inj_nid) {
   const DevGraph& g = P->G->g;
-  const size_t cap = (size_t)std::max<int64_t>(1, g.S);
-  P->sort_capacity = cap;
-  int rc = SNC_OK;
-  if (!rc) rc = dalloc(&P->keys, cap);
-  if (!rc) rc = dalloc(&P->keys_alt, cap);
-  if (!rc) rc = dalloc(&P->vals, cap);
-  if (!rc) rc = dalloc(&P->vals_alt, cap);
-  if (!rc) rc = dalloc(&P->hist, sort_hist_words(g.n));
-  if (!rc) rc = dalloc(&P->ev_ptr, (size_t)g.n + 2);
-  if (!rc) rc = dalloc(&P->fbits, ((size_t)g.n + 31) / 32);
-  int bits = 1;
-  while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)g.n * (uint64_t)g.L) ++bits;
-  P->core.sort_bits = bits;
+  std::vector<int32_t> ccol((size_t)g.n + 1, 0);
+  for (int j = 0; j < g.n; ++j) ccol[j + 1] = ccol[j] + (g.in_degree[j] + 31) / 32 * 32;
+  std::vector<int4> seg;
+  std::vector<int32_t> sow;
+  int rc = sort_prepare();
+  if (!rc) rc = sort_build_segments(ccol, g.n, seg, sow);
+  if (rc) return rc;
+  // each synapse's segment, from its column position (host copy of cpos)
+  std::vector<int32_t> cpos((size_t)std::max<int64_t>(1, g.S), 0), sse((size_t)std::max<int64_t>(1, g.S), 0);
+  if (g.S > 0 && hipMemcpy(cpos.data(), g.cpos, sizeof(int32_t) * (size_t)g.S,
+                           hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(SNC_E_HIP, "column positions readback failed");
+  for (int64_t e = 0; e < g.S; ++e) sse[(size_t)e] = sow[(size_t)(cpos[(size_t)e] >> 5)];
+  P->n_seg = (int)seg.size();
+  if (!rc) rc = dupload(&P->sseg, seg.data(), seg.size());
+  if (!rc) rc = dupload(&P->sseg_e, sse.data(), sse.size());
+  if (!rc) rc = dalloc(&P->slist, 2 * (size_t)std::max<int64_t>(1, g.P));
+  if (!rc) rc = dalloc(&P->scnt, 2 * (size_t)std::max(1, P->n_seg) * kSortCntStride);
+  P->scnt_init.assign(2 * (size_t)P->n_seg * kSortCntStride, 0);
+  for (int q = 0; q < P->n_seg; ++q)
+    P->scnt_init[(size_t)q * kSortCntStride] =
+        P->scnt_init[(size_t)(P->n_seg + q) * kSortCntStride] = seg[q].z;
+  const std::vector<int32_t> sinj = sort_injection_ranges(seg, inj_ptr, inj_nid, P->T);
+  if (!rc) rc = dupload(&P->sinj_ptr, sinj.data(), sinj.size());
   return rc;
 }
 
@@ -537,9 +553,15 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   const bool fused = !P->persistent && variant != SNC_VARIANT_SORT && !pull;
   if (!rc && fused && num_steps > 0)
     rc = dalloc(&P->inj_tab, (size_t)B * num_steps * R * kInjInline);
-  if (variant == SNC_VARIANT_SORT && (uint64_t)n * (uint64_t)g.L >= (1ull << 32))
-    return fail(SNC_E_UNSUPPORTED, "hip-sort keys need n * ring_len < 2^32");
-  if (!rc && variant == SNC_VARIANT_SORT) rc = sort_alloc(P.get());
+  if (!rc && variant == SNC_VARIANT_SORT) rc = sort_alloc(P.get(), ip, inid);
+  if (!rc && variant == SNC_VARIANT_SORT && (size_t)P->n_seg > P->n_blk_slots) {
+    // stats slots: one per segment workgroup
+    dfree(P->blk_spikes);
+    dfree(P->blk_events);
+    P->n_blk_slots = (size_t)P->n_seg;
+    rc = dalloc(&P->blk_spikes, (size_t)B * P->n_blk_slots);
+    if (!rc) rc = dalloc(&P->blk_events, (size_t)B * P->n_blk_slots);
+  }
   if (!rc && P->persistent) {
     const std::vector<int32_t> hp = persist_heavy_ptr(g, P->heavy_host, P->pcfg);
     rc = dupload(&P->heavy_blk_ptr, hp.data(), hp.size());
@@ -584,12 +606,14 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
   a.blk_spikes = P->blk_spikes;
   a.blk_events = P->blk_events;
   a.raster = nullptr;
-  a.keys = P->keys;
-  a.keys_alt = P->keys_alt;
-  a.vals = P->vals;
-  a.vals_alt = P->vals_alt;
-  a.hist = P->hist;
-  a.ev_ptr = P->ev_ptr;
+  a.sseg = P->sseg;
+  a.sseg_e = P->sseg_e;
+  a.slist = P->slist;
+  a.scnt = P->scnt;
+  a.slist_stride = g.P;
+  a.sinj_ptr = P->sinj_ptr;
+  a.n_seg = P->n_seg;
+  a.sort_slot0 = 0;
   a.fbits = P->fbits;
   a.inj_blk_ptr = P->inj_blk_ptr;
   a.blk_nid = P->blk_nid;
@@ -770,6 +794,12 @@ int reset_state(snc_hip_plan* P, bool raster) {
   fill(P->ring, sizeof(float) * P->ring_total);
   fill(P->refr, sizeof(int32_t) * B * n);
   fill(P->fcnt, sizeof(int32_t) * 2);
+  if (P->scnt) {  // hip-sort: each counter starts at its segment's list start
+    const hipError_t ec = hipMemcpyAsync(P->scnt, P->scnt_init.data(),
+                                         sizeof(int32_t) * P->scnt_init.size(),
+                                         hipMemcpyHostToDevice, st);
+    if (!rc && ec != hipSuccess) rc = hip_fail(ec, "segment counter reset");
+  }
   fill(P->nspk, sizeof(int32_t) * B * n);
   fill(P->dirty, P->dirty_total);
   fill(P->bits, sizeof(uint32_t) * P->bits_total);
@@ -777,8 +807,7 @@ int reset_state(snc_hip_plan* P, bool raster) {
   fill(P->blk_spikes, sizeof(unsigned long long) * B * P->n_blk_slots);
   fill(P->blk_events, sizeof(unsigned long long) * B * P->n_blk_slots);
   if (raster) fill(P->raster, sizeof(uint32_t) * B * (size_t)P->T * P->a.words);
-  if (!rc && (P->persistent || P->variant == SNC_VARIANT_SORT))
-    rc = fwd_launch_first_injection(P->core, P->a);
+  if (!rc && P->persistent) rc = fwd_launch_first_injection(P->core, P->a);
   return rc;
 }
 

@@ -11,6 +11,38 @@
 
 namespace snc_amd {
 
+// Appends the wave's fired neurons to the fired list, records the raster
+// words and returns the wave's spike count (valid in every lane).
+__device__ __forceinline__ int wave_emit_fired(bool f, int i, int32_t* __restrict__ fired,
+                                               int32_t* __restrict__ fcnt,
+                                               uint32_t* __restrict__ raster_row, int words) {
+  const uint64_t m = ballot(f);
+  if (m == 0) return 0;
+  const int cnt = __popcll(m);
+  const int lane = lane_id();
+  int base = 0;
+  if (lane == 0) base = atomicAdd(fcnt, cnt);
+  base = __shfl(base, 0, kWave);
+  if (f) fired[base + rank_below(m)] = i;
+  if (raster_row && lane == 0) {
+    const int w0 = (i - lane) >> 5;  // the wave covers 64 consecutive ids
+    if ((uint32_t)m) raster_row[w0] = (uint32_t)m;
+    if ((uint32_t)(m >> 32) && w0 + 1 < words) raster_row[w0 + 1] = (uint32_t)(m >> 32);
+  }
+  return cnt;
+}
+
+// Block reduction of per-wave counts (lane 0's value of each wave) into the
+// block's own stats slot. Every thread of the block must call it.
+__device__ __forceinline__ void block_accumulate(int wave_count, unsigned long long* slot) {
+  __shared__ int s_sum;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  if (lane_id() == 0 && wave_count) atomicAdd(&s_sum, wave_count);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_sum) *slot += (unsigned long long)s_sum;
+}
+
 // One LIF step (cpu_backend.cpp:129-143 with the reset-mode extension).
 __device__ __forceinline__ bool lif_step(float& v, int& refr, float ext, float slot,
                                          const LifK& p) {

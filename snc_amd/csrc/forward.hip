@@ -1,6 +1,6 @@
-// Shared forward kernels: fills, injection staging, and the integrate/fire
-// kernel of the hip-sort strategy (sort.hip). hip-atomic and hip-bucket run
-// the fused step kernels of step.hip; the persistent form is persist.hip.
+// Shared forward kernels: fills and injection staging. hip-atomic and
+// hip-bucket run the fused step kernels of step.hip, hip-sort the kernels of
+// sort.hip; the persistent form is persist.hip.
 //
 // Semantics follow the reference CPU path step for step
 // (src/runtime/cpu_backend.cpp:95-183):
@@ -78,67 +78,6 @@ __global__ void k_inj_table(const int32_t* __restrict__ blk_ptr, const int32_t* 
   }
 }
 
-// Appends the wave's fired neurons to the fired list, records the raster
-// words and returns the wave's spike count (valid in every lane).
-__device__ __forceinline__ int wave_emit_fired(bool f, int i, int32_t* __restrict__ fired,
-                                               int32_t* __restrict__ fcnt,
-                                               uint32_t* __restrict__ raster_row, int words) {
-  const uint64_t m = ballot(f);
-  if (m == 0) return 0;
-  const int cnt = __popcll(m);
-  const int lane = lane_id();
-  int base = 0;
-  if (lane == 0) base = atomicAdd(fcnt, cnt);
-  base = __shfl(base, 0, kWave);
-  if (f) fired[base + rank_below(m)] = i;
-  if (raster_row && lane == 0) {
-    const int w0 = (i - lane) >> 5;  // the wave covers 64 consecutive ids
-    if ((uint32_t)m) raster_row[w0] = (uint32_t)m;
-    if ((uint32_t)(m >> 32) && w0 + 1 < words) raster_row[w0 + 1] = (uint32_t)(m >> 32);
-  }
-  return cnt;
-}
-
-// Block reduction of per-wave counts into the block's own stats slot.
-__device__ __forceinline__ void block_accumulate(int wave_count, unsigned long long* slot) {
-  __shared__ int s_sum;
-  if (threadIdx.x == 0) s_sum = 0;
-  __syncthreads();
-  if (lane_id() == 0 && wave_count) atomicAdd(&s_sum, wave_count);
-  __syncthreads();
-  if (threadIdx.x == 0 && s_sum) *slot += (unsigned long long)s_sum;
-}
-
-// Integrate/fire for every neuron (step t, slot tm = t % L), hip-sort: also
-// writes the step's fired bitmap, which k_sort_compact turns into the
-// ordered fired list.
-__global__ __launch_bounds__(256) void k_lif(StepArgs a, int t, int tm) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  bool f = false;
-  if (i < a.n) {
-    float* slotp = a.ring + (size_t)tm * a.n + i;
-    float v = a.v[i];
-    int r = a.refr[i];
-    f = lif_step(v, r, a.ext[i], *slotp, a.lif);
-    a.v[i] = v;
-    a.refr[i] = r;
-    a.ext[i] = 0.0f;
-    *slotp = 0.0f;
-    if (f) a.nspk[i] += 1;
-  }
-  if (a.fbits) {  // every wave rewrites its own words, zeros included
-    const uint64_t m = ballot(f);
-    const int first = i - lane_id();
-    if (lane_id() == 0 && first < a.n) {
-      a.fbits[first >> 5] = (uint32_t)m;
-      if ((first >> 5) + 1 < a.words) a.fbits[(first >> 5) + 1] = (uint32_t)(m >> 32);
-    }
-  }
-  const int cnt = wave_emit_fired(f, i, a.fired, a.fcnt + (t & 1),
-                                  a.raster ? a.raster + (size_t)t * a.words : nullptr, a.words);
-  block_accumulate(cnt, a.blk_spikes + blockIdx.x);
-}
-
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -159,11 +98,6 @@ int fwd_launch_fill(hipStream_t st, void* p, uint32_t value, size_t words) {
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int64_t)words, 256, 2048)), dim3(256), 0, st,
                      (uint32_t*)p, value, words);
   return hip_last("fill launch failed");
-}
-
-int fwd_launch_lif(const FwdPlanCore& pc, const StepArgs& a, int t) {
-  hipLaunchKernelGGL(k_lif, dim3(pc.lif_grid), dim3(256), 0, pc.stream, a, t, t % a.L);
-  return hip_last("LIF launch failed");
 }
 
 int fwd_launch_first_injection(const FwdPlanCore& pc, const StepArgs& a) {

@@ -26,6 +26,11 @@ constexpr int kInjInline = 16;
 constexpr int kInjSpill = -2;
 
 // Everything a step kernel touches; passed by value.
+// Ints between two hip-sort list counters: each counter has a 128-byte
+// cache line of its own, so the expansion's appends to different segments
+// do not queue on shared lines.
+constexpr int kSortCntStride = 32;
+
 struct StepArgs {
   int n, L, T, words;
   LifK lif;
@@ -57,13 +62,15 @@ struct StepArgs {
   unsigned long long* blk_events;
   uint32_t* raster;  // [T][words] or null
   uint32_t* fbits;   // [words] fired bitmap of the current step (hip-sort) or null
-  // sort strategy scratch
-  uint32_t* keys;
-  uint32_t* keys_alt;
-  int32_t* vals;
-  int32_t* vals_alt;
-  int32_t* hist;
-  int32_t* ev_ptr;
+  // hip-sort: target segments and their event lists, by step parity
+  const int4* sseg;            // [n_seg] {first target, end target, first position, end position}
+  const int32_t* sseg_e;       // [S] segment of each synapse's target
+  int32_t* slist;              // [2][P] event lists: segment s's at sseg[s].z
+  int64_t slist_stride;        // P
+  int32_t* scnt;               // [2][n_seg][kSortCntStride]: next free list position per step parity and segment
+  const int32_t* sinj_ptr;     // [T][n_seg + 1] each step's injections, per segment
+  int n_seg;
+  int sort_slot0;              // first stats slot of the segment workgroups
   // fused step kernel
   const int32_t* inj_blk_ptr;  // [T][n_reg_blocks + 1] per-(step, block) injection ranges
   const int32_t* blk_nid;      // injections in (step, block) order, heavy targets excluded
@@ -115,11 +122,9 @@ struct FwdPlanCore {
   int lif_grid = 1, expand_grid = 1;
   int n_heavy = 0;
   int64_t max_inj_per_step = 0;
-  int sort_bits = 0;  // key bits of hip-sort: ceil(log2(n * L))
 };
 
 int fwd_launch_fill(hipStream_t st, void* p, uint32_t value, size_t words);
-int fwd_launch_lif(const FwdPlanCore& pc, const StepArgs& a, int t);  // hip-sort
 int fwd_launch_first_injection(const FwdPlanCore& pc, const StepArgs& a);
 // Fills a.inj_tab from the per-(sample, step, block) lists.
 int fwd_launch_inj_table(hipStream_t st, const StepArgs& a);
@@ -129,8 +134,17 @@ int fused_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t);
 bool fused_block_ok(int block);
 int fused_light_max_in_degree();  // largest in-degree the step kernel folds per lane
 
-// hip-sort (sort.hip)
-size_t sort_hist_words(int n);
+// hip-sort (sort.hip). Segments: runs of targets starting on multiples of 32,
+// at most 1024 targets and 65536 column positions each (a longer single
+// group of 32 gets its own segment), from the padded column starts ccol [n + 1].
+int sort_build_segments(const std::vector<int32_t>& ccol, int n, std::vector<int4>& seg,
+                        std::vector<int32_t>& seg_of_word);
+// [T][n_seg + 1]: where each step's (neuron-sorted) injections of each segment
+// start in inj_nid, from the per-step ranges inj_ptr [T + 1].
+std::vector<int32_t> sort_injection_ranges(const std::vector<int4>& seg,
+                                           const std::vector<int32_t>& inj_ptr,
+                                           const std::vector<int32_t>& inj_nid, int T);
 int sort_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t);
+int sort_prepare();  // the step kernel's LDS request
 
 }  // namespace snc_amd
