@@ -44,7 +44,7 @@ extern "C" {
  *      extended with the hip-* strategies) ---------------------------------- */
 #define SNC_VARIANT_ATOMIC 5 /* hip-atomic: float atomics into the ring      */
 #define SNC_VARIANT_BUCKET 6 /* hip-bucket: per-target ordered fold, bit-exact */
-#define SNC_VARIANT_SORT 7   /* hip-sort: canonical expand + stable radix sort */
+#define SNC_VARIANT_SORT 7   /* hip-sort: per-segment counting sort of each step's events */
 #define SNC_VARIANT_AUTO 8   /* hip-auto: autotuner picks per graph           */
 #define SNC_VARIANT_AUTO_EXACT 9 /* hip-auto-exact: autotuner over the bit-exact
                                     strategies (bucket, sort) only            */
@@ -371,8 +371,6 @@ int snc_hip_train_epoch(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t
 int snc_hip_train_evaluate(snc_hip_train* s, const int32_t* idx, int32_t n, int32_t mode,
                            snc_train_stats* st);
 
-/* cudatrain::reset_stats / collect_stats: per-synapse deliveries and
- * per-neuron fires accumulated over training steps (not evaluation). */
 /* Where the batched kernel's time goes (no counterpart in the reference: an
  * observability measurement). Runs one batched minibatch of dataset samples
  * idx[0..B) with a timed build of the kernel, computing the contributions but
@@ -389,6 +387,8 @@ int snc_hip_train_evaluate(snc_hip_train* s, const int32_t* idx, int32_t n, int3
 int snc_hip_train_profile(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t epoch,
                           double* mean_us, double* max_us, double* kernel_ms);
 
+/* cudatrain::reset_stats / collect_stats: per-synapse deliveries and
+ * per-neuron fires accumulated over training steps (not evaluation). */
 int snc_hip_train_reset_stats(snc_hip_train* s);
 int snc_hip_train_collect_stats(const snc_hip_train* s, int64_t* syn_deliveries,
                                 int64_t* neuron_fires);
