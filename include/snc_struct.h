@@ -30,6 +30,11 @@ extern "C" {
 
 #define SNC_GROW_HEURISTIC 0
 #define SNC_GROW_LEARNED 1
+/* Learned, over the whole candidate pool: every window candidate of every
+ * non-input post is scored each step and the highest-scoring unconnected
+ * pairs regrow as many synapses as were pruned, wherever their posts are
+ * (the fallback tournament fills any shortfall). */
+#define SNC_GROW_LEARNED_POOL 2
 
 /* StructConfig (include/snc/connectome.hpp:33-48). */
 typedef struct snc_struct_config {
@@ -80,6 +85,11 @@ int snc_conn_layered_local(const int32_t* layers, int32_t n_layers, int32_t syna
 int snc_conn_sheet_recurrent(const int32_t* layers, int32_t n_layers, int32_t fan_in,
                              int32_t rec_fan_in, float radius, int32_t delay, float w_init,
                              uint64_t seed, snc_connectome** out);
+/* Thins a connectome to a synapse budget (no reference counterpart: it sets
+ * up a co-training base "under an N-synapse budget"): removes the synapses
+ * of smallest (|w|, CSR position) until at most `budget` remain; ages and
+ * the grow log of the survivors are kept. */
+int snc_conn_prune_to(snc_connectome* c, int32_t budget);
 void snc_conn_free(snc_connectome* c);
 int32_t snc_conn_num_synapses(const snc_connectome* c);
 int snc_conn_to_graph(const snc_connectome* c, snc_graph_buf** out);

@@ -29,7 +29,8 @@ struct snc_hip_mlp {
   float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;  // FP32 master, Adam state, grad
   uint16_t *w1b = nullptr, *w2b = nullptr, *w3b = nullptr;
   float *X = nullptr, *y = nullptr, *out = nullptr, *G = nullptr, *loss = nullptr, *loss1 = nullptr;
-  int64_t cap_x = 0, cap_g = 0;
+  uint16_t* Xb = nullptr;  // scoring input: features as bf16 bits [cap_xb][16]
+  int64_t cap_x = 0, cap_g = 0, cap_xb = 0;
   int adam_step = 0;
   MlpDev dev() const { return {w1b, w2b, w3b, w + kMlpOffB1, w + kMlpOffB2, w + kMlpOffB3}; }
   ~snc_hip_mlp() {
@@ -37,7 +38,8 @@ struct snc_hip_mlp {
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
     for (void* p : {(void*)w, (void*)m, (void*)v, (void*)g, (void*)w1b, (void*)w2b, (void*)w3b,
-                    (void*)X, (void*)y, (void*)out, (void*)G, (void*)loss, (void*)loss1})
+                    (void*)X, (void*)y, (void*)out, (void*)G, (void*)loss, (void*)loss1,
+                    (void*)Xb})
       if (p) (void)hipFree(p);
   }
 };
@@ -146,7 +148,7 @@ int grad_device(snc_hip_mlp* m, int64_t off, int64_t M, std::vector<float>& grad
 
 namespace snc_amd {
 
-int mlp_score_device(snc_hip_mlp* m, const float* X, int64_t M, float* out, double* kernel_ms) {
+int mlp_score_device(snc_hip_mlp* m, const uint16_t* X, int64_t M, float* out, double* kernel_ms) {
   if (M <= 0) return SNC_OK;
   int dev = 0, cus = 256;
   hipDeviceProp_t prop;
@@ -240,11 +242,15 @@ int snc_hip_mlp_score(snc_hip_mlp* m, const float* features, int64_t M, float* s
   if (M == 0) return SNC_OK;
   hip_clear();
   int rc = ensure_x(m, M);
+  if (!rc) rc = grow((void**)&m->Xb, m->cap_xb, M * kMlpIn, sizeof(uint16_t));
   if (rc) return rc;
-  if (hipMemcpyAsync(m->X, features, sizeof(float) * M * kMlpIn, hipMemcpyHostToDevice,
+  // the features go up as the bf16 values the scorer computes with
+  std::vector<uint16_t> xb((size_t)M * kMlpIn);
+  mlp_features_bf16(features, M * kMlpIn, xb.data());
+  if (hipMemcpyAsync(m->Xb, xb.data(), sizeof(uint16_t) * xb.size(), hipMemcpyHostToDevice,
                      m->stream) != hipSuccess)
     return fail(SNC_E_HIP, "feature upload failed");
-  rc = mlp_score_device(m, m->X, M, m->out, kernel_ms);
+  rc = mlp_score_device(m, m->Xb, M, m->out, kernel_ms);
   if (rc) return rc;
   if (hipMemcpy(scores, m->out, sizeof(float) * M, hipMemcpyDeviceToHost) != hipSuccess)
     return fail(SNC_E_HIP, "score download failed");

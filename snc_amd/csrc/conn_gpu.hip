@@ -9,7 +9,9 @@
 // order, keeps the cells within the radius (distance in double, as the host),
 // compacts them in order with ballots, then computes the window maximum and
 // each candidate's rank (fire descending, id ascending) and writes the 16
-// features; the MLP scores them where they lie.
+// features as bf16 (the scorer's input precision); the MLP scores them where
+// they lie. The host gets back pres, scores, ranks and window maxima, from
+// which it rebuilds the float features of the candidates it grows.
 
 #include <hip/hip_runtime.h>
 
@@ -101,7 +103,9 @@ struct CandArgs {
   const int32_t* offs;   // [n_freed + 1] (pass 2)
   int32_t* count;        // [n_freed] (pass 1)
   int32_t* cand;         // [M] pre of each candidate (pass 2)
-  float* feat;           // [M][16] (pass 2)
+  uint16_t* feat;        // [M][16] features as bf16 bits, the scorer's input (pass 2)
+  float* rank;           // [M] window rank of each candidate, feature 13 (pass 2)
+  double* wmax;          // [n_freed] largest fire count in each post's window (pass 2)
 };
 
 __device__ __forceinline__ double dist2d(const CandArgs& a, int q, int p) {
@@ -165,6 +169,7 @@ __global__ __launch_bounds__(kWave) void k_cand_features(CandArgs a) {
   for (int k = lane; k < c; k += kWave) mx = fmax(mx, a.fire[s_q[k]]);
   for (int d = kWave / 2; d > 0; d >>= 1) mx = fmax(mx, __shfl_xor(mx, d, kWave));
   const int base = a.offs[u];
+  if (lane == 0) a.wmax[u] = mx;
   const double rp = a.rate[p];
   for (int k = lane; k < c; k += kWave) {
     const int q = s_q[k];
@@ -177,7 +182,7 @@ __global__ __launch_bounds__(kWave) void k_cand_features(CandArgs a) {
     }
     const double rank = c > 1 ? (double)before / (double)(c - 1) : 0.0;
     const double rq = a.rate[q];
-    float* f = a.feat + (size_t)(base + k) * 16;
+    float f[16];
     f[0] = (float)rq;
     f[1] = (float)rp;
     f[2] = (float)(rq * rp * 100.0);
@@ -194,6 +199,15 @@ __global__ __launch_bounds__(kWave) void k_cand_features(CandArgs a) {
     f[13] = (float)rank;
     f[14] = a.f14[p];
     f[15] = 1.0f;
+    // the scorer rounds every feature to bf16 (round to nearest even) anyway
+    uint32_t h[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      h[i] = (uint32_t)bf16_rne(f[2 * i]) | ((uint32_t)bf16_rne(f[2 * i + 1]) << 16);
+    uint4* dst = reinterpret_cast<uint4*>(a.feat + (size_t)(base + k) * 16);
+    dst[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    dst[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    a.rank[base + k] = f[13];
     a.cand[base + k] = q;
   }
 }
@@ -258,18 +272,21 @@ int conn_gpu_prune_select(const float* w, const int32_t* age, int64_t E, int pro
 
 int conn_gpu_candidates(const ConnSheetView& v, const std::vector<int>& freed, float window,
                         snc_hip_mlp* model, std::vector<int>& offs, std::vector<int>& all_q,
-                        std::vector<float>& all_f, std::vector<float>& score, double* score_ms) {
+                        std::vector<float>& rank, std::vector<double>& wmax,
+                        std::vector<float>& score, double* score_ms) {
   const int nf = (int)freed.size();
   offs.assign((size_t)nf + 1, 0);
   all_q.clear();
-  all_f.clear();
+  rank.clear();
+  wmax.assign((size_t)nf, 0.0);
   score.clear();
   if (score_ms) *score_ms = 0.0;
   if (nf == 0) return SNC_OK;
   hip_clear();
   DevBuf<int32_t> d_layer, d_lb, d_lc, d_side, d_freed, d_offs, d_count, d_cand;
-  DevBuf<float> d_px, d_py, d_f5, d_f6, d_f7, d_f8, d_f9, d_f10, d_f14, d_feat, d_score;
-  DevBuf<double> d_fire, d_rate;
+  DevBuf<float> d_px, d_py, d_f5, d_f6, d_f7, d_f8, d_f9, d_f10, d_f14, d_rank, d_score;
+  DevBuf<uint16_t> d_feat;
+  DevBuf<double> d_fire, d_rate, d_wmax;
   const size_t n = (size_t)v.n, L = (size_t)v.num_layers;
   int rc = d_layer.upload(v.layer.data(), n);
   if (!rc) rc = d_lb.upload(v.layer_begin.data(), L);
@@ -328,11 +345,15 @@ int conn_gpu_candidates(const ConnSheetView& v, const std::vector<int>& freed, f
   rc = d_offs.upload(offs.data(), offs.size());
   if (!rc) rc = d_cand.alloc((size_t)M);
   if (!rc) rc = d_feat.alloc((size_t)M * 16);
+  if (!rc) rc = d_rank.alloc((size_t)M);
+  if (!rc) rc = d_wmax.alloc((size_t)nf);
   if (!rc) rc = d_score.alloc((size_t)M);
   if (rc) return rc;
   a.offs = d_offs.p;
   a.cand = d_cand.p;
   a.feat = d_feat.p;
+  a.rank = d_rank.p;
+  a.wmax = d_wmax.p;
   hipLaunchKernelGGL(k_cand_features, dim3(nf), dim3(kWave), 0, nullptr, a);
   rc = hip_last("candidate feature launch");
   if (rc) return rc;
@@ -341,10 +362,11 @@ int conn_gpu_candidates(const ConnSheetView& v, const std::vector<int>& freed, f
   rc = mlp_score_device(model, d_feat.p, M, d_score.p, score_ms);
   if (rc) return rc;
   all_q.resize((size_t)M);
-  all_f.resize((size_t)M * 16);
+  rank.resize((size_t)M);
   score.resize((size_t)M);
   e = hipMemcpy(all_q.data(), d_cand.p, (size_t)M * 4, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(all_f.data(), d_feat.p, (size_t)M * 64, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(rank.data(), d_rank.p, (size_t)M * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(wmax.data(), d_wmax.p, (size_t)nf * 8, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(score.data(), d_score.p, (size_t)M * 4, hipMemcpyDeviceToHost);
   return e == hipSuccess ? SNC_OK : hip_fail(e, "candidate download");
 }

@@ -33,10 +33,13 @@ int conn_gpu_prune_select(const float* w, const int32_t* age, int64_t E, int pro
 
 // Window candidates of every freed post (disc of radius `window` over the
 // previous and, for hidden posts, the own layer; by layer then id), their
-// 16 features and MLP scores. offs[u]..offs[u+1] index post u's candidates in
-// all_q / score, and all_f holds 16 floats per candidate.
+// 16 features (on the device, bf16) and MLP scores. offs[u]..offs[u+1] index
+// post u's candidates in all_q / rank / score; rank is each candidate's
+// window rank (feature 13) and wmax[u] post u's window maximum fire count,
+// the two window statistics of the features.
 int conn_gpu_candidates(const ConnSheetView& v, const std::vector<int>& freed, float window,
                         snc_hip_mlp* model, std::vector<int>& offs, std::vector<int>& all_q,
-                        std::vector<float>& all_f, std::vector<float>& score, double* score_ms);
+                        std::vector<float>& rank, std::vector<double>& wmax,
+                        std::vector<float>& score, double* score_ms);
 
 }  // namespace snc_amd

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <random>
 #include <vector>
 
@@ -39,7 +40,8 @@ struct Edge {
   int64_t log_id;  // grow-log entry, -1 if none
 };
 
-float layer_pos(int k, int cnt) { return cnt > 1 ? (float)k / (cnt - 1) : 0.5f; }
+// Position of the k-th of cnt neurons of a layer on [0, 1] (0.5 for one).
+float unit_coord(int k, int cnt) { return cnt > 1 ? (float)k / (cnt - 1) : 0.5f; }
 
 }  // namespace
 
@@ -70,14 +72,16 @@ struct snc_connectome {
     for (const auto& a : adj) m += (int)a.size();
     return m;
   }
-  // Window of candidate pres for post p (connectome.cpp:171-176).
-  bool window(int p, float w, int& b0, int& klo, int& khi) const {
-    const int pre_l = layer[p] - 1;
-    if (pre_l < 0) return false;
-    b0 = layer_begin[pre_l];
-    const int bn = layer_count[pre_l];
-    klo = std::clamp((int)std::ceil((pos[p] - w) * (bn - 1)), 0, bn - 1);
-    khi = std::clamp((int)std::floor((pos[p] + w) * (bn - 1)), 0, bn - 1);
+  // Candidate pres of post p on a layered connectome: the neurons of layer
+  // l-1 whose positions lie within w of p's, as an index range [first,
+  // last] of that layer (connectome.cpp:171-176).
+  bool layered_window(int p, float w, int& layer0, int& first, int& last) const {
+    const int src = layer[p] - 1;
+    if (src < 0) return false;
+    layer0 = layer_begin[src];
+    const int top = layer_count[src] - 1;
+    first = std::clamp((int)std::ceil((pos[p] - w) * top), 0, top);
+    last = std::clamp((int)std::floor((pos[p] + w) * top), 0, top);
     return true;
   }
   // 2-D distance of two sheet neurons, in double (the device computes the same).
@@ -91,9 +95,9 @@ struct snc_connectome {
   void window_list(int p, float w, std::vector<int>& out) const {
     out.clear();
     if (!sheet) {
-      int b0, klo, khi;
-      if (window(p, w, b0, klo, khi))
-        for (int k = klo; k <= khi; ++k) out.push_back(b0 + k);
+      int layer0, first, last;
+      if (layered_window(p, w, layer0, first, last))
+        for (int k = first; k <= last; ++k) out.push_back(layer0 + k);
       return;
     }
     const int l = layer[p];
@@ -218,6 +222,317 @@ using snc_amd::conn_gpu_prune_select;
 using snc_amd::ConnSheetView;
 using snc_amd::fail;
 
+namespace {
+
+// Synapses regrown per step: grow_per_epoch, or with reward modulation a
+// share of it that falls as the reward rises above chance, never below
+// reward_floor (connectome.cpp:124-130).
+int regrowth_quota(const snc_struct_config& cfg, float reward) {
+  if (!cfg.reward_modulated) return cfg.grow_per_epoch;
+  const float above_chance =
+      std::clamp((reward - cfg.reward_chance) / std::max(1e-3f, 1.0f - cfg.reward_chance), 0.0f,
+                 1.0f);
+  const float share = std::clamp(1.0f - above_chance, cfg.reward_floor, 1.0f);
+  return std::max(0, (int)std::lround(cfg.grow_per_epoch * share));
+}
+
+// One structural step on a connectome: prune, regrow (heuristic or learned),
+// fill up by tournament, age. Edges are addressed by their CSR position,
+// edge_base[pre] + slot, fixed between the prune's selection and its removal.
+class StructuralStep {
+ public:
+  StructuralStep(snc_connectome* c, const snc_struct_config& cfg, const int64_t* fires,
+                 int64_t n_fires, double activity_norm, bool log)
+      : c_(c), cfg_(cfg), fires_(fires), n_fires_(n_fires), norm_(activity_norm), log_(log) {}
+
+  int pruned() const { return (int)freed_.size(); }
+  int grown() const { return grown_; }
+
+  // Removes the `quota` prune-eligible synapses (age >= protect_rounds) of
+  // smallest |w| (connectome.cpp:132-160) and records each one's post, in
+  // CSR order, as a post to regrow. The layered connectome selects with the
+  // reference's partial ordering by |w| alone (ties as its nth_element
+  // leaves them); the sheet connectome with the total order (|w|, CSR
+  // position), on the host or on the device.
+  int prune(int quota) {
+    std::vector<int64_t> base((size_t)c_->n + 1, 0);
+    for (int i = 0; i < c_->n; ++i) base[i + 1] = base[i] + (int64_t)c_->adj[i].size();
+    struct Eligible {
+      float mag;
+      int64_t at;  // CSR position
+    };
+    std::vector<Eligible> pool;
+    for (int i = 0; i < c_->n; ++i)
+      for (size_t k = 0; k < c_->adj[i].size(); ++k)
+        if (c_->adj[i][k].age >= cfg_.protect_rounds)
+          pool.push_back({std::fabs(c_->adj[i][k].weight), base[i] + (int64_t)k});
+    const int take = std::min(quota, (int)pool.size());
+    std::vector<uint8_t> drop((size_t)base[c_->n], 0);
+    if (c_->sheet && cfg_.on_device) {
+      std::vector<float> w;
+      std::vector<int32_t> age;
+      w.reserve(drop.size());
+      age.reserve(drop.size());
+      for (const auto& row : c_->adj)
+        for (const Edge& e : row) {
+          w.push_back(e.weight);
+          age.push_back(e.age);
+        }
+      const int rc = conn_gpu_prune_select(w.data(), age.data(), (int64_t)w.size(),
+                                           cfg_.protect_rounds, take, drop.data());
+      if (rc) return rc;
+    } else {
+      if (take > 0) {
+        if (c_->sheet)
+          std::nth_element(pool.begin(), pool.begin() + take, pool.end(),
+                           [](const Eligible& x, const Eligible& y) {
+                             return x.mag != y.mag ? x.mag < y.mag : x.at < y.at;
+                           });
+        else
+          std::nth_element(pool.begin(), pool.begin() + take, pool.end(),
+                           [](const Eligible& x, const Eligible& y) { return x.mag < y.mag; });
+      }
+      for (int j = 0; j < take; ++j) drop[(size_t)pool[j].at] = 1;
+    }
+    // Logged synapses meeting their first eligible prune get their outcome:
+    // survived or not, and the share of eligible |w| below theirs (ties
+    // counted half).
+    std::vector<float> mags;
+    for (const Eligible& x : pool) {
+      const int pre = (int)(std::upper_bound(base.begin(), base.end(), x.at) - base.begin()) - 1;
+      const Edge& e = c_->adj[pre][(size_t)(x.at - base[pre])];
+      if (e.log_id < 0 || c_->log_label[e.log_id] >= 0) continue;
+      c_->log_label[e.log_id] = drop[(size_t)x.at] ? 0 : 1;
+      if (mags.empty()) {
+        for (const Eligible& y : pool) mags.push_back(y.mag);
+        std::sort(mags.begin(), mags.end());
+      }
+      const auto lo = std::lower_bound(mags.begin(), mags.end(), x.mag);
+      const auto hi = std::upper_bound(lo, mags.end(), x.mag);
+      const double below = (double)(lo - mags.begin()) + 0.5 * (double)(hi - lo);
+      c_->log_rank[e.log_id] = (float)(below / (double)mags.size());
+    }
+    for (int i = 0; i < c_->n; ++i) {
+      std::vector<Edge>& row = c_->adj[i];
+      size_t kept = 0;
+      for (size_t k = 0; k < row.size(); ++k) {
+        if (drop[(size_t)(base[i] + (int64_t)k)])
+          freed_.push_back(row[k].post);
+        else
+          row[kept++] = row[k];
+      }
+      row.resize(kept);
+    }
+    // the features see the connectome as the prune left it
+    fx_.reset(new FeatureCtx(c_, fires_, n_fires_, norm_, cfg_.delay));
+    return SNC_OK;
+  }
+
+  // The reference's regrowth (connectome.cpp:162-190): each freed post, in
+  // prune order, regrows from the unconnected pre of its window that fired
+  // most (the first such in window order). Logging runs may swap the choice
+  // for a uniformly drawn open window member (cfg.explore).
+  void grow_heuristic() {
+    const bool explore = log_ && cfg_.explore > 0.0f;
+    std::uniform_real_distribution<float> coin(0.0f, 1.0f);
+    for (int p : freed_) {
+      if (grown_ >= pruned()) break;
+      int q = strongest_open_pre(p);
+      if (q >= 0 && explore && coin(c_->explore_rng) < cfg_.explore) {
+        std::vector<int> open;
+        c_->window_list(p, cfg_.locality_window, scan_);
+        for (int cand : scan_)
+          if (!c_->connected(cand, p)) open.push_back(cand);
+        if (!open.empty())
+          q = open[std::uniform_int_distribution<size_t>(0, open.size() - 1)(c_->explore_rng)];
+      }
+      if (q < 0) continue;
+      int64_t id = -1;
+      if (log_) {
+        window_features(p, wq_, wf_);
+        for (size_t k = 0; k < wq_.size(); ++k)
+          if (wq_[k] == q) id = log_features(&wf_[k * kFeat]);
+      }
+      add_edge(q, p, id);
+    }
+  }
+
+  // Learned regrowth. Per post: each freed post regrows from the open
+  // window candidate the MLP scores highest (the first such in window
+  // order). Pool: every candidate of every non-input post's window is
+  // scored, and the `pruned` highest-scoring open pairs grow (ties: post
+  // order, then window order), wherever their posts are.
+  int grow_learned(snc_hip_mlp* model, bool pool, snc_struct_report* rep) {
+    std::vector<int> posts;
+    if (pool) {
+      for (int p = 0; p < c_->n; ++p)
+        if (c_->layer[p] >= 1) posts.push_back(p);
+    } else {
+      posts = freed_;
+    }
+    std::vector<int> offs(posts.size() + 1, 0), cand;
+    std::vector<float> feats, score, rank;
+    std::vector<double> wmax;
+    double ms = 0.0;
+    const bool device = c_->sheet && cfg_.on_device;
+    if (device) {
+      // Candidates, features and scores on the device; back come the pres,
+      // scores and the two window statistics, from which the log's float
+      // features of the grown candidates are rebuilt here.
+      const ConnSheetView sv = sheet_view(c_, *fx_);
+      const int rc = conn_gpu_candidates(sv, posts, cfg_.locality_window, model, offs, cand, rank,
+                                         wmax, score, &ms);
+      if (rc) return rc;
+    } else {
+      for (size_t u = 0; u < posts.size(); ++u) {
+        window_features(posts[u], wq_, wf_);
+        cand.insert(cand.end(), wq_.begin(), wq_.end());
+        feats.insert(feats.end(), wf_.begin(), wf_.end());
+        offs[u + 1] = (int)cand.size();
+      }
+      score.assign(cand.size(), 0.0f);
+      if (!cand.empty()) {
+        const int rc = snc_hip_mlp_score(model, feats.data(), (int64_t)cand.size(), score.data(), &ms);
+        if (rc) return rc;
+      }
+    }
+    rep->candidates_scored = (int64_t)cand.size();
+    rep->score_ms = ms;
+    float f[kFeat];
+    auto grow_candidate = [&](size_t u, int k) {
+      const int q = cand[k], p = posts[u];
+      int64_t id = -1;
+      if (log_) {
+        if (device)
+          fx_->features(q, p, wmax[u], (double)rank[k], f);
+        else
+          std::memcpy(f, &feats[(size_t)k * kFeat], sizeof(f));
+        id = log_features(f);
+      }
+      add_edge(q, p, id);
+    };
+    if (!pool) {
+      for (size_t u = 0; u < posts.size() && grown_ < pruned(); ++u) {
+        int pick = -1;
+        float top = -INFINITY;
+        for (int k = offs[u]; k < offs[u + 1]; ++k)
+          if (!c_->connected(cand[k], posts[u]) && score[k] > top) {
+            top = score[k];
+            pick = k;
+          }
+        if (pick >= 0) grow_candidate(u, pick);
+      }
+      return SNC_OK;
+    }
+    std::vector<int> order(cand.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+      return score[x] != score[y] ? score[x] > score[y] : x < y;
+    });
+    std::vector<size_t> post_of(cand.size());
+    for (size_t u = 0; u < posts.size(); ++u)
+      for (int k = offs[u]; k < offs[u + 1]; ++k) post_of[k] = u;
+    for (int k : order) {
+      if (grown_ >= pruned()) break;
+      if (!c_->connected(cand[k], posts[post_of[k]])) grow_candidate(post_of[k], k);
+    }
+    return SNC_OK;
+  }
+
+  // Until as many synapses grew as were pruned (or the attempts run out):
+  // draw four non-input posts uniformly, take the one that fired least (the
+  // first such), regrow it from its strongest open pre (connectome.cpp:192-211).
+  void grow_tournament() {
+    const int lo = c_->layer_begin.size() > 1 ? c_->layer_begin[1] : 0;
+    std::uniform_int_distribution<int> draw(lo, std::max(0, c_->n - 1));
+    const int budget = cfg_.max_attempts_mult * std::max(1, pruned());
+    for (int tries = 0; grown_ < pruned() && tries < budget; ++tries) {
+      int p = -1;
+      for (int t = 0; t < 4; ++t) {
+        const int x = draw(c_->rng);
+        if (c_->layer[x] != 0 && (p < 0 || fires(x) < fires(p))) p = x;
+      }
+      if (p < 0) continue;
+      const int q = strongest_open_pre(p);
+      if (q >= 0) add_edge(q, p, -1);
+    }
+  }
+
+  // Every synapse younger than the initial ones ages by one step.
+  void age_edges() {
+    for (auto& row : c_->adj)
+      for (Edge& e : row)
+        if (e.age < kAgeInit) ++e.age;
+  }
+
+ private:
+  long long fires(int id) const { return id < n_fires_ ? fires_[id] : 0LL; }
+
+  void add_edge(int pre, int post, int64_t log_id) {
+    c_->adj[pre].push_back({post, cfg_.delay, cfg_.w_grow_init, 0, log_id});
+    ++grown_;
+  }
+
+  int strongest_open_pre(int p) {
+    c_->window_list(p, cfg_.locality_window, scan_);
+    int pick = -1;
+    long long most = -1;
+    for (int q : scan_)
+      if (!c_->connected(q, p) && fires(q) > most) {
+        most = fires(q);
+        pick = q;
+      }
+    return pick;
+  }
+
+  // The window members of post p in window order and their 16 features.
+  void window_features(int p, std::vector<int>& qs, std::vector<float>& feats) {
+    qs.clear();
+    feats.clear();
+    c_->window_list(p, cfg_.locality_window, qs);
+    if (qs.empty()) return;
+    // window maximum, and each member's rank: fire descending, id ascending
+    double top = 0.0;
+    std::vector<std::pair<double, int>> by_fire;
+    by_fire.reserve(qs.size());
+    for (int q : qs) {
+      top = std::max(top, fx_->fire(q));
+      by_fire.push_back({-fx_->fire(q), q});
+    }
+    std::sort(by_fire.begin(), by_fire.end());
+    const double last = by_fire.size() > 1 ? (double)(by_fire.size() - 1) : 0.0;
+    rank_of_.assign((size_t)c_->n, 0.0);
+    for (size_t r = 0; r < by_fire.size(); ++r)
+      rank_of_[by_fire[r].second] = last > 0 ? (double)r / last : 0.0;
+    feats.resize(qs.size() * kFeat);
+    for (size_t k = 0; k < qs.size(); ++k)
+      fx_->features(qs[k], p, top, rank_of_[qs[k]], &feats[k * kFeat]);
+  }
+
+  int64_t log_features(const float* f) {
+    const int64_t id = (int64_t)c_->log_label.size();
+    c_->log_feat.insert(c_->log_feat.end(), f, f + kFeat);
+    c_->log_label.push_back(-1);
+    c_->log_rank.push_back(-1.0f);
+    return id;
+  }
+
+  snc_connectome* c_;
+  const snc_struct_config& cfg_;
+  const int64_t* fires_;
+  int64_t n_fires_;
+  double norm_;
+  bool log_;
+  std::unique_ptr<FeatureCtx> fx_;  // built by prune()
+  std::vector<int> freed_;
+  int grown_ = 0;
+  std::vector<int> scan_, wq_;
+  std::vector<float> wf_;
+  std::vector<double> rank_of_;
+};
+
+}  // namespace
+
 extern "C" {
 
 int snc_conn_layered_local(const int32_t* layers, int32_t n_layers, int32_t budget, int32_t delay,
@@ -241,7 +556,7 @@ int snc_conn_layered_local(const int32_t* layers, int32_t n_layers, int32_t budg
     c->layer_count[l] = layers[l];
     for (int k = 0; k < layers[l]; ++k, ++idx) {
       c->layer[idx] = l;
-      c->pos[idx] = layer_pos(k, layers[l]);
+      c->pos[idx] = unit_coord(k, layers[l]);
       if (l == 0) {
         c->role[idx] = SNC_ROLE_INPUT;
         c->channel[idx] = k;
@@ -251,20 +566,25 @@ int snc_conn_layered_local(const int32_t* layers, int32_t n_layers, int32_t budg
       }
     }
   }
-  long long fwd = 0;
-  for (int l = 0; l + 1 < n_layers; ++l) fwd += layers[l];
-  const int fanout = fwd > 0 ? std::max(1, (int)(budget / fwd)) : 0;
+  // Every neuron of a non-final layer projects to a run of `span` adjacent
+  // neurons of the next layer around its own position there, jittered by up
+  // to half a target spacing; span spreads the budget evenly over all
+  // projecting neurons (connectome.cpp:38-62, same draws).
+  int64_t sources = 0;
+  for (int l = 0; l + 1 < n_layers; ++l) sources += layers[l];
+  const int per_source = sources > 0 ? std::max(1, (int)(budget / sources)) : 0;
   std::uniform_real_distribution<float> jitter(-0.5f, 0.5f);
   for (int l = 0; l + 1 < n_layers; ++l) {
-    const int from0 = c->layer_begin[l], from_n = c->layer_count[l];
-    const int to0 = c->layer_begin[l + 1], to_n = c->layer_count[l + 1];
-    const int k = std::min(fanout, to_n);
-    for (int a = 0; a < from_n; ++a) {
-      const float pa = layer_pos(a, from_n) + jitter(c->rng) / std::max(1, to_n);
-      int centre = (int)std::lround(pa * (to_n - 1));
-      centre = std::clamp(centre, 0, std::max(0, to_n - 1));
-      const int lo = std::clamp(centre - k / 2, 0, std::max(0, to_n - k));
-      for (int j = 0; j < k; ++j) c->adj[from0 + a].push_back({to0 + lo + j, delay, w_init, kAgeInit, -1});
+    const int next = c->layer_count[l + 1];
+    const int span = std::min(per_source, next);
+    const int last_start = std::max(0, next - span);
+    for (int s = 0; s < c->layer_count[l]; ++s) {
+      const float at = unit_coord(s, c->layer_count[l]) + jitter(c->rng) / std::max(1, next);
+      const int mid = std::clamp((int)std::lround(at * (next - 1)), 0, std::max(0, next - 1));
+      const int start = std::clamp(mid - span / 2, 0, last_start);
+      std::vector<Edge>& row = c->adj[c->layer_begin[l] + s];
+      for (int j = start; j < start + span; ++j)
+        row.push_back({c->layer_begin[l + 1] + j, delay, w_init, kAgeInit, -1});
     }
   }
   *out = c.release();
@@ -305,7 +625,7 @@ int snc_conn_sheet_recurrent(const int32_t* layers, int32_t n_layers, int32_t fa
     c->side[l] = side;
     for (int k = 0; k < cnt; ++k, ++idx) {
       c->layer[idx] = l;
-      c->pos[idx] = layer_pos(k, cnt);
+      c->pos[idx] = unit_coord(k, cnt);
       c->px[idx] = side > 1 ? (float)(k % side) / (float)(side - 1) : 0.5f;
       c->py[idx] = side > 1 ? (float)(k / side) / (float)(side - 1) : 0.5f;
       if (l == 0) {
@@ -343,6 +663,34 @@ int snc_conn_sheet_recurrent(const int32_t* layers, int32_t n_layers, int32_t fa
     for (int q : pres[p]) c->adj[q].push_back({p, delay, w_init, kAgeInit, -1});
   }
   *out = c.release();
+  return SNC_OK;
+}
+
+int snc_conn_prune_to(snc_connectome* c, int32_t budget) {
+  if (!c || budget < 0) return fail(SNC_E_INVALID, "bad budget");
+  const int64_t m = c->num_synapses();
+  if (m <= budget) return SNC_OK;
+  struct Keyed {
+    float mag;
+    int64_t at;
+  };
+  std::vector<Keyed> all;
+  all.reserve((size_t)m);
+  for (const auto& row : c->adj)
+    for (const Edge& e : row) all.push_back({std::fabs(e.weight), (int64_t)all.size()});
+  const int64_t cut = m - budget;
+  std::nth_element(all.begin(), all.begin() + cut, all.end(), [](const Keyed& x, const Keyed& y) {
+    return x.mag != y.mag ? x.mag < y.mag : x.at < y.at;
+  });
+  std::vector<uint8_t> drop((size_t)m, 0);
+  for (int64_t j = 0; j < cut; ++j) drop[(size_t)all[j].at] = 1;
+  int64_t at = 0;
+  for (auto& row : c->adj) {
+    size_t kept = 0;
+    for (size_t k = 0; k < row.size(); ++k, ++at)
+      if (!drop[(size_t)at]) row[kept++] = row[k];
+    row.resize(kept);
+  }
   return SNC_OK;
 }
 
@@ -403,247 +751,23 @@ int snc_conn_structural_update(snc_connectome* c, const int64_t* syn_deliveries,
   (void)syn_deliveries;
   (void)n_syn;
   if (!c || !cfg || !rep) return fail(SNC_E_INVALID, "null argument");
+  if (grow_mode != SNC_GROW_HEURISTIC && grow_mode != SNC_GROW_LEARNED &&
+      grow_mode != SNC_GROW_LEARNED_POOL)
+    return fail(SNC_E_INVALID, "unknown grow mode");
   std::memset(rep, 0, sizeof(*rep));
-  int grow_target = cfg->grow_per_epoch;
-  if (cfg->reward_modulated) {
-    const float denom = std::max(1e-3f, 1.0f - cfg->reward_chance);
-    const float comp = std::clamp((reward - cfg->reward_chance) / denom, 0.0f, 1.0f);
-    const float frac = std::clamp(1.0f - comp, cfg->reward_floor, 1.0f);
-    grow_target = std::max(0, (int)std::lround(cfg->grow_per_epoch * frac));
-  }
-
-  // --- prune the K weakest |w| among prune-eligible edges ---
-  struct Cand {
-    float score;
-    int i, k;
-  };
-  std::vector<Cand> cand;
-  for (int i = 0; i < c->n; ++i)
-    for (int k = 0; k < (int)c->adj[i].size(); ++k)
-      if (c->adj[i][k].age >= cfg->protect_rounds) cand.push_back({std::fabs(c->adj[i][k].weight), i, k});
-  const int K = std::min(grow_target, (int)cand.size());
-  std::vector<std::vector<char>> rm(c->n);
-  for (int i = 0; i < c->n; ++i) rm[i].assign(c->adj[i].size(), 0);
-  if (c->sheet && cfg->on_device) {
-    // The K smallest (|w|, CSR position) on the device.
-    std::vector<float> w;
-    std::vector<int32_t> age;
-    for (const auto& a : c->adj)
-      for (const Edge& e : a) {
-        w.push_back(e.weight);
-        age.push_back(e.age);
-      }
-    std::vector<uint8_t> mask(w.size(), 0);
-    const int rc = conn_gpu_prune_select(w.data(), age.data(), (int64_t)w.size(),
-                                         cfg->protect_rounds, K, mask.data());
-    if (rc) return rc;
-    int64_t q = 0;
-    for (int i = 0; i < c->n; ++i)
-      for (size_t k = 0; k < c->adj[i].size(); ++k) rm[i][k] = (char)mask[q++];
-  } else if (c->sheet) {
-    // Total order (|w|, CSR position): cand is built in CSR order.
-    if (K > 0)
-      std::nth_element(cand.begin(), cand.begin() + K, cand.end(), [&](const Cand& a, const Cand& b) {
-        return a.score != b.score ? a.score < b.score : (a.i != b.i ? a.i < b.i : a.k < b.k);
-      });
-    for (int j = 0; j < K; ++j) rm[cand[j].i][cand[j].k] = 1;
-  } else {
-    if (K > 0)
-      std::nth_element(cand.begin(), cand.begin() + K, cand.end(),
-                       [](const Cand& a, const Cand& b) { return a.score < b.score; });
-    for (int j = 0; j < K; ++j) rm[cand[j].i][cand[j].k] = 1;
-  }
-  // Grow-log labels: an edge decides its outcome at its first eligible prune:
-  // whether it survived, and where its |w| ranks among the eligible edges
-  // (fraction below it, ties counted half).
-  std::vector<float> sorted_scores;
-  for (const Cand& x : cand) {
-    const Edge& e = c->adj[x.i][x.k];
-    if (e.log_id < 0 || c->log_label[e.log_id] >= 0) continue;
-    c->log_label[e.log_id] = rm[x.i][x.k] ? 0 : 1;
-    if (sorted_scores.empty()) {
-      sorted_scores.reserve(cand.size());
-      for (const Cand& y : cand) sorted_scores.push_back(y.score);
-      std::sort(sorted_scores.begin(), sorted_scores.end());
-    }
-    const auto lo = std::lower_bound(sorted_scores.begin(), sorted_scores.end(), x.score);
-    const auto hi = std::upper_bound(lo, sorted_scores.end(), x.score);
-    const double below = (double)(lo - sorted_scores.begin()) + 0.5 * (double)(hi - lo);
-    c->log_rank[e.log_id] = (float)(below / (double)sorted_scores.size());
-  }
-  int pruned = 0;
-  std::vector<int> freed;
-  freed.reserve(K);
-  for (int i = 0; i < c->n; ++i) {
-    std::vector<Edge> keep;
-    keep.reserve(c->adj[i].size());
-    for (int k = 0; k < (int)c->adj[i].size(); ++k) {
-      if (rm[i][k]) {
-        ++pruned;
-        freed.push_back(c->adj[i][k].post);
-      } else {
-        keep.push_back(c->adj[i][k]);
-      }
-    }
-    c->adj[i].swap(keep);
-  }
-
-  // --- grow ---
-  auto fire_of = [&](int id) -> long long { return id < n_neur ? neuron_fires[id] : 0LL; };
-  std::vector<int> wl;
-  auto grow_local = [&](int p) -> int {
-    c->window_list(p, cfg->locality_window, wl);
-    int best = -1;
-    long long best_fire = -1;
-    for (int q : wl) {
-      if (c->connected(q, p)) continue;
-      if (fire_of(q) > best_fire) {
-        best_fire = fire_of(q);
-        best = q;
-      }
-    }
-    return best;
-  };
-  FeatureCtx fx(c, neuron_fires, n_neur, activity_norm, cfg->delay);
-  std::vector<int> wf;
-  auto window_features = [&](int p, std::vector<int>& qs, std::vector<float>& feats) {
-    qs.clear();
-    feats.clear();
-    c->window_list(p, cfg->locality_window, wf);
-    if (wf.empty()) return;
-    double mx = 0.0;
-    std::vector<std::pair<double, int>> order;
-    for (int q : wf) {
-      mx = std::max(mx, fx.fire(q));
-      order.push_back({fx.fire(q), q});
-    }
-    std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) {
-      return a.first != b.first ? a.first > b.first : a.second < b.second;
-    });
-    std::vector<double> rank(c->n, 0.0);
-    for (size_t r = 0; r < order.size(); ++r)
-      rank[order[r].second] = order.size() > 1 ? (double)r / (order.size() - 1) : 0.0;
-    for (int q : wf) {
-      qs.push_back(q);
-      feats.resize(feats.size() + kFeat);
-      fx.features(q, p, mx, rank[q], feats.data() + feats.size() - kFeat);
-    }
-  };
-  auto log_edge = [&](int q, int p, const float* f) -> int64_t {
-    if (!log_grown) return -1;
-    const int64_t id = (int64_t)c->log_label.size();
-    c->log_feat.insert(c->log_feat.end(), f, f + kFeat);
-    c->log_label.push_back(-1);
-    c->log_rank.push_back(-1.0f);
-    (void)q;
-    (void)p;
-    return id;
-  };
-
-  int grown = 0;
-  const bool learned = grow_mode == SNC_GROW_LEARNED && model != nullptr;
-  if (learned) {
-    // Score every window candidate of every freed post in one call, then
-    // regrow each freed post from its best-scoring still-unconnected pre.
-    std::vector<int> offs(freed.size() + 1, 0), all_q;
-    std::vector<float> all_f, qs_feat;
-    std::vector<int> qs;
-    std::vector<float> score;
-    double ms = 0.0;
-    if (c->sheet && cfg->on_device) {
-      // Candidates, features and scores on the device; back come the
-      // candidates' pres, scores and features (the features for the log).
-      const ConnSheetView sv = sheet_view(c, fx);
-      const int rc = conn_gpu_candidates(sv, freed, cfg->locality_window, model, offs, all_q,
-                                         all_f, score, &ms);
-      if (rc) return rc;
-    } else {
-      for (size_t u = 0; u < freed.size(); ++u) {
-        window_features(freed[u], qs, qs_feat);
-        all_q.insert(all_q.end(), qs.begin(), qs.end());
-        all_f.insert(all_f.end(), qs_feat.begin(), qs_feat.end());
-        offs[u + 1] = (int)all_q.size();
-      }
-      score.assign(all_q.size(), 0.0f);
-      if (!all_q.empty()) {
-        const int rc =
-            snc_hip_mlp_score(model, all_f.data(), (int64_t)all_q.size(), score.data(), &ms);
-        if (rc) return rc;
-      }
-    }
-    rep->candidates_scored = (int64_t)all_q.size();
-    rep->score_ms = ms;
-    for (size_t u = 0; u < freed.size() && grown < pruned; ++u) {
-      const int p = freed[u];
-      int best = -1;
-      float bs = -INFINITY;
-      for (int k = offs[u]; k < offs[u + 1]; ++k) {
-        const int q = all_q[k];
-        if (c->connected(q, p)) continue;
-        if (score[k] > bs) {
-          bs = score[k];
-          best = k;
-        }
-      }
-      if (best >= 0) {
-        const int q = all_q[best];
-        c->adj[q].push_back({p, cfg->delay, cfg->w_grow_init, 0, log_edge(q, p, &all_f[(size_t)best * kFeat])});
-        ++grown;
-      }
-    }
-  } else {
-    std::vector<int> qs;
-    std::vector<float> qf;
-    const bool explore = log_grown && cfg->explore > 0.0f;
-    std::uniform_real_distribution<float> u01(0.0f, 1.0f);
-    for (int p : freed) {
-      if (grown >= pruned) break;
-      int q = grow_local(p);
-      if (q >= 0 && explore && u01(c->explore_rng) < cfg->explore) {
-        std::vector<int> open;
-        c->window_list(p, cfg->locality_window, wl);
-        for (int cq : wl)
-          if (!c->connected(cq, p)) open.push_back(cq);
-        if (!open.empty())
-          q = open[std::uniform_int_distribution<size_t>(0, open.size() - 1)(c->explore_rng)];
-      }
-      if (q >= 0) {
-        int64_t lid = -1;
-        if (log_grown) {
-          window_features(p, qs, qf);
-          for (size_t k = 0; k < qs.size(); ++k)
-            if (qs[k] == q) lid = log_edge(q, p, &qf[k * kFeat]);
-        }
-        c->adj[q].push_back({p, cfg->delay, cfg->w_grow_init, 0, lid});
-        ++grown;
-      }
-    }
-  }
-  // Fallback onto under-driven posts (4-way tournament), heuristic choice.
-  std::uniform_int_distribution<int> any_post(c->layer_begin.size() > 1 ? c->layer_begin[1] : 0,
-                                              std::max(0, c->n - 1));
-  int attempts = 0;
-  const int max_attempts = cfg->max_attempts_mult * std::max(1, pruned);
-  while (grown < pruned && attempts < max_attempts) {
-    ++attempts;
-    int p = -1;
-    for (int t = 0; t < 4; ++t) {
-      const int q = any_post(c->rng);
-      if (c->layer[q] == 0) continue;
-      if (p < 0 || fire_of(q) < fire_of(p)) p = q;
-    }
-    if (p < 0) continue;
-    const int q = grow_local(p);
-    if (q >= 0) {
-      c->adj[q].push_back({p, cfg->delay, cfg->w_grow_init, 0, -1});
-      ++grown;
-    }
-  }
-  for (auto& a : c->adj)
-    for (Edge& e : a)
-      if (e.age < kAgeInit) ++e.age;
-  rep->pruned = pruned;
-  rep->grown = grown;
+  StructuralStep step(c, *cfg, neuron_fires, n_neur, activity_norm, log_grown != 0);
+  int rc = step.prune(regrowth_quota(*cfg, reward));
+  if (rc) return rc;
+  const bool learned = grow_mode != SNC_GROW_HEURISTIC && model != nullptr;
+  if (learned)
+    rc = step.grow_learned(model, grow_mode == SNC_GROW_LEARNED_POOL, rep);
+  else
+    step.grow_heuristic();
+  if (rc) return rc;
+  step.grow_tournament();
+  step.age_edges();
+  rep->pruned = step.pruned();
+  rep->grown = step.grown();
   rep->synapses = c->num_synapses();
   return SNC_OK;
 }

@@ -38,13 +38,7 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 constexpr int kScoreDepth = 4;  // candidate tiles whose loads a wave keeps in flight
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__host__ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
-  uint32_t u;
-  memcpy(&u, &f, 4);
-  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+__host__ __device__ __forceinline__ uint16_t f32_to_bf16(float f) { return bf16_rne(f); }
 __host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
   const uint32_t u = (uint32_t)h << 16;
   float f;
@@ -57,7 +51,9 @@ __device__ __forceinline__ float q(float f) { return bf16_to_f32(f32_to_bf16(f))
 // inference (MFMA)
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_mlp_score(const MlpDev p, const float* __restrict__ X,
+// X holds the candidates' features as bf16 bits, [M][16]: the layer-1 MFMA
+// takes its B operand in bf16, so 32 bytes per candidate is all it reads.
+__global__ __launch_bounds__(256) void k_mlp_score(const MlpDev p, const uint16_t* __restrict__ X,
                                                    int64_t M, float* __restrict__ out) {
   const int lane = lane_id();
   const int r = lane & 15;          // A row / B,D column within a 16-block
@@ -92,12 +88,11 @@ __global__ __launch_bounds__(256) void k_mlp_score(const MlpDev p, const float* 
   // Each wave takes kScoreDepth tiles per round and issues all their feature
   // loads before the first MFMA, so one memory latency covers the round.
   for (int64_t base = wave; base < tiles; base += kScoreDepth * nwaves) {
-    float4 xs[kScoreDepth];
+    uint2 xs[kScoreDepth];  // features kq..kq+3 of candidate r: four bf16
 #pragma unroll
     for (int u = 0; u < kScoreDepth; ++u) {
       const int64_t cand = (base + u * nwaves) * 16 + r;
-      xs[u] = cand < M ? *reinterpret_cast<const float4*>(X + cand * kMlpIn + kq)
-                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      xs[u] = cand < M ? *reinterpret_cast<const uint2*>(X + cand * kMlpIn + kq) : make_uint2(0u, 0u);
     }
 #pragma unroll
     for (int u = 0; u < kScoreDepth; ++u) {
@@ -106,10 +101,10 @@ __global__ __launch_bounds__(256) void k_mlp_score(const MlpDev p, const float* 
       const int64_t cand = tile * 16 + r;
       // B operand of layer 1: X^T, lane holds features kq..kq+3 of candidate r.
       bf16x4 bx;
-      bx[0] = (short)f32_to_bf16(xs[u].x);
-      bx[1] = (short)f32_to_bf16(xs[u].y);
-      bx[2] = (short)f32_to_bf16(xs[u].z);
-      bx[3] = (short)f32_to_bf16(xs[u].w);
+      bx[0] = (short)(xs[u].x & 0xffffu);
+      bx[1] = (short)(xs[u].x >> 16);
+      bx[2] = (short)(xs[u].y & 0xffffu);
+      bx[3] = (short)(xs[u].y >> 16);
       bf16x4 h1[4];
 #pragma unroll
       for (int ob = 0; ob < 4; ++ob) {
@@ -252,7 +247,7 @@ __global__ void k_mlp_quantize(const float* __restrict__ w, MlpDev p) {
 // launchers
 // ---------------------------------------------------------------------------
 
-int mlp_launch_score(const MlpDev& p, const float* X, int64_t M, float* out, int blocks,
+int mlp_launch_score(const MlpDev& p, const uint16_t* X, int64_t M, float* out, int blocks,
                      hipStream_t st) {
   hipLaunchKernelGGL(k_mlp_score, dim3(blocks), dim3(256), 0, st, p, X, M, out);
   return hip_last("MLP score launch failed");
@@ -282,5 +277,9 @@ int mlp_launch_quantize(const float* w, const MlpDev& p, hipStream_t st) {
 }
 
 uint16_t mlp_bf16(float f) { return f32_to_bf16(f); }
+
+void mlp_features_bf16(const float* x, int64_t count, uint16_t* out) {
+  for (int64_t k = 0; k < count; ++k) out[k] = f32_to_bf16(x[k]);
+}
 
 }  // namespace snc_amd

@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib as L
 from .graph import _from_buf
 
-GROW_HEURISTIC, GROW_LEARNED = 0, 1
+GROW_HEURISTIC, GROW_LEARNED, GROW_LEARNED_POOL = 0, 1, 2
 
 
 class StructConfigC(C.Structure):
@@ -58,6 +58,7 @@ _SIGS = {
     "snc_conn_sheet_recurrent": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32,
                                            C.c_float, C.c_int32, C.c_float, C.c_uint64,
                                            C.POINTER(C.c_void_p)]),
+    "snc_conn_prune_to": (C.c_int, [C.c_void_p, C.c_int32]),
     "snc_conn_free": (None, [C.c_void_p]),
     "snc_conn_num_synapses": (C.c_int32, [C.c_void_p]),
     "snc_conn_to_graph": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -105,6 +106,10 @@ class Connectome:
         L.track(obj)
         obj.layers = list(layers)
         return obj
+
+    def prune_to(self, budget):
+        """Remove the smallest-|w| synapses until at most `budget` remain."""
+        L.check(L.lib().snc_conn_prune_to(self._h, int(budget)))
 
     @property
     def num_synapses(self) -> int:

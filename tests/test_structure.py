@@ -189,3 +189,24 @@ def test_sheet_candidate_distance_feature():
     post = np.array([64, 80, 99, 71], np.int32)
     F = con.candidate_features(pre, post, np.zeros(104, np.int64))
     assert np.array_equal(F[:, 3], _dist(x, y, pre, post).astype(np.float32))
+
+
+def test_prune_to_keeps_the_largest_abs_weights():
+    con = Connectome.sheet_recurrent([64, 36, 4], fan_in=8, rec_fan_in=4, radius=0.2, seed=5)
+    m = con.num_synapses
+    rng = np.random.default_rng(5)
+    w = (rng.standard_normal(m) * 0.2).astype(np.float32)
+    w[::7] = w[3]  # ties: the earlier CSR position goes first
+    con.set_edge_weights(w)
+    g0 = con.to_graph()
+    con.prune_to(m)  # at the budget already: unchanged
+    assert con.num_synapses == m
+    con.prune_to(100)
+    assert con.num_synapses == 100
+    order = sorted(range(m), key=lambda e: (abs(float(w[e])), e))
+    keep = np.zeros(m, bool)
+    keep[order[m - 100:]] = True
+    # survivors in CSR order, with their weights and posts
+    assert np.array_equal(con.edge_weights(), w[keep])
+    g1 = con.to_graph()
+    assert np.array_equal(g1.post_ids, np.asarray(g0.post_ids)[keep])

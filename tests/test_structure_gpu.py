@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from snc_amd.grow import GrowMLP
-from snc_amd.structure import GROW_HEURISTIC, GROW_LEARNED, Connectome, StructConfig
+from snc_amd.structure import (GROW_HEURISTIC, GROW_LEARNED, GROW_LEARNED_POOL, Connectome,
+                               StructConfig)
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +28,7 @@ def _graph_equal(a, b):
             and np.array_equal(ga.weights, gb.weights))
 
 
-@pytest.mark.parametrize("mode", [GROW_HEURISTIC, GROW_LEARNED])
+@pytest.mark.parametrize("mode", [GROW_HEURISTIC, GROW_LEARNED, GROW_LEARNED_POOL])
 def test_device_step_equals_host_step(mode):
     host, dev = _pair(7)
     mlp = GrowMLP(seed=3)
@@ -52,8 +53,11 @@ def test_device_step_equals_host_step(mode):
     Xa, ya = host.grow_log()
     Xb, yb = dev.grow_log()
     assert np.array_equal(Xa.view(np.uint32), Xb.view(np.uint32)) and np.array_equal(ya, yb)
-    if mode == GROW_LEARNED:
+    if mode != GROW_HEURISTIC:
         assert Xa.shape[0] > 0
+    if mode == GROW_LEARNED_POOL:
+        # every window candidate of every non-input post was scored
+        assert a["candidates_scored"] > 20 * sum(LAYERS[1:])
 
 
 def test_device_cotraining_learns():
