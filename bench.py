@@ -583,7 +583,7 @@ def main():
             mlp.score(X)
             kms.append(mlp.last_kernel_ms)
         k = sum(kms) / len(kms)
-        nbytes = (4 * N_FEATURES + 4) * m  # fp32 features in, fp32 score out
+        nbytes = (2 * N_FEATURES + 4) * m  # bf16 features in, fp32 score out
         flops = 2 * (16 * 64 + 64 * 64 + 64) * m  # the three layers' multiply-adds
         print(json.dumps({"grow_score": {
             "candidates": m, "calls": args.steps, "kernel_ms": k,
@@ -737,6 +737,10 @@ def main():
 
         out["grow_ab"] = ab_compare(log=log)
         out["grow_ab_sheet"] = ab_compare(log=log, kind="sheet")
+        # the structural co-training workload as stated: the batched e-prop
+        # network under 3000 synapses, the whole candidate pool scored
+        out["grow_ab_c4"] = ab_compare(log=log, kind="c4", label="survival", w_grow=0.2,
+                                       onpolicy=1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_repeats)
     print(json.dumps(out))

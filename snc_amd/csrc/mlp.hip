@@ -110,9 +110,12 @@ __global__ __launch_bounds__(256) void k_mlp_score(const MlpDev p, const uint16_
       for (int ob = 0; ob < 4; ++ob) {
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
         c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ob], bx, c, 0, 0, 0);
+        // ReLU outputs are finite and >= 0, where the hardware's packed
+        // conversion (v_cvt_pk_bf16_f32, round to nearest even) equals
+        // f32_to_bf16; it converts two values per instruction.
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          h1[ob][i] = (short)f32_to_bf16(fmaxf(c[i] + b1v[ob][i], 0.0f));
+          h1[ob][i] = __builtin_bit_cast(short, (__bf16)fmaxf(c[i] + b1v[ob][i], 0.0f));
       }
       float part = 0.0f;
 #pragma unroll
