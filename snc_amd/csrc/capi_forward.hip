@@ -133,7 +133,8 @@ struct snc_hip_plan {
   size_t bits_total = 0, dirty_total = 0, ring_total = 0;  // elements allocated
   std::vector<int64_t> last_spikes, last_events;           // per sample, last run
   int4* sseg = nullptr;  // hip-sort segments, each synapse's segment, lists, counters
-  int32_t *sseg_e = nullptr, *slist = nullptr, *scnt = nullptr, *sinj_ptr = nullptr;
+  int32_t *sseg_e = nullptr, *scnt = nullptr, *sinj_ptr = nullptr;
+  uint2* slist = nullptr;
   int n_seg = 0;
   std::vector<int32_t> scnt_init;  // scnt at reset: each counter at its segment's first column position
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // [raster off, on]
@@ -183,11 +184,13 @@ struct snc_hip_plan {
   }
 };
 
-// State of the hip-sort strategy: the target segments, their event lists
-// and counters per step parity, and each step's injections per segment. A
-// segment's list holds at most one event per column position of the segment
-// per step (a pre fires at most once per step), so the lists of one parity
-// share one P-sized array, segment s at its first column position.
+// State of the hip-sort strategy: the target segments, their arrival queues
+// and counters per ring slot, and each step's injections per segment. A
+// queue of slot s holds at most one record per column position of its
+// segment (a synapse deposits into slot s once, and the queue is emptied at
+// the step that reads slot s before the synapse can deposit into it again),
+// so the queues of one slot share one P-sized array, segment s at its first
+// column position.
 static int sort_alloc(snc_hip_plan* P, const std::vector<int32_t>& inj_ptr,
                       const std::vector<int32_t>& inj_nid) {
   const DevGraph& g = P->G->g;
@@ -207,12 +210,16 @@ static int sort_alloc(snc_hip_plan* P, const std::vector<int32_t>& inj_ptr,
   P->n_seg = (int)seg.size();
   if (!rc) rc = dupload(&P->sseg, seg.data(), seg.size());
   if (!rc) rc = dupload(&P->sseg_e, sse.data(), sse.size());
-  if (!rc) rc = dalloc(&P->slist, 2 * (size_t)std::max<int64_t>(1, g.P));
-  if (!rc) rc = dalloc(&P->scnt, 2 * (size_t)std::max(1, P->n_seg) * kSortCntStride);
-  P->scnt_init.assign(2 * (size_t)P->n_seg * kSortCntStride, 0);
-  for (int q = 0; q < P->n_seg; ++q)
-    P->scnt_init[(size_t)q * kSortCntStride] =
-        P->scnt_init[(size_t)(P->n_seg + q) * kSortCntStride] = seg[q].z;
+  const size_t L = (size_t)g.L;
+  if (L * (size_t)std::max<int64_t>(1, g.P) * sizeof(uint2) > (size_t)kSortQueueBytesMax)
+    return fail(SNC_E_UNSUPPORTED, "hip-sort arrival queues (ring slots x column positions x 8 B) "
+                                   "above 64 GiB");
+  if (!rc) rc = dalloc(&P->slist, L * (size_t)std::max<int64_t>(1, g.P));
+  if (!rc) rc = dalloc(&P->scnt, L * (size_t)std::max(1, P->n_seg) * kSortCntStride);
+  P->scnt_init.assign(L * (size_t)P->n_seg * kSortCntStride, 0);
+  for (size_t sl = 0; sl < L; ++sl)
+    for (int q = 0; q < P->n_seg; ++q)
+      P->scnt_init[(sl * P->n_seg + q) * kSortCntStride] = seg[q].z;
   const std::vector<int32_t> sinj = sort_injection_ranges(seg, inj_ptr, inj_nid, P->T);
   if (!rc) rc = dupload(&P->sinj_ptr, sinj.data(), sinj.size());
   return rc;

@@ -30,6 +30,8 @@ constexpr int kInjSpill = -2;
 // cache line of its own, so the expansion's appends to different segments
 // do not queue on shared lines.
 constexpr int kSortCntStride = 32;
+// Largest hip-sort queue storage a plan allocates (L x P records of 8 bytes).
+constexpr long long kSortQueueBytesMax = 64ll << 30;
 
 struct StepArgs {
   int n, L, T, words;
@@ -62,12 +64,12 @@ struct StepArgs {
   unsigned long long* blk_events;
   uint32_t* raster;  // [T][words] or null
   uint32_t* fbits;   // [words] fired bitmap of the current step (hip-sort) or null
-  // hip-sort: target segments and their event lists, by step parity
+  // hip-sort: target segments and their arrival queues, per ring slot
   const int4* sseg;            // [n_seg] {first target, end target, first position, end position}
   const int32_t* sseg_e;       // [S] segment of each synapse's target
-  int32_t* slist;              // [2][P] event lists: segment s's at sseg[s].z
+  uint2* slist;                // [L][P] queue records {column position, value bits}: segment s's at sseg[s].z
   int64_t slist_stride;        // P
-  int32_t* scnt;               // [2][n_seg][kSortCntStride]: next free list position per step parity and segment
+  int32_t* scnt;               // [L][n_seg][kSortCntStride]: next free queue position per slot and segment
   const int32_t* sinj_ptr;     // [T][n_seg + 1] each step's injections, per segment
   int n_seg;
   int sort_slot0;              // first stats slot of the segment workgroups

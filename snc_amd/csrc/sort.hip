@@ -1,37 +1,39 @@
-// hip-sort: deterministic delivery by sorting each step's events.
+// hip-sort: deterministic delivery by sorting each step's arrivals.
 //
 // Sort key. Every synapse has a column position cpos[e]: its rank in the
 // order (post, delay descending, pre, CSR position) of the padded incoming
-// columns. For one step, the deposits onto one (target, ring slot) share a
-// delay, so sorting the step's events by column position puts them in
-// (pre, CSR position) order -- the order the CPU path adds them
-// (cpu_backend.cpp:173-181) -- and the sorted runs of a target are its ring
-// slots' runs. Adding each run, in that order, onto the slot's value is the
-// CPU's sum bit for bit. This replaces the reference's Thrust sort_by_key +
-// reduce_by_key over all S synapses (cuda_backend.cu:138-169, 317-328), whose
-// segmented reduction has no fixed association order.
+// columns. The deposits that arrive at one target at one step come from
+// steps t - d in order of their origin (longest delay first) and, within an
+// origin step, in (pre, CSR position) order -- the order the CPU path adds
+// them onto the ring slot it reads at step t (cpu_backend.cpp:173-181). That
+// is column order, so sorting a step's arrivals by column position and
+// adding each target's run from zero is the CPU's slot sum bit for bit. This
+// replaces the reference's Thrust sort_by_key + reduce_by_key over all S
+// synapses (cuda_backend.cu:138-169, 317-328), whose segmented reduction has
+// no fixed association order.
 //
 // Targets are cut into segments: runs of at most 1024 targets, starting on a
-// multiple of 32, whose columns span at most kWin positions. Each segment has
-// an event list per step parity. One launch per step, one workgroup per
+// multiple of 32, whose columns span at most kWin positions. Each (ring slot,
+// segment) pair has an arrival queue of records {column position, value}: a
+// deposit made at step t with delay d is appended to the queue of slot
+// (t + d) % L of its target's segment. One launch per step, one workgroup per
 // segment (k_sort_step):
-//   1. sort the segment's events of step t by counting: presence bits in
-//      LDS, an exclusive popcount prefix over the words (every event's sorted
-//      rank), the events' values and slots staged at their ranks;
-//   2. fold each target's sorted runs onto the ring: one thread per target,
-//      or, for a target with many events this step, one wave whose lanes take
-//      one run (one ring slot) each;
-//   3. integrate/fire the segment's targets for step t + 1 (their injections
-//      come from the step's per-segment range);
-//   4. expand the segment's fired rows: every event of step t + 1 (its
-//      column position) is appended to its target segment's list of the
-//      other parity, with one atomic per distinct segment per wave (the list
-//      order is arbitrary; step 1 sorts it).
-// A launch before step 0 does 3 and 4 for step 0. A segment longer than kWin
-// positions (a hub's column) is sorted one window of kWin positions at a
-// time, and a window with more events than the staging holds is folded in
-// rank order, kStage events at a time; both keep every (target, slot) sum in
-// sorted order.
+//   1. take the segment's queue of slot t % L (and reset it: the next
+//      deposits into that slot are made at step t + 1 or later),
+//   2. sort it by counting: presence bits of the column positions in LDS,
+//      an exclusive popcount prefix over the words (every record's rank),
+//      the values staged at their ranks,
+//   3. each target adds its run of staged values in order, from zero, and
+//      integrates/fires for step t (its injections come from the step's
+//      per-segment range),
+//   4. expand the segment's fired rows: one atomic reservation and one
+//      8-byte record store per deposit (queue order is arbitrary; step 2
+//      sorts it).
+// A segment longer than kWin positions (a hub's column) is sorted one window
+// of kWin positions at a time, and a window with more records than the
+// staging holds is added in rank order, kStage values at a time; both keep
+// every target's sum in column order. There is no ring: a queue holds the
+// deposits themselves, so no target's partial sum is read back or written.
 
 #include <hip/hip_runtime.h>
 
@@ -50,19 +52,18 @@ constexpr int kSortThreads = 1024;  // = most targets of a segment
 constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kWin = 65536;         // column positions per window
 constexpr int kWinWords = kWin / 32;
-constexpr int kStage = 4096;        // staged events per fold round
-constexpr int kCache = 4096;        // event positions kept in LDS
-constexpr int kLightRun = 16;       // a target with more events this round folds by wave
+constexpr int kStage = 4096;        // staged values per round
+constexpr int kCache = 4096;        // records kept in LDS
 
 // Dynamic LDS of k_sort_step: presence bits, word prefixes (+ wave sums),
-// cached positions, staged events, per-target staging (next-step drive,
-// wave-fold list, fired list).
+// cached records, staged values, per-target staging (injections, fired
+// rows).
 constexpr int kPrefInts = kWinWords + 1 + kSortWaves + 15;
 constexpr size_t kSortLds = sizeof(uint32_t) * kWinWords + sizeof(int) * kPrefInts +
-                            sizeof(uint32_t) * kCache + sizeof(uint2) * kStage +
-                            sizeof(float) * kSortThreads + 2 * sizeof(int32_t) * kSortThreads;
-static_assert(kPrefInts % 4 == 0, "the position cache must stay 16-byte aligned");
-static_assert(kSortLds <= 160 * 1024, "hip-sort workgroup LDS above the CU's 160 KB");
+                            sizeof(uint2) * kCache + sizeof(float) * kStage +
+                            sizeof(float) * kSortThreads + sizeof(int2) * kSortThreads;
+static_assert(kPrefInts % 4 == 0, "the record cache must stay 16-byte aligned");
+static_assert(kSortLds <= 80 * 1024, "two hip-sort workgroups must fit in the CU's 160 KB of LDS");
 
 // ---------------------------------------------------------------------------
 // host: segments and per-segment injection ranges
@@ -109,148 +110,81 @@ std::vector<int32_t> sort_injection_ranges(const std::vector<int4>& seg,
 // device
 // ---------------------------------------------------------------------------
 
-// Ordered fold of target j's staged events [r0, r1) (column order) onto its
-// ring slots: consecutive events of one slot are one run, added in order onto
-// the slot's value; the slot values of up to kBatch runs load together. Slot
-// s1 (the next step's, which the same thread integrates right after) is not
-// read or written here: its value lives in the register slot1.
-__device__ __forceinline__ void fold_sorted(const uint2* s_ev, int r0, int r1, float* ring,
-                                            int n, int j, int s1, float& slot1) {
-  constexpr int kBatch = 8;
-  int cur = -1;
-  float acc = 0.0f;
-  auto put = [&](int sl, float v) {
-    if (sl == s1) slot1 = v;
-    else ring[(size_t)sl * n + j] = v;
-  };
-  for (int r = r0; r < r1; r += kBatch) {
-    int sl[kBatch];
-    float val[kBatch], old[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      sl[u] = -1;
-      val[u] = 0.0f;
-      if (r + u < r1) {
-        val[u] = __uint_as_float(s_ev[r + u].x);
-        sl[u] = (int)s_ev[r + u].y;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u)
-      old[u] = sl[u] >= 0 && sl[u] != cur && sl[u] != s1 ? ring[(size_t)sl[u] * n + j] : 0.0f;
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      if (sl[u] < 0) break;
-      if (sl[u] != cur) {
-        if (cur >= 0) put(cur, acc);
-        cur = sl[u];
-        acc = cur == s1 ? slot1 : old[u];
-      }
-      acc = __fadd_rn(acc, val[u]);
-    }
-  }
-  if (cur >= 0) put(cur, acc);
-}
-
-// Wave fold of target j's staged events [r0, r1). Within one step a target's
-// events of one ring slot are one contiguous run (column order is delay
-// descending), so the wave marks where each run starts, 64 positions at a
-// time, and the lane holding a start adds that run, in order, onto its
-// slot's value: the runs (one per slot) proceed in parallel.
-__device__ __forceinline__ void fold_wave(const uint2* s_ev, int r0, int r1, float* ring, int n,
-                                          int j) {
-  const int lane = lane_id();
-  for (int base = r0; base < r1; base += kWave) {
-    const int r = base + lane;
-    if (r >= r1) continue;
-    const uint32_t sl = s_ev[r].y;
-    if (r > r0 && s_ev[r - 1].y == sl) continue;  // not the start of a run
-    float* dst = ring + (size_t)sl * n + j;
-    float acc = *dst;
-    int q = r;
-    for (; q + 4 <= r1 && s_ev[q + 3].y == sl; q += 4) {
-      acc = __fadd_rn(acc, __uint_as_float(s_ev[q].x));
-      acc = __fadd_rn(acc, __uint_as_float(s_ev[q + 1].x));
-      acc = __fadd_rn(acc, __uint_as_float(s_ev[q + 2].x));
-      acc = __fadd_rn(acc, __uint_as_float(s_ev[q + 3].x));
-    }
-    for (; q < r1 && s_ev[q].y == sl; ++q) acc = __fadd_rn(acc, __uint_as_float(s_ev[q].x));
-    *dst = acc;
-  }
-}
-
-// One step of one segment; t = -1 is the launch before step 0 (no fold).
-// Eight waves per SIMD (at most 64 VGPRs and 96 SGPRs; the kernel arguments
-// spill to VGPR lanes): two workgroups fit on a CU, LDS included, so graphs of
-// up to 512 segments run every segment at once on the 256 CUs.
+// One step t of one segment. Eight waves per SIMD (at most 64 VGPRs and 96
+// SGPRs; the kernel arguments spill to VGPR lanes): two workgroups fit on a
+// CU, LDS included, so graphs of up to 512 segments run every segment at
+// once on the 256 CUs.
 __global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 k_sort_step(StepArgs a, int t) {
   extern __shared__ __align__(16) unsigned char smem[];
-  uint32_t* s_bits = (uint32_t*)smem;                        // [kWinWords]
-  int* s_pref = (int*)(s_bits + kWinWords);                  // [kWinWords + 1]
-  int* s_wave = s_pref + kWinWords + 1;                      // [kSortWaves]
-  uint32_t* s_pos = (uint32_t*)(s_pref + kPrefInts);         // [kCache]
-  uint2* s_ev = (uint2*)(s_pos + kCache);                    // [kStage]
-  float* s_ext = (float*)(s_ev + kStage);                    // [kSortThreads] next-step drive
-  int32_t* s_heavy = (int32_t*)(s_ext + kSortThreads);       // [kSortThreads] wave-fold list, then row ends
-  int32_t* s_fired = s_heavy + kSortThreads;                 // [kSortThreads] wave-fold ids, then row starts
-  __shared__ int s_count, s_nheavy;
-  __shared__ unsigned long long s_ev_total;
+  uint32_t* s_bits = (uint32_t*)smem;                 // [kWinWords]
+  int* s_pref = (int*)(s_bits + kWinWords);           // [kWinWords + 1]
+  int* s_wave = s_pref + kWinWords + 1;               // [kSortWaves]
+  uint2* s_rec = (uint2*)(s_pref + kPrefInts);        // [kCache]
+  float* s_val = (float*)(s_rec + kCache);            // [kStage]
+  float* s_ext = s_val + kStage;                      // [kSortThreads] this step's injections
+  int2* s_rows = (int2*)(s_ext + kSortThreads);       // [kSortThreads] fired rows
+  __shared__ int s_count;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const int sg = blockIdx.x;
   const int4 g = a.sseg[sg];  // targets [x, y), column positions [z, w)
   const int j = g.x + tid;    // this thread's target
   const bool own = j < g.y;
-  const int tm = t < 0 ? -1 : t % a.L;  // slot of step t (t = -1: none)
-  const bool lif = t + 1 < a.T;
-  int s1 = tm + 1;                      // slot of step t + 1
-  if (s1 >= a.L) s1 -= a.L;
-  const int par = t & 1;                // list parity of step t's events (t >= 0)
-  // the next step's state of the own target, loaded before the sort
-  float v = 0.0f, slot1 = 0.0f;
+  const int tm = t % a.L;
+  // this step's injection range of the segment, with the first loads
+  const int* ir = a.sinj_ptr + (size_t)t * (a.n_seg + 1) + sg;
+  const int inj0 = ir[0], inj1 = ir[1];
+  // the own target's state, loaded before the sort
+  float v = 0.0f, in = 0.0f;
   int refr = 0, cj0 = 0, cj1 = 0, rb0 = 0, re0 = 0;
   if (own) {
     cj0 = a.ccol[j];
     cj1 = a.ccol[j + 1];
-    if (lif) {
-      v = a.v[j];
-      refr = a.refr[j];
-      rb0 = a.row_ptr[j];  // the own row, expanded below if the target fires
-      re0 = a.row_ptr[j + 1];
-    }
-    slot1 = a.ring[(size_t)s1 * a.n + j];
+    v = a.v[j];
+    refr = a.refr[j];
+    rb0 = a.row_ptr[j];  // the own row, expanded below if the target fires
+    re0 = a.row_ptr[j + 1];
+  }
+  // The queue of slot tm; its first records are loaded before the count is
+  // known (within the queue's capacity, so in bounds).
+  const uint2* q = a.slist + (size_t)tm * a.slist_stride + g.z;
+  const int cap = g.w - g.z;
+  uint2 head = make_uint2(0u, 0u);
+  if (tid < cap) head = q[tid];
+  // the first injection entry of this thread, loaded in the same round
+  int inj_k = inj0 + tid, inj_n = -1;
+  float inj_a = 0.0f;
+  if (inj_k < inj1) {
+    inj_n = a.inj_nid[inj_k];
+    inj_a = a.inj_amp[inj_k];
   }
   s_ext[tid] = 0.0f;
   if (tid == 0) {
-    // the counters hold the next free list position (absolute, from the
-    // segment's first column position), so appends need no segment table
-    s_count = 0;
-    if (t >= 0) {
-      int32_t* cnt = a.scnt + (size_t)(par * a.n_seg + sg) * kSortCntStride;
-      s_count = *cnt - g.z;
-      *cnt = g.z;  // step t + 2's expansion appends from the start
-    }
-    s_nheavy = 0;
-    s_ev_total = 0ull;
+    // the counter holds the next free queue position, absolute (from the
+    // segment's first column position), so producers need no segment table
+    int32_t* c = a.scnt + ((size_t)tm * a.n_seg + sg) * kSortCntStride;
+    s_count = *c - g.z;
+    *c = g.z;  // deposits into slot tm resume at step t + 1
   }
   __syncthreads();
-  if (lif) {  // step t + 1's injections into this segment's targets
-    const int* r = a.sinj_ptr + (size_t)(t + 1) * (a.n_seg + 1) + sg;
-    for (int k = r[0] + tid; k < r[1]; k += kSortThreads) s_ext[a.inj_nid[k] - g.x] = a.inj_amp[k];
-  }
+  // this step's injections into the segment's targets
+  if (inj_n >= 0) s_ext[inj_n - g.x] = inj_a;
+  for (int k = inj_k + kSortThreads; k < inj1; k += kSortThreads) s_ext[a.inj_nid[k] - g.x] = a.inj_amp[k];
   const int count = s_count;
-  const int32_t* lst = a.slist + (size_t)par * a.slist_stride + g.z;
   const bool cached = count <= kCache;
-  if (cached)
-    for (int k = tid; k < count; k += kSortThreads) s_pos[k] = (uint32_t)lst[k];
-  auto pos = [&](int k) { return cached ? (int)s_pos[k] : lst[k]; };
+  if (cached) {
+    if (tid < count) s_rec[tid] = head;
+    for (int k = tid + kSortThreads; k < count; k += kSortThreads) s_rec[k] = q[k];
+  }
+  // record k is only ever read by thread k % kSortThreads, its writer
+  auto rec = [&](int k) { return cached ? s_rec[k] : q[k]; };
   for (int wp = g.z; count > 0 && wp < g.w; wp += kWin) {
     const int wend = min(wp + kWin, g.w);
     const int nw = (wend - wp) >> 5;
-    for (int q = tid; q < nw; q += kSortThreads) s_bits[q] = 0u;
+    for (int w = tid; w < nw; w += kSortThreads) s_bits[w] = 0u;
     __syncthreads();
     for (int k = tid; k < count; k += kSortThreads) {
-      const int p = pos(k);
+      const int p = (int)rec(k).x;
       if (p >= wp && p < wend) atomicOr(&s_bits[(p - wp) >> 5], 1u << (p & 31));
     }
     __syncthreads();
@@ -259,8 +193,8 @@ k_sort_step(StepArgs a, int t) {
     int c = 0;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      const int q = tid * kPer + u;
-      c += q < nw ? __popc(s_bits[q]) : 0;
+      const int w = tid * kPer + u;
+      c += w < nw ? __popc(s_bits[w]) : 0;
     }
     const int incl = wave_inclusive_scan(c);
     if (lane == kWave - 1) s_wave[wave] = incl;
@@ -269,16 +203,16 @@ k_sort_step(StepArgs a, int t) {
     for (int k = 0; k < wave; ++k) run += s_wave[k];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      const int q = tid * kPer + u;
-      if (q < nw) {
-        s_pref[q] = run;
-        run += __popc(s_bits[q]);
+      const int w = tid * kPer + u;
+      if (w < nw) {
+        s_pref[w] = run;
+        run += __popc(s_bits[w]);
       }
     }
     if (tid == kSortThreads - 1) s_pref[nw] = run;  // the last thread's running total
     __syncthreads();
     const int total = s_pref[nw];
-    // this thread's target: its events are the ranks of its column's words
+    // this thread's target: its arrivals are the ranks of its column's words
     int tr0 = 0, tr1 = 0;
     if (own) {
       const int a0 = max(cj0, wp), a1 = min(cj1, wend);
@@ -290,56 +224,33 @@ k_sort_step(StepArgs a, int t) {
     for (int r0 = 0; r0 < total; r0 += kStage) {
       const int r1 = min(total, r0 + kStage);
       for (int k = tid; k < count; k += kSortThreads) {
-        const int p = pos(k);
+        const uint2 e = rec(k);
+        const int p = (int)e.x;
         if (p < wp || p >= wend) continue;
-        const int q = (p - wp) >> 5;
-        const int rk = s_pref[q] + __popc(s_bits[q] & ((1u << (p & 31)) - 1u));
-        if (rk < r0 || rk >= r1) continue;
-        int s = tm + (int)a.cdel[p];
-        if (s >= a.L) s -= a.L;
-        s_ev[rk - r0] = make_uint2(__float_as_uint(a.cval[p]), (uint32_t)s);
+        const int w = (p - wp) >> 5;
+        const int rk = s_pref[w] + __popc(s_bits[w] & ((1u << (p & 31)) - 1u));
+        if (rk >= r0 && rk < r1) s_val[rk - r0] = __uint_as_float(e.y);
       }
       __syncthreads();
-      const int f0 = max(tr0, r0), f1 = min(tr1, r1);
-      const bool heavy = f1 - f0 > kLightRun;
-      if (heavy) {
-        // the wave fold reads and writes every slot in the ring: the register
-        // copy of slot s1 goes back first, and comes back after it
-        a.ring[(size_t)s1 * a.n + j] = slot1;
-        // the target's ranks in this round (both below kStage) and its id;
-        // the fired list's space is free until the integrate/fire
-        const int h = atomicAdd(&s_nheavy, 1);
-        s_heavy[h] = (f0 - r0) | ((f1 - r0) << 16);
-        s_fired[h] = j;
-      } else if (f0 < f1) {
-        fold_sorted(s_ev, f0 - r0, f1 - r0, a.ring, a.n, j, s1, slot1);
+      // the own run, in column order, four staged values per round
+      int r = max(tr0, r0) - r0;
+      const int re = min(tr1, r1) - r0;
+      for (; r + 4 <= re; r += 4) {
+        const float x0 = s_val[r], x1 = s_val[r + 1], x2 = s_val[r + 2], x3 = s_val[r + 3];
+        in = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(in, x0), x1), x2), x3);
       }
-      __syncthreads();
-      const int nh = s_nheavy;
-      for (int h = wave; h < nh; h += kSortWaves) {
-        const int e = s_heavy[h];
-        fold_wave(s_ev, e & 0xffff, e >> 16, a.ring, a.n, s_fired[h]);
-      }
-      __syncthreads();
-      if (heavy) slot1 = a.ring[(size_t)s1 * a.n + j];
-      if (tid == 0) s_nheavy = 0;
+      for (; r < re; ++r) in = __fadd_rn(in, s_val[r]);
       __syncthreads();
     }
   }
-  if (!lif) {
-    // last step: the deposits onto slot s1 stay in the ring
-    if (own) a.ring[(size_t)s1 * a.n + j] = slot1;
-    return;
-  }
-  // integrate/fire of the segment's targets for step t + 1 (the barrier
-  // orders the injections' LDS writes before their reads when no fold ran)
+  // integrate/fire of step t (the barrier orders the injections' LDS writes
+  // before their reads when no sort ran)
   __syncthreads();
   bool f = false;
   if (own) {
-    f = lif_step(v, refr, s_ext[tid], slot1, a.lif);
+    f = lif_step(v, refr, s_ext[tid], in, a.lif);
     a.v[j] = v;
     a.refr[j] = refr;
-    a.ring[(size_t)s1 * a.n + j] = 0.0f;
     if (f) a.nspk[j] += 1;
   }
   // the fired list in id order: wave counts, their prefix, each lane's rank
@@ -351,72 +262,74 @@ k_sort_step(StepArgs a, int t) {
     off += k < wave ? s_wave[k] : 0;
     nf += s_wave[k];
   }
-  if (f) {
-    const int q = off + rank_below(m);
-    s_fired[q] = rb0;  // the fired rows, in id order: CSR bounds
-    s_heavy[q] = re0;
-  }
-  // The segment starts on a multiple of 32 targets, so a wave's raster words
-  // are its own.
+  if (f) s_rows[off + rank_below(m)] = make_int2(rb0, re0);
+  // The segment starts on a multiple of 32 targets and only own targets
+  // fire, so every nonzero raster word of a wave is the segment's own.
   if (a.raster && lane == 0 && m) {
-    uint32_t* row = a.raster + (size_t)(t + 1) * a.words + ((g.x + wave * kWave) >> 5);
+    uint32_t* row = a.raster + (size_t)t * a.words + ((g.x + wave * kWave) >> 5);
     if ((uint32_t)m) row[0] = (uint32_t)m;
     if ((uint32_t)(m >> 32)) row[1] = (uint32_t)(m >> 32);
   }
   __syncthreads();
-  // expansion of the fired rows into step t + 1's lists: wave per row, each
-  // lane two events in flight
-  const int npar = (t + 1) & 1;
-  int32_t* nlist = a.slist + (size_t)npar * a.slist_stride;
-  int32_t* ncnt = a.scnt + (size_t)npar * a.n_seg * kSortCntStride;
-  unsigned long long ev = 0;
-  for (int k = wave; k < nf; k += kSortWaves) {
-    const int rb = s_fired[k], re = s_heavy[k];
-    ev += (unsigned long long)(re - rb);
-    for (int e0 = rb; e0 < re; e0 += 2 * kWave) {
-      int p[2], sgu[2], leader[2], rank[2], base[2];
-      bool valid[2];
+  // Expansion of the fired rows, flattened over the workgroup: deposit k of
+  // the rows' concatenation goes to thread k % kSortThreads, kExpand per
+  // thread per round, so a segment with many fired rows still pays one round
+  // of loads, one of reservations and one of stores. The exclusive prefix of
+  // the row lengths (in the sort's presence bits, free now) maps k to its
+  // row. Deposits of the last step would never be read and are skipped.
+  constexpr int kExpand = 4;
+  int* s_pre = (int*)s_bits;
+  const int my_len = tid < nf ? s_rows[tid].y - s_rows[tid].x : 0;
+  const int my_inc = wave_inclusive_scan(my_len);
+  if (lane == kWave - 1) s_wave[wave] = my_inc;  // every read of the fired counts came before
+  __syncthreads();
+  int below = 0, events = 0;
+  for (int k = 0; k < kSortWaves; ++k) {
+    below += k < wave ? s_wave[k] : 0;
+    events += s_wave[k];
+  }
+  if (tid < nf) s_pre[tid] = below + my_inc - my_len;
+  __syncthreads();
+  if (t + 1 < a.T) {
+    for (int k0 = 0; k0 < events; k0 += kExpand * kSortThreads) {
+      int p[kExpand], sgu[kExpand], sl[kExpand], idx[kExpand];
+      float w[kExpand];
+      bool valid[kExpand];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int e = e0 + u * kWave + lane;
-        valid[u] = e < re;
-        p[u] = valid[u] ? a.cpos[e] : 0;
-        sgu[u] = valid[u] ? a.sseg_e[e] : -1;
-      }
-      // group the wave's events by segment: every lane learns its group's
-      // leader and its rank in the group; leaders reserve list space with one
-      // atomic each, both halves in one round trip
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        uint64_t rem = ballot(valid[u]);
-        int cnt = 0;
-        leader[u] = 0;
-        rank[u] = 0;
-        while (rem) {
-          const int l = __builtin_ctzll(rem);
-          const int lseg = __shfl(sgu[u], l, kWave);
-          const uint64_t gm = ballot(valid[u] && sgu[u] == lseg);
-          if ((gm >> lane) & 1ull) {
-            leader[u] = l;
-            rank[u] = rank_below(gm);
+      for (int u = 0; u < kExpand; ++u) {
+        const int k = k0 + u * kSortThreads + tid;
+        valid[u] = k < events;
+        int e = 0;
+        if (valid[u]) {
+          int lo = 0, hi = nf - 1;  // the last row starting at or before k
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pre[mid] <= k) lo = mid;
+            else hi = mid - 1;
           }
-          if (lane == l) cnt = __popcll(gm);
-          rem &= ~gm;
+          e = s_rows[lo].x + (k - s_pre[lo]);
+          p[u] = a.cpos[e];
+          w[u] = a.wsig[e];
+          sgu[u] = a.sseg_e[e];
+          sl[u] = tm + (int)a.delay[e];
         }
-        base[u] = valid[u] && lane == leader[u] ? atomicAdd(ncnt + (size_t)sgu[u] * kSortCntStride, cnt) : 0;
       }
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int bs = __shfl(base[u], leader[u], kWave);
-        if (valid[u]) nlist[bs + rank[u]] = p[u];
-      }
+      for (int u = 0; u < kExpand; ++u)
+        if (valid[u]) {
+          if (sl[u] >= a.L) sl[u] -= a.L;
+          idx[u] = atomicAdd(a.scnt + ((size_t)sl[u] * a.n_seg + sgu[u]) * kSortCntStride, 1);
+        }
+#pragma unroll
+      for (int u = 0; u < kExpand; ++u)
+        if (valid[u])
+          a.slist[(size_t)sl[u] * a.slist_stride + idx[u]] =
+              make_uint2((uint32_t)p[u], __float_as_uint(w[u]));
     }
   }
-  if (lane == 0 && ev) atomicAdd(&s_ev_total, ev);
-  __syncthreads();
   if (tid == 0) {
     if (nf) a.blk_spikes[a.sort_slot0 + sg] += (unsigned long long)nf;
-    if (s_ev_total) a.blk_events[a.sort_slot0 + sg] += s_ev_total;
+    if (events) a.blk_events[a.sort_slot0 + sg] += (unsigned long long)events;
   }
 }
 
@@ -432,10 +345,7 @@ int sort_prepare() {
 }
 
 int sort_launch_step(const FwdPlanCore& pc, const StepArgs& a, int t) {
-  hipStream_t st = pc.stream;
-  if (t == 0)  // step 0's integrate/fire and expansion
-    hipLaunchKernelGGL(k_sort_step, dim3(a.n_seg), dim3(kSortThreads), kSortLds, st, a, -1);
-  hipLaunchKernelGGL(k_sort_step, dim3(a.n_seg), dim3(kSortThreads), kSortLds, st, a, t);
+  hipLaunchKernelGGL(k_sort_step, dim3(a.n_seg), dim3(kSortThreads), kSortLds, pc.stream, a, t);
   return hip_last("sort step launch failed");
 }
 
