@@ -54,6 +54,7 @@ constexpr int kWin = 65536;         // column positions per window
 constexpr int kWinWords = kWin / 32;
 constexpr int kStage = 4096;        // staged values per round
 constexpr int kCache = 4096;        // records kept in LDS
+constexpr int kHead = 512;          // records loaded before the queue's count is known
 
 // Dynamic LDS of k_sort_step: presence bits, word prefixes (+ wave sums),
 // cached records, staged values, per-target staging (injections, fired
@@ -145,12 +146,12 @@ k_sort_step(StepArgs a, int t) {
     rb0 = a.row_ptr[j];  // the own row, expanded below if the target fires
     re0 = a.row_ptr[j + 1];
   }
-  // The queue of slot tm; its first records are loaded before the count is
-  // known (within the queue's capacity, so in bounds).
+  // The queue of slot tm; its first kHead records are loaded before the
+  // count is known (within the queue's capacity, so in bounds).
   const uint2* q = a.slist + (size_t)tm * a.slist_stride + g.z;
   const int cap = g.w - g.z;
   uint2 head = make_uint2(0u, 0u);
-  if (tid < cap) head = q[tid];
+  if (tid < min(cap, kHead)) head = q[tid];
   // the first injection entry of this thread, loaded in the same round
   int inj_k = inj0 + tid, inj_n = -1;
   float inj_a = 0.0f;
@@ -173,8 +174,8 @@ k_sort_step(StepArgs a, int t) {
   const int count = s_count;
   const bool cached = count <= kCache;
   if (cached) {
-    if (tid < count) s_rec[tid] = head;
-    for (int k = tid + kSortThreads; k < count; k += kSortThreads) s_rec[k] = q[k];
+    if (tid < min(count, kHead)) s_rec[tid] = head;
+    for (int k = tid < kHead ? tid + kSortThreads : tid; k < count; k += kSortThreads) s_rec[k] = q[k];
   }
   // record k is only ever read by thread k % kSortThreads, its writer
   auto rec = [&](int k) { return cached ? s_rec[k] : q[k]; };
