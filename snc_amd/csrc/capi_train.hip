@@ -85,6 +85,7 @@ struct snc_hip_train {
   int32_t* idx = nullptr;
   float* hist = nullptr;  // [batch][max_steps][2][hn] psi / trace histories
   float *C = nullptr, *ring_g = nullptr, *logits = nullptr;
+  uint2* arq = nullptr;  // arrival queues [batch][L][S] (ring not in LDS)
   double* loss = nullptr;
   int32_t* correct = nullptr;
   unsigned long long *blk_spikes = nullptr, *blk_events = nullptr, *neur_fire = nullptr;
@@ -123,7 +124,8 @@ struct snc_hip_train {
                     (void*)ecol_pre, (void*)ecol_syn, (void*)wq, (void*)Ls_g, (void*)X,
                     (void*)labels,
                     (void*)lengths, (void*)ev_base, (void*)ev_step_ptr, (void*)ev_nid,
-                    (void*)ev_amp, (void*)idx, (void*)hist, (void*)C, (void*)ring_g, (void*)logits,
+                    (void*)ev_amp, (void*)idx, (void*)hist, (void*)C, (void*)ring_g, (void*)arq,
+                    (void*)logits,
                     (void*)loss, (void*)correct, (void*)blk_spikes, (void*)blk_events,
                     (void*)neur_fire, (void*)raster, (void*)p_v, (void*)p_tr, (void*)p_psi,
                     (void*)p_ext, (void*)p_ring, (void*)p_logit, (void*)p_L, (void*)p_refr,
@@ -444,8 +446,23 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
     return fail(SNC_E_UNSUPPORTED, "per-sample state does not fit in LDS");
   s->ring_lds = train_lds_bytes(a, true) <= lds_budget;
   if (!s->ring_lds) {
-    rc = talloc(&s->ring_g, (size_t)batch * s->L * n);
-    if (rc) return rc;
+    // Arrival queues (each sample's deposits, per ring slot, sorted when
+    // their step comes) up to kArqBytesMax; beyond, the ring in HBM.
+    const size_t arq_bytes = (size_t)batch * s->L * (size_t)std::max<int64_t>(1, s->S) * sizeof(uint2);
+    a.arq_cap = std::max<int64_t>(1, s->S);
+    a.arq = nullptr;
+    if (arq_bytes <= kArqBytesMax) {
+      a.arq = reinterpret_cast<uint2*>(1);  // sizes the LDS of the queue mode below
+      if (std::max(train_lds_bytes(a, false), train_elig_lds_bytes(n, 4)) <= lds_budget) {
+        rc = talloc(&s->arq, (size_t)batch * s->L * (size_t)a.arq_cap);
+        if (rc) return rc;
+      }
+      a.arq = s->arq;
+    }
+    if (!a.arq) {
+      rc = talloc(&s->ring_g, (size_t)batch * s->L * n);
+      if (rc) return rc;
+    }
   }
   a.ring_g = s->ring_g;
 

@@ -177,6 +177,8 @@ __device__ void sample_epilogue(const TrainArgs& a, int b, const float* logits_i
 // ---------------------------------------------------------------------------
 
 constexpr int kFoldBatch = 8;       // staged arrivals a fold round loads together
+constexpr int kDeposit = 4;         // deposits a lane loads before queueing them
+constexpr int kHeld = 6;            // arrivals a thread keeps in registers while sorting
 
 // Ordered fold of target j's arrivals into its ring slots. Columns are
 // ordered delay descending, so one step's arrivals for one ring slot are one
@@ -427,17 +429,28 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
   float* s_inpx = (float*)(s_inp + a.n_inputs);      // [n_inputs] drive value / probability
   uint8_t* s_role = (uint8_t*)(s_inpx + a.n_inputs); // [n] role
   float* s_ring = (float*)(smem + state_bytes + const_bytes);  // [L][n] when RING_LDS
+  // Arrival queues instead of a ring in HBM (block-uniform): the queues of
+  // this sample in HBM, their fill counts in LDS, and each target's sum of
+  // the coming step's arrivals in LDS where the ring would start.
+  const bool queue = !RING_LDS && a.arq != nullptr;
+  uint2* arq = queue ? a.arq + (size_t)b * L * a.arq_cap : nullptr;
+  float* s_in = s_ring;  // [n] (queue)
+  __shared__ int s_qcnt[256];
   // Fold staging, in the bytes the history chunks take after the step loop:
   // the bitmap's exclusive popcount prefix per word, then one {weight, delay}
-  // per arrival of the step, at its rank; `cap` arrivals fit.
+  // (one weight, with the queues) per arrival of the step, at its rank; `cap`
+  // arrivals fit.
   const size_t fold_base =
-      state_bytes + const_bytes + (RING_LDS ? (sizeof(float) * (size_t)L * n + 15) / 16 * 16 : 0);
+      state_bytes + const_bytes +
+      (RING_LDS ? (sizeof(float) * (size_t)L * n + 15) / 16 * 16 : queue ? train_in_bytes(n) : 0);
   // Word prefixes are 16-bit: a staged step has at most 65535 arrivals.
   const size_t pref_bytes = (sizeof(uint16_t) * ((size_t)a.bits_words + 1) + 15) / 16 * 16;
   uint16_t* s_pref = (uint16_t*)(smem + fold_base);
   uint2* s_cval = (uint2*)(smem + fold_base + pref_bytes);
+  float* s_val = (float*)s_cval;  // (queue)
+  const size_t stage_elem = queue ? sizeof(float) : sizeof(uint2);
   const int cap = (size_t)a.lds_bytes > fold_base + pref_bytes
-                      ? (int)min(((size_t)a.lds_bytes - fold_base - pref_bytes) / sizeof(uint2),
+                      ? (int)min(((size_t)a.lds_bytes - fold_base - pref_bytes) / stage_elem,
                                  (size_t)65535)
                       : 0;
   __shared__ int s_nf, s_arr;
@@ -486,7 +499,12 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
     }
   for (int k = tid; k < a.bits_words; k += kTrainThreads) s_bits[k] = 0u;
   for (int k = tid; k < a.classes; k += kTrainThreads) s_logit[k] = 0.0f;
-  for (size_t k = tid; k < (size_t)L * n; k += kTrainThreads) ring[k] = 0.0f;
+  if (queue) {
+    for (int j = tid; j < n; j += kTrainThreads) s_in[j] = 0.0f;
+    for (int k = tid; k < L; k += kTrainThreads) s_qcnt[k] = 0;
+  } else {
+    for (size_t k = tid; k < (size_t)L * n; k += kTrainThreads) ring[k] = 0.0f;
+  }
   if (tid == 0) {
     s_nf = 0;
     s_arr = 0;
@@ -550,11 +568,12 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
         int r = s_refr[j];
         float psi;
         const uint8_t role = s_role[j];
-        f = train_lif(a, role, v, r, s_ext[j], slot[j], psi);
+        f = train_lif(a, role, v, r, s_ext[j], queue ? s_in[j] : slot[j], psi);
         s_v[j] = v;
         s_refr[j] = r;
         s_ext[j] = 0.0f;
-        slot[j] = 0.0f;
+        if (queue) s_in[j] = 0.0f;
+        else slot[j] = 0.0f;
         const float tr = trace_step(a.lif.decay, s_tr[j], f);
         s_tr[j] = tr;
         if (a.C) {  // histories feed the update only
@@ -597,9 +616,34 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
     for (int k = wave; k < nf; k += kTrainThreads / kWave) {
       const int i = s_fired[k];
       const int rb = s_rowp[i], re = s_rowp[i + 1];
-      for (int e = rb + lane; e < re; e += kWave) {
-        const int p = a.cpos[e];
-        atomicOr(&s_bits[p >> 5], 1u << (p & 31));
+      if (queue) {
+        // each deposit joins the queue of the slot it arrives at; a lane
+        // loads kDeposit deposits before it reserves and stores any
+        for (int e0 = rb + lane; e0 < re; e0 += kDeposit * kWave) {
+          int sl[kDeposit];
+          uint2 rec[kDeposit];
+#pragma unroll
+          for (int u = 0; u < kDeposit; ++u) {
+            const int e = e0 + u * kWave;
+            sl[u] = -1;
+            if (e < re) {
+              sl[u] = tm + (int)a.sdel[e];
+              rec[u] = make_uint2((uint32_t)a.cpos[e], __float_as_uint(a.w_csr[e]));
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kDeposit; ++u)
+            if (sl[u] >= 0) {
+              if (sl[u] >= L) sl[u] -= L;
+              const int q = atomicAdd(&s_qcnt[sl[u]], 1);
+              arq[(size_t)sl[u] * a.arq_cap + q] = rec[u];
+            }
+        }
+      } else {
+        for (int e = rb + lane; e < re; e += kWave) {
+          const int p = a.cpos[e];
+          atomicOr(&s_bits[p >> 5], 1u << (p & 31));
+        }
       }
       ev += (unsigned long long)(re - rb);
     }
@@ -609,7 +653,98 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
     }
     __syncthreads();
     phase_mark<PROF>(s_ph, &s_last, 2, tid);
-    const bool staged = s_arr <= cap;  // block-uniform
+    if (queue) {
+      // The arrivals of step t + 1, sorted by column position (presence
+      // bits, their prefix: each record's rank), each target's run added
+      // from zero in column order -- the ring slot sum of the CPU path.
+      const int s1 = tm + 1 == L ? 0 : tm + 1;
+      const int count = t + 1 < T ? s_qcnt[s1] : 0;
+      const uint2* qa = arq + (size_t)s1 * a.arq_cap;
+      // the first kHeld records of each thread stay in registers for the
+      // staging below
+      uint2 held[kHeld];
+#pragma unroll
+      for (int u = 0; u < kHeld; ++u) {
+        const int k = tid + u * kTrainThreads;
+        held[u] = k < count ? qa[k] : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < kHeld; ++u)
+        if (tid + u * kTrainThreads < count) {
+          const int p = (int)held[u].x;
+          atomicOr(&s_bits[p >> 5], 1u << (p & 31));
+        }
+      for (int k = tid + kHeld * kTrainThreads; k < count; k += kTrainThreads) {
+        const int p = (int)qa[k].x;
+        atomicOr(&s_bits[p >> 5], 1u << (p & 31));
+      }
+      __syncthreads();
+      if (count > 0 && count <= cap) {
+        const int W = a.bits_words;
+        const int per = (W + kTrainThreads - 1) / kTrainThreads;
+        const int lo = min(W, tid * per), hi = min(W, lo + per);
+        int cnt = 0;
+        for (int w = lo; w < hi; ++w) cnt += __popc(s_bits[w]);
+        const int incl = wave_inclusive_scan(cnt);
+        if (lane == kWave - 1) s_wsum[wave] = incl;
+        __syncthreads();
+        int run = incl - cnt;
+        for (int k = 0; k < wave; ++k) run += s_wsum[k];
+        for (int w = lo; w < hi; ++w) {
+          s_pref[w] = (uint16_t)run;
+          run += __popc(s_bits[w]);
+        }
+        if (hi == W) s_pref[W] = (uint16_t)run;
+        __syncthreads();
+        phase_mark<PROF>(s_ph, &s_last, 3, tid);
+        auto stage = [&](uint2 e) {
+          const int p = (int)e.x, q = p >> 5;
+          const int rk = (int)s_pref[q] + __popc(s_bits[q] & ((1u << (p & 31)) - 1u));
+          s_val[rk] = __uint_as_float(e.y);
+        };
+#pragma unroll
+        for (int u = 0; u < kHeld; ++u)
+          if (tid + u * kTrainThreads < count) stage(held[u]);
+        for (int k = tid + kHeld * kTrainThreads; k < count; k += kTrainThreads) stage(qa[k]);
+        __syncthreads();
+        phase_mark<PROF>(s_ph, &s_last, 4, tid);
+        for (int j = tid; j < n; j += kTrainThreads) {
+          const int r0 = (int)s_pref[s_ccol[j] >> 5], r1 = (int)s_pref[s_ccol[j + 1] >> 5];
+          float acc = 0.0f;
+          int r = r0;
+          for (; r + 4 <= r1; r += 4)
+            acc = __fadd_rn(__fadd_rn(__fadd_rn(__fadd_rn(acc, s_val[r]), s_val[r + 1]),
+                                      s_val[r + 2]),
+                            s_val[r + 3]);
+          for (; r < r1; ++r) acc = __fadd_rn(acc, s_val[r]);
+          s_in[j] = acc;
+        }
+#pragma unroll
+        for (int u = 0; u < kHeld; ++u)
+          if (tid + u * kTrainThreads < count) s_bits[(int)held[u].x >> 5] = 0u;
+        for (int k = tid + kHeld * kTrainThreads; k < count; k += kTrainThreads)
+          s_bits[(int)qa[k].x >> 5] = 0u;
+      } else if (count > 0) {
+        // more arrivals than the staging holds: each target walks its
+        // column's set bits in order, weights from the column-ordered copy
+        for (int j = tid; j < n; j += kTrainThreads) {
+          float acc = 0.0f;
+          for (int w = s_ccol[j] >> 5; w < (s_ccol[j + 1] >> 5); ++w) {
+            uint32_t bw = s_bits[w];
+            if (!bw) continue;
+            s_bits[w] = 0u;
+            while (bw) {
+              const int p = (w << 5) | __builtin_ctz(bw);
+              bw &= bw - 1u;
+              acc = __fadd_rn(acc, a.wc[p]);
+            }
+          }
+          s_in[j] = acc;
+        }
+      }
+      if (tid == 0 && t + 1 < T) s_qcnt[s1] = 0;  // every thread read the count before a barrier
+    }
+    const bool staged = !queue && s_arr <= cap;  // block-uniform
     if (staged) {
       // (B2) exclusive popcount prefix of the bitmap words (block scan)
       const int W = a.bits_words;
@@ -648,7 +783,8 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
     }
     // (C) ordered fold into the ring (staged: the bitmap is only cleared);
     // next step's injections
-    if (staged) {
+    if (queue) {
+    } else if (staged) {
       for (int j = tid; j < n; j += kTrainThreads)
         fold_staged(a, j, (int)s_pref[s_ccol[j] >> 5], (int)s_pref[s_ccol[j + 1] >> 5], s_cval,
                     ring, n, tm);
@@ -1036,6 +1172,7 @@ size_t train_lds_bytes(const TrainArgs& a, bool ring_lds) {
   size_t b = train_state_bytes(a.n, a.words, a.bits_words, a.classes, train_rand_words(a)) +
              train_const_bytes(a.n, a.n_inputs);
   if (ring_lds) b += sizeof(float) * (size_t)a.L * a.n;
+  else if (a.arq) b += train_in_bytes(a.n);
   return b;
 }
 

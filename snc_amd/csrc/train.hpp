@@ -53,7 +53,12 @@ struct TrainArgs {
   int hn;              // history row length: n rounded up to a multiple of 4
   int lds_bytes;       // dynamic LDS of the batched launch
   float* C;            // [B][S] contributions in unit order (entry q: synapse ecol_syn[q]), or null
-  float* ring_g;       // [B][L][n] when the ring is not in LDS
+  float* ring_g;       // [B][L][n] when the ring is not in LDS and no arrival queues
+  // Arrival queues (the batched kernel when the ring does not fit in LDS):
+  // per sample and ring slot, arq_cap records {column position, weight bits}
+  // of the deposits that arrive at that slot's step; null: ring_g instead.
+  uint2* arq;
+  int64_t arq_cap;
   float* logits;       // [B][classes]
   double* loss;        // [B]
   int32_t* correct;    // [B]
@@ -75,6 +80,9 @@ struct TrainArgs {
   int* wq;                  // [2B]: published flag, then claimed rounds, per sample
   float* Ls_g;              // [B][n] learning signals of each sample
 };
+
+// Largest arrival-queue storage a training session allocates.
+constexpr size_t kArqBytesMax = (size_t)32 << 30;
 
 // Threads of a batched-trainer workgroup (one sample per workgroup).
 constexpr int kTrainThreads = 1024;
@@ -136,6 +144,12 @@ struct RewardArgs {
   float* adv;      // [B] reward advantage
   float decay;
 };
+
+// LDS bytes of the queue mode's per-target arrival sums (n floats, rounded
+// up to 16), where the ring would start.
+__host__ __device__ inline size_t train_in_bytes(int n) {
+  return (sizeof(float) * (size_t)n + 15) / 16 * 16;
+}
 
 // LDS of the update phase: the learning signal (n floats, rounded up to 16
 // bytes), then the transposed psi and trace chunks of `steps` steps.
