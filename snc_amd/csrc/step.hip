@@ -43,6 +43,7 @@ namespace snc_amd {
 constexpr int kLightMaxWords = 16;     // light targets: in-degree <= 512
 constexpr int kHubWordsPerThread = 8;  // column words a hub-block thread loads per round
 constexpr int kHubCap = 2048;          // LDS entries of one gather pass
+constexpr int kGather = 8;             // value loads a light target keeps in flight
 
 // LDS of one block: the regular step's arrays, or (hip-bucket) a hub block's
 // gather list -- a block is one or the other, so they share the bytes.
@@ -158,16 +159,34 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepArgs a0, int t, int tm) {
 #pragma unroll
     for (int k = 0; k < kLightMaxWords; ++k)
       if (wv[k]) bnow[w0 + k] = 0u;
-    float acc = 0.0f;  // the cleared slot the CPU adds onto
+    // The set bits' values, kGather loads in flight per round (a load per
+    // arrival in turn would cost the block its busiest target's arrival
+    // count in round trips), added in column order onto the cleared slot.
+    float acc = 0.0f;
+    int pb[kGather];
+    int nb = 0;
+    auto flush = [&]() {
+      float x[kGather];
+#pragma unroll
+      for (int u = 0; u < kGather; ++u) x[u] = u < nb ? a.cval[pb[u]] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (u < nb) acc = __fadd_rn(acc, x[u]);
+      nb = 0;
+    };
 #pragma unroll
     for (int k = 0; k < kLightMaxWords; ++k) {
       uint32_t bw = wv[k];
       while (bw) {
         const int p = ((w0 + k) << 5) | __builtin_ctz(bw);
         bw &= bw - 1u;
-        acc = __fadd_rn(acc, a.cval[p]);
+#pragma unroll
+        for (int u = 0; u < kGather; ++u)
+          if (nb == u) pb[u] = p;
+        if (++nb == kGather) flush();
       }
     }
+    flush();
     slot = acc;
   }
   __syncthreads();
