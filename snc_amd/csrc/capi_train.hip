@@ -444,14 +444,21 @@ int snc_hip_train_create(const snc_graph_view* g, const snc_train_config* cfg, i
   const size_t lds_budget = lds_max > 256 ? lds_max - 256 : 0;
   if (std::max(train_lds_bytes(a, false), train_elig_lds_bytes(n, 4)) > lds_budget)
     return fail(SNC_E_UNSUPPORTED, "per-sample state does not fit in LDS");
-  s->ring_lds = train_lds_bytes(a, true) <= lds_budget;
+  // SNC_HIP_TRAIN_DELAY selects where the batched kernel keeps the deposits
+  // in flight: "queues" (arrival queues) or "hbm" (the ring in HBM) even when
+  // the ring would fit in LDS; unset or anything else, the ring in LDS when
+  // it fits, else the queues, else the ring in HBM.
+  const char* dm = std::getenv("SNC_HIP_TRAIN_DELAY");
+  const bool want_queues = dm && std::strcmp(dm, "queues") == 0;
+  const bool want_hbm = dm && std::strcmp(dm, "hbm") == 0;
+  s->ring_lds = !want_queues && !want_hbm && train_lds_bytes(a, true) <= lds_budget;
   if (!s->ring_lds) {
     // Arrival queues (each sample's deposits, per ring slot, sorted when
     // their step comes) up to kArqBytesMax; beyond, the ring in HBM.
     const size_t arq_bytes = (size_t)batch * s->L * (size_t)std::max<int64_t>(1, s->S) * sizeof(uint2);
     a.arq_cap = std::max<int64_t>(1, s->S);
     a.arq = nullptr;
-    if (arq_bytes <= kArqBytesMax) {
+    if (!want_hbm && arq_bytes <= kArqBytesMax) {
       a.arq = reinterpret_cast<uint2*>(1);  // sizes the LDS of the queue mode below
       if (std::max(train_lds_bytes(a, false), train_elig_lds_bytes(n, 4)) <= lds_budget) {
         rc = talloc(&s->arq, (size_t)batch * s->L * (size_t)a.arq_cap);

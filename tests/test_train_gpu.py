@@ -247,8 +247,13 @@ def test_single_sample_steps_match_the_compiled_reference_trainer(reward, mode):
     assert np.abs(ref.weights - start).max() > 1e-3
 
 
-def test_delay_rich_recurrent_ring_outside_lds():
-    """Recurrent 10 % graph with delays 1..31: the ring lives in global memory."""
+@pytest.mark.parametrize("delay_store", [None, "queues", "hbm"])
+def test_delay_rich_recurrent_ring_outside_lds(monkeypatch, delay_store):
+    """Recurrent 10 % graph with delays 1..31, with the deposits in flight kept
+    where the session chooses (the ring in LDS here), in per-slot arrival
+    queues, or in a ring in HBM (SNC_HIP_TRAIN_DELAY forces the last two)."""
+    if delay_store:
+        monkeypatch.setenv("SNC_HIP_TRAIN_DELAY", delay_store)
     g = G.random_recurrent_graph(70, 400, 20, 0.1, 0.1, 1, 31, 0.25, seed=5)
     B = 6
     ev, lengths, y = I.event_streams(B, channels=70, classes=20, n_centres=20, sd=3.0,
@@ -259,6 +264,28 @@ def test_delay_rich_recurrent_ring_outside_lds():
     orc = O.Trainer(g, ocfg)
     w0 = orc.weights
     _compare(sess, orc, w0, np.arange(B), ev, lengths, y, BATCHED)
+
+
+def test_delay_stores_give_identical_weights(monkeypatch):
+    """The ring in LDS, the arrival queues and the ring in HBM add every
+    target's arrivals in the same order: after two minibatches with updates
+    the weights are bit-identical."""
+    g = G.random_recurrent_graph(70, 300, 20, 0.1, 0.1, 1, 31, 0.25, seed=6)
+    B = 8
+    ev, lengths, y = I.event_streams(B, channels=70, classes=20, n_centres=20, sd=3.0,
+                                     events_per_sample=300, t_min=30, t_max=50, seed=9)
+    gcfg, _ = _cfgs(50, reset_mode=1, lr=0.3, w_init=0.25)
+    out = {}
+    for mode in ("lds", "queues", "hbm"):
+        monkeypatch.setenv("SNC_HIP_TRAIN_DELAY", mode)
+        sess = HipTrainSession(g, gcfg, batch=B, max_steps=50)
+        sess.upload_events(ev, B, y, lengths)
+        for _ in range(2):
+            sess.step(np.arange(B, dtype=np.int32), epoch=1, mode=BATCHED, apply=True)
+        out[mode] = sess.get_weights()
+        sess.close()
+    assert np.array_equal(out["lds"], out["queues"])
+    assert np.array_equal(out["lds"], out["hbm"])
 
 
 def test_epochs_learn_prototype_classes():
