@@ -107,7 +107,9 @@ __device__ __forceinline__ bool train_lif(const TrainArgs& a, uint8_t role, floa
       fired = true;
     }
   }
-  const float d = __fdiv_rn(fabsf(__fsub_rn(vpre, thr)), thr > 0.0f ? thr : 1.0f);
+  // |vpre - thr| / thr; a threshold of 1 (x / 1 = x exactly) skips the division
+  const float dist = fabsf(__fsub_rn(vpre, thr));
+  const float d = thr == 1.0f ? dist : __fdiv_rn(dist, thr > 0.0f ? thr : 1.0f);
   psi = (a.lif.subtract && in_refr) ? 0.0f
                                     : (d < 1.0f ? __fmul_rn(a.gamma, __fsub_rn(1.0f, d)) : 0.0f);
   return fired;
