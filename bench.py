@@ -269,10 +269,45 @@ def eprop_bench(w, epochs, plain_batches, elig, cpu_samples=0):
             f"(x{bsps / psps:.1f}), identical={identical}, acc {acc0:.3f} -> {curve}")
     else:
         log(f"e-prop {w.name}: batched {bsps:.1f} samples/s, acc {acc0:.3f} -> {curve}")
+    out["global_T_max"] = eprop_global_t(w, epochs, elig)
+    log(f"e-prop {w.name} with every sample at T_max = {w.max_steps}: "
+        f"{out['global_T_max']['samples_per_s']:.1f} samples/s, "
+        f"acc -> {out['global_T_max']['test_acc_per_epoch']}")
     if cpu_samples > 0:
         out["cpu_baseline"] = eprop_cpu_baseline(w, cpu_samples)
         out["speedup_batched_vs_cpu"] = bsps / out["cpu_baseline"]["value"]
     return out
+
+
+def eprop_global_t(w, epochs, elig):
+    """Sensitivity to the per-sample lengths: the same batched training with
+    every sample run for the workload's longest length T_max (lr / T_max for
+    all, the reference's global-T rule) instead of its own T_b, from the same
+    start weights and minibatch orders: samples/s and held-out accuracy per
+    epoch."""
+    import dataclasses
+
+    import numpy as np
+    from snc_amd.train import BATCHED
+
+    wt = dataclasses.replace(w, lengths=np.full_like(w.lengths, w.max_steps))
+    s = _train_session(wt, elig)
+    n = w.labels.size
+    tr = np.arange(w.n_train, dtype=np.int32)
+    te = np.arange(w.n_train, n, dtype=np.int32)
+    wall, samples, curve = 0.0, 0, []
+    for ep in range(1, epochs + 1):
+        seed = 12345 if ep == 1 else 12345 + ep
+        o = np.random.default_rng(seed).permutation(tr).astype(np.int32)
+        t0 = time.perf_counter()
+        for i in range(0, o.size, w.batch):
+            st, _, _ = s.step(o[i:i + w.batch], epoch=ep, mode=BATCHED, apply=True)
+            samples += st.samples
+        wall += time.perf_counter() - t0
+        curve.append(s.evaluate(te))
+    s.close()
+    return {"T": int(w.max_steps), "samples_per_s": samples / wall,
+            "test_acc_per_epoch": curve}
 
 
 def eprop_cpu_baseline(w, n_samples):
