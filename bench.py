@@ -774,7 +774,7 @@ def main():
         out["grow_ab_sheet"] = ab_compare(log=log, kind="sheet")
         # the structural co-training workload as stated: the batched e-prop
         # network under 3000 synapses, the whole candidate pool scored
-        out["grow_ab_c4"] = ab_compare(log=log, kind="c4", label="survival", w_grow=0.2,
+        out["grow_ab_c4"] = ab_compare(log=log, kind="c4", label="rank", w_grow=-1.0,
                                        onpolicy=1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_repeats)

@@ -40,7 +40,8 @@ extern "C" {
 typedef struct snc_struct_config {
   int32_t grow_per_epoch;
   int32_t delay;
-  float w_grow_init;
+  float w_grow_init; /* negative: the surviving synapses' median |w| at each step
+                       (no reference counterpart; the reference's value is fixed) */
   float locality_window;
   int32_t protect_rounds;
   int32_t max_attempts_mult;

@@ -325,6 +325,20 @@ class StructuralStep {
     }
     // the features see the connectome as the prune left it
     fx_.reset(new FeatureCtx(c_, fires_, n_fires_, norm_, cfg_.delay));
+    // a negative w_grow_init starts grown synapses at the surviving synapses'
+    // median |w| (the lower middle one of an even count)
+    w_start_ = cfg_.w_grow_init;
+    if (w_start_ < 0.0f) {
+      std::vector<float> m;
+      for (const auto& row : c_->adj)
+        for (const Edge& e : row) m.push_back(std::fabs(e.weight));
+      w_start_ = 0.0f;
+      if (!m.empty()) {
+        const size_t mid = (m.size() - 1) / 2;
+        std::nth_element(m.begin(), m.begin() + (ptrdiff_t)mid, m.end());
+        w_start_ = m[mid];
+      }
+    }
     return SNC_OK;
   }
 
@@ -469,7 +483,7 @@ class StructuralStep {
   long long fires(int id) const { return id < n_fires_ ? fires_[id] : 0LL; }
 
   void add_edge(int pre, int post, int64_t log_id) {
-    c_->adj[pre].push_back({post, cfg_.delay, cfg_.w_grow_init, 0, log_id});
+    c_->adj[pre].push_back({post, cfg_.delay, w_start_, 0, log_id});
     ++grown_;
   }
 
@@ -524,6 +538,7 @@ class StructuralStep {
   double norm_;
   bool log_;
   std::unique_ptr<FeatureCtx> fx_;  // built by prune()
+  float w_start_ = 0.0f;            // weight of a grown synapse (set by prune())
   std::vector<int> freed_;
   int grown_ = 0;
   std::vector<int> scan_, wq_;

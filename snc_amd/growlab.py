@@ -210,7 +210,8 @@ if __name__ == "__main__":
     ap.add_argument("--onpolicy", type=int, default=0,
                     help="rounds of logging the model's own choices and retraining")
     ap.add_argument("--w-grow", type=float, default=0.02,
-                    help="starting weight of a grown synapse (StructConfig.w_grow_init)")
+                    help="starting weight of a grown synapse (StructConfig.w_grow_init; "
+                         "negative: the surviving synapses' median |w|)")
     a = ap.parse_args()
     out = ab_compare(tuple(int(x) for x in a.seeds.split(",")),
                      tuple(int(x) for x in a.train_seeds.split(",")), a.outer, a.mlp_epochs,

@@ -33,7 +33,7 @@ class StructConfig:
     """StructConfig (connectome.hpp:33-48), same defaults."""
     grow_per_epoch: int = 200
     delay: int = 1
-    w_grow_init: float = 0.02
+    w_grow_init: float = 0.02  # negative: the surviving synapses' median |w| at each step
     locality_window: float = 0.2
     protect_rounds: int = 2
     max_attempts_mult: int = 8

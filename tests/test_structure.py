@@ -210,3 +210,27 @@ def test_prune_to_keeps_the_largest_abs_weights():
     assert np.array_equal(con.edge_weights(), w[keep])
     g1 = con.to_graph()
     assert np.array_equal(g1.post_ids, np.asarray(g0.post_ids)[keep])
+
+
+def test_negative_grow_init_starts_grown_synapses_at_the_survivors_median():
+    con = Connectome([40, 30, 6], 400, 1, 0.02, 3)
+    rng = np.random.default_rng(3)
+    g0 = con.to_graph()
+    w = (rng.standard_normal(g0.num_synapses) * 0.3).astype(np.float32)
+    con.set_edge_weights(w)
+    before = {(int(i), int(j)) for i, j in zip(np.repeat(np.arange(g0.num_neurons), np.diff(g0.row_ptr)), g0.post_ids)}
+    cfg = StructConfig(grow_per_epoch=25, w_grow_init=-1.0, locality_window=0.3, protect_rounds=0, seed=3)
+    fires = rng.integers(0, 50, g0.num_neurons).astype(np.int64)
+    rep = con.structural_update(np.zeros(g0.num_synapses, np.int64), fires, cfg, mode=GROW_HEURISTIC)
+    assert rep["grown"] > 0
+    assert rep["pruned"] == 25
+    # the survivors are all but the 25 smallest |w| (every synapse eligible)
+    survivors = np.sort(np.abs(w))[25:]
+    med = survivors[(survivors.size - 1) // 2]
+    g1 = con.to_graph()
+    w1 = con.edge_weights()
+    pre1 = np.repeat(np.arange(g1.num_neurons), np.diff(g1.row_ptr))
+    new_pair = np.array([(int(i), int(j)) not in before for i, j in zip(pre1, g1.post_ids)])
+    assert new_pair.any() and np.all(w1[new_pair] == med)
+    # every grown synapse (a regrown pruned pair included) carries the median
+    assert np.count_nonzero(w1 == med) >= rep["grown"]
