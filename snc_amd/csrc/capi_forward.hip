@@ -124,8 +124,9 @@ struct snc_hip_plan {
   int32_t *inj_blk_ptr = nullptr, *blk_nid = nullptr;
   float *blk_amp = nullptr, *inj_heavy = nullptr;
   int2* inj_tab = nullptr;
-  uint32_t* F = nullptr;      // hip-pull: fired words [L][n]
+  uint32_t* F = nullptr;      // hip-pull: fired words [L][n], 4 or 8 bytes each
   float* hub_part = nullptr;  // hip-pull: hub chunk partials
+  size_t F_words = 0;         // hip-pull: 32-bit words of F
   PullArgs pa{};
   size_t n_blk_slots = 0;
   int halves = 1;
@@ -413,7 +414,9 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     if (cfg->heavy_threshold < 1 || cfg->heavy_threshold > kPullMaxCap)
       return fail(SNC_E_INVALID, "hip-pull heavy_threshold (light in-degree cap) must be in [1, 1024]");
     if (cfg->persistent) return fail(SNC_E_UNSUPPORTED, "hip-pull has no persistent kernel");
-    if (B > kPullMaxSamples) return fail(SNC_E_UNSUPPORTED, "hip-pull plans hold at most 32 samples");
+    if (lif->refractory > 255)
+      return fail(SNC_E_UNSUPPORTED, "hip-pull keeps refractory counters in one byte (at most 255 steps)");
+    if (B > kPullMaxSamples) return fail(SNC_E_UNSUPPORTED, "hip-pull plans hold at most 64 samples");
   } else {
     if (!fused_block_ok(cfg->block_neurons))
       return fail(SNC_E_INVALID, "block_neurons must be 128, 256 or 512");
@@ -657,10 +660,8 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
       }
     pool = (pool + 63) / 64 * 64;
     if (pool > (1 << 20)) return fail(SNC_E_UNSUPPORTED, "hip-pull: injection pool exceeds LDS");
-    int lrc = pull_set_lds(B, (int)pool);
-    if (lrc) return lrc;
-    // Sliced columns per light cap, built once per graph and kept while it
-    // lives (plans of different caps may be alive together).
+    // Halo-indexed columns per light cap, built once per graph and kept
+    // while it lives (plans of different caps may be alive together).
     PullGraph* pgp = nullptr;
     for (PullGraph* q : G->pulls)
       if (q->cap == cfg->heavy_threshold) pgp = q;
@@ -674,29 +675,42 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     }
     PullGraph& pg = *pgp;
     if (pg.hub_host != P->heavy_host) return fail(SNC_E_UNSUPPORTED, "hip-pull hub sets differ");
-    int prc = dalloc(&P->F, (size_t)g.L * n);
+    if (pg.n_blocks != R) return fail(SNC_E_INVALID, "hip-pull block count differs from the plan's");
+    int lrc = pull_set_lds(B, pg.halo_words, (int)pool);
+    if (lrc) return lrc;
+    const size_t word_u32 = (size_t)pull_word_bytes(B) / 4;
+    // the step kernels index every array with 32-bit byte offsets
+    if ((uint64_t)g.P * 4 >= (1ull << 32) || (uint64_t)n * B * 4 >= (1ull << 32) ||
+        (uint64_t)g.L * n * word_u32 * 4 >= (1ull << 32))
+      return fail(SNC_E_UNSUPPORTED, "hip-pull: graph or batch too large for 32-bit offsets");
+    int prc = dalloc(&P->F, (size_t)g.L * n * word_u32);
     if (!prc) prc = dalloc(&P->hub_part, (size_t)std::max(1, pg.n_chunks) * kPullMaxSamples);
     if (prc) return prc;
+    P->F_words = (size_t)g.L * n * word_u32;
     PullArgs& pa = P->pa;
     pa.n = n;
     pa.L = g.L;
     pa.B = B;
+    pa.T = num_steps;
     pa.words = a.words;
-    pa.n_light_blocks = R;
-    pa.lif = a.lif;
-    pa.sample_mask = B >= 32 ? 0xFFFFFFFFu : ((1u << B) - 1u);
+    pa.n_blocks = R;
     pa.pool = (int)pool;
-    pa.row_ptr = g.row_ptr;
-    pa.slab = pg.slab;
-    pa.slab_w = pg.slab_w;
-    pa.slice_ptr = pg.slice_ptr;
-    pa.slice_len = pg.slice_len;
-    pa.is_hub = pg.is_hub;
+    pa.halo = pg.halo_words;
+    pa.direct = pg.direct ? 1 : 0;
+    pa.lif = a.lif;
+    pa.lw = pg.lw;
+    pa.lh = pg.lh;
+    pa.ldir = pg.ldir;
+    pa.loff = pg.loff;
+    pa.lcnt = pg.lcnt;
+    pa.halo_ptr = pg.halo_ptr;
+    pa.halo_pk = pg.halo;
     pa.hub_ids = pg.hub_ids;
-    pa.hub_e = pg.hub_e;
+    pa.hub_pk = pg.hub_pk;
     pa.hub_w = pg.hub_w;
     pa.chunk_hub = pg.chunk_hub;
-    pa.chunk_beg = pg.chunk_beg;
+    pa.chunk_pos = pg.chunk_pos;
+    pa.chunk_len = pg.chunk_len;
     pa.hub_chunk_ptr = pg.hub_chunk_ptr;
     pa.n_hub = pg.n_hub;
     pa.n_chunks = pg.n_chunks;
@@ -708,12 +722,9 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     pa.hinj_stride = a.hinj_stride;
     pa.F = P->F;
     pa.hub_part = P->hub_part;
-    pa.v = P->v;
-    pa.refr = P->refr;
+    pa.v = P->v;  // hip-pull state is sample-minor: [n][B]
+    pa.refr = reinterpret_cast<uint8_t*>(P->refr);  // one byte per neuron and sample
     pa.nspk = P->nspk;
-    pa.blk_spikes = P->blk_spikes;
-    pa.blk_events = P->blk_events;
-    pa.blk_stride = (int64_t)P->n_blk_slots;
     pa.raster = nullptr;
     pa.raster_stride = a.raster_stride;
   }
@@ -810,7 +821,7 @@ int reset_state(snc_hip_plan* P, bool raster) {
   fill(P->nspk, sizeof(int32_t) * B * n);
   fill(P->dirty, P->dirty_total);
   fill(P->bits, sizeof(uint32_t) * P->bits_total);
-  if (P->F) fill(P->F, sizeof(uint32_t) * (size_t)P->G->g.L * P->G->g.n);
+  if (P->F) fill(P->F, sizeof(uint32_t) * P->F_words);
   fill(P->blk_spikes, sizeof(unsigned long long) * B * P->n_blk_slots);
   fill(P->blk_events, sizeof(unsigned long long) * B * P->n_blk_slots);
   if (raster) fill(P->raster, sizeof(uint32_t) * B * (size_t)P->T * P->a.words);
@@ -911,18 +922,35 @@ int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stat
   const size_t n = (size_t)g.n, B = (size_t)P->B, slots = P->n_blk_slots;
   std::vector<unsigned long long> bs(B * slots), be(B * slots);
   std::vector<int32_t> nspk(B * n);
-  if (hipMemcpy(bs.data(), P->blk_spikes, bs.size() * sizeof(bs[0]), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(be.data(), P->blk_events, be.size() * sizeof(be[0]), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(nspk.data(), P->nspk, sizeof(int32_t) * B * n, hipMemcpyDeviceToHost) != hipSuccess)
+  const bool pull = P->variant == SNC_VARIANT_PULL;
+  if (hipMemcpy(nspk.data(), P->nspk, sizeof(int32_t) * B * n, hipMemcpyDeviceToHost) != hipSuccess)
     return fail(SNC_E_HIP, "result readback failed");
   std::memset(stats, 0, sizeof(*stats));
   P->last_spikes.assign(B, 0);
   P->last_events.assign(B, 0);
-  for (size_t b = 0; b < B; ++b)
-    for (size_t k = 0; k < slots; ++k) {
-      P->last_spikes[b] += (int64_t)bs[b * slots + k];
-      P->last_events[b] += (int64_t)be[b * slots + k];
-    }
+  if (pull) {
+    // hip-pull keeps no per-step counters and its state is sample-minor:
+    // transpose, then count spikes and events per sample from the per-neuron
+    // counts (each spike of neuron i carries its out-degree's events).
+    std::vector<int32_t> t(B * n);
+    for (size_t i = 0; i < n; ++i)
+      for (size_t b = 0; b < B; ++b) t[b * n + i] = nspk[i * B + b];
+    nspk.swap(t);
+    for (size_t b = 0; b < B; ++b)
+      for (size_t i = 0; i < n; ++i) {
+        P->last_spikes[b] += nspk[b * n + i];
+        P->last_events[b] += (int64_t)nspk[b * n + i] * g.out_degree[i];
+      }
+  } else {
+    if (hipMemcpy(bs.data(), P->blk_spikes, bs.size() * sizeof(bs[0]), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(be.data(), P->blk_events, be.size() * sizeof(be[0]), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(SNC_E_HIP, "result readback failed");
+    for (size_t b = 0; b < B; ++b)
+      for (size_t k = 0; k < slots; ++k) {
+        P->last_spikes[b] += (int64_t)bs[b * slots + k];
+        P->last_events[b] += (int64_t)be[b * slots + k];
+      }
+  }
   for (size_t b = 0; b < B; ++b) {
     stats->spikes += P->last_spikes[b];
     stats->synaptic_events += P->last_events[b];
@@ -941,9 +969,17 @@ int snc_hip_plan_run(snc_hip_plan* P, snc_forward_capture* cap, snc_forward_stat
           if (g.role[i] == SNC_ROLE_OUTPUT && g.channel[i] >= 0)
             cap->logits[b * C + g.channel[i]] += (float)nspk[b * n + i];
     }
-    if (cap->final_v &&
-        hipMemcpy(cap->final_v, P->v, sizeof(float) * B * n, hipMemcpyDeviceToHost) != hipSuccess)
-      return fail(SNC_E_HIP, "voltage readback failed");
+    if (cap->final_v) {
+      if (pull) {
+        std::vector<float> vv(B * n);
+        if (hipMemcpy(vv.data(), P->v, sizeof(float) * B * n, hipMemcpyDeviceToHost) != hipSuccess)
+          return fail(SNC_E_HIP, "voltage readback failed");
+        for (size_t i = 0; i < n; ++i)
+          for (size_t b = 0; b < B; ++b) cap->final_v[b * n + i] = vv[i * B + b];
+      } else if (hipMemcpy(cap->final_v, P->v, sizeof(float) * B * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        return fail(SNC_E_HIP, "voltage readback failed");
+      }
+    }
     if (want_raster) {
       const size_t W = (size_t)P->a.words, T = (size_t)P->T;
       std::vector<uint32_t> r(B * T * W);

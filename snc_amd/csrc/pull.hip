@@ -1,4 +1,4 @@
-// hip-pull kernels and the sliced-column build (see pull.hpp).
+// hip-pull kernels and the halo-indexed column build (see pull.hpp).
 
 #include <hip/hip_runtime.h>
 
@@ -14,10 +14,13 @@ namespace snc_amd {
 
 namespace {
 
-constexpr int kPullWalk = 16;      // column entries a lane has in flight per round
-constexpr int kPullLifGroup = 8;   // samples whose state loads together
-constexpr int kPullHubPerThread = kPullHubChunk / kPullBlock;
-static_assert(kPullHubChunk % kPullBlock == 0, "hub chunks split evenly over a block");
+constexpr int kPullWaves = kPullThreads / kWave;
+constexpr int kHubRounds = kPullHubChunk / kWave;
+constexpr int kHubThreads = 256;  // chunk waves per hub-chunk block: 4
+constexpr int kBatch = 4;         // light targets whose loads go out together
+constexpr int kHaloPer = (kPullHaloCap + kPullThreads - 1) / kPullThreads;
+static_assert(kPullHubChunk % kWave == 0, "hub chunks are whole rounds of a wave");
+static_assert(kPullWaveTargets % kBatch == 0, "whole batches per wave");
 
 template <class T>
 void dfree(T*& p) {
@@ -26,398 +29,516 @@ void dfree(T*& p) {
 }
 
 template <class T>
-int dalloc(T** p, size_t count) {
+int dupload(T** p, const std::vector<T>& h) {
   *p = nullptr;
-  if (!count) return SNC_OK;
-  const hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (h.empty()) return SNC_OK;
+  const size_t bytes = h.size() * sizeof(T);
+  hipError_t e = hipMalloc((void**)p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
     hip_clear();
     return fail(SNC_E_NOMEM, "device allocation failed (pull columns)");
   }
-  account_alloc(count * sizeof(T));
-  return SNC_OK;
+  account_alloc(bytes);
+  e = hipMemcpy(*p, h.data(), bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? SNC_OK : hip_fail(e, "pull column upload");
 }
 
 template <class T>
-int dupload(T** p, const std::vector<T>& h) {
-  int rc = dalloc(p, h.size());
-  if (!rc && !h.empty()) {
-    const hipError_t e = hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
-    if (e != hipSuccess) rc = hip_fail(e, "pull column upload");
-  }
-  return rc;
+int dload(std::vector<T>& h, const T* d, size_t count) {
+  h.resize(count);
+  if (!count) return SNC_OK;
+  const hipError_t e = hipMemcpy(h.data(), d, count * sizeof(T), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? SNC_OK : hip_fail(e, "pull column readback");
 }
 
-// One thread per pre row: every synapse goes to its target's slice (light)
-// or its hub column, at its rank k in the target's column order (cpos is the
-// padded column position, ccol the column start).
-__global__ void k_pull_build(int n, const int32_t* __restrict__ row_ptr,
-                             const int32_t* __restrict__ post, const float* __restrict__ wsig,
-                             const uint8_t* __restrict__ delay, const int32_t* __restrict__ cpos,
-                             const int32_t* __restrict__ ccol, const int32_t* __restrict__ hub_of,
-                             const int64_t* __restrict__ hub_off,
-                             const int64_t* __restrict__ slice_ptr, uint32_t* __restrict__ slab,
-                             float* __restrict__ slab_w, uint32_t* __restrict__ hub_e,
-                             float* __restrict__ hub_w) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
-    const int j = post[e];
-    const int64_t k = cpos[e] - ccol[j];
-    const uint32_t packed = ((uint32_t)i << 8) | (uint32_t)delay[e];
-    const int h = hub_of[j];
-    if (h >= 0) {
-      hub_e[hub_off[h] + k] = packed;
-      hub_w[hub_off[h] + k] = wsig[e];
-    } else {
-      const int64_t q = slice_ptr[j >> 6] + k * 64 + (j & 63);
-      slab[q] = packed;
-      slab_w[q] = wsig[e];
-    }
-  }
+// Loads and stores at a 32-bit element offset from a base pointer: the
+// offset stays one VGPR and the base an SGPR pair (the saddr form), which
+// saves the 64-bit address arithmetic per lane. Every hip-pull array it
+// indexes holds less than 4 GiB (checked when the plan is built).
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     (size_t)(i * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(i * (uint32_t)sizeof(T))) = v;
 }
 
-// F word of the synapse's pre at its origin step t - d, for arrival step t
-// (tm = t % L; 1 <= d < L, so the slot is (tm - d) mod L).
-__device__ __forceinline__ uint32_t fired_word(const PullArgs& a, uint32_t packed, int tm) {
-  int s = tm - (int)(packed & 0xFFu);
+// The fired word of a direct-form entry ((pre << 8) | delay) for arrival
+// step t (tm = t % L; 1 <= d < L, so the origin slot is (tm - d) mod L).
+template <class Word>
+__device__ __forceinline__ Word direct_word(const PullArgs& a, uint32_t pk, int tm) {
+  int s = tm - (int)(pk & 0xFFu);
   if (s < 0) s += a.L;
-  return a.F[(size_t)s * a.n + (packed >> 8)] & a.sample_mask;
+  return ld32(static_cast<const Word*>(a.F), (uint32_t)s * (uint32_t)a.n + ((pk >> 8) & 0x7FFFFFu));
 }
 
-// LDS of one block: per-thread accumulator rows of `pitch` floats (B or B+1,
-// odd so neighbouring rows start in different banks), then the light block's
-// injection pool, per-target list heads and per-sample counters.
-__host__ __device__ inline int acc_pitch(int B) { return B | 1; }
-__host__ __device__ inline size_t pull_lds_bytes(int B, int pool) {
-  return (size_t)kPullBlock * acc_pitch(B) * 4 + (size_t)pool * 12 + (size_t)kPullBlock * 4 +
-         (size_t)kPullMaxSamples * 16 + 16;
+// A light entry's fired word: its LDS halo slot, or (DIRECT kernels only)
+// F itself. Lanes past the column's end read the halo's zero slot, which
+// follows the block's words. o is the entry's place in the light stream.
+template <class Word, bool DIRECT>
+__device__ __forceinline__ Word entry_word(const PullArgs& a, const Word* s_halo, bool in,
+                                           uint32_t h, uint32_t o, int tm) {
+  if constexpr (DIRECT) {
+    Word w = s_halo[in && h != kHaloDirect ? h : (uint32_t)a.halo];
+    if (in && h == kHaloDirect) w = direct_word<Word>(a, ld32(a.ldir, o), tm);
+    return w;
+  } else {
+    return s_halo[in ? h : (uint32_t)a.halo];
+  }
 }
 
-// Light block: 4 waves, wave w steps slice 4 * block + w, lane = target.
-__device__ void pull_light(const PullArgs& a, int t, int blk, unsigned char* lds) {
-  const int pitch = acc_pitch(a.B);
-  float* s_acc = reinterpret_cast<float*>(lds);  // [kPullBlock][pitch]
-  int2* s_pool = reinterpret_cast<int2*>(lds + (size_t)kPullBlock * pitch * 4);  // (local | b << 16, amp)
-  int* s_next = reinterpret_cast<int*>(s_pool + a.pool);
-  int* s_head = s_next + a.pool;  // [kPullBlock]
-  unsigned long long* s_spk = reinterpret_cast<unsigned long long*>(s_head + kPullBlock);
-  unsigned long long* s_ev = s_spk + kPullMaxSamples;
-  int* s_pool_n = reinterpret_cast<int*>(s_ev + kPullMaxSamples);
-
-  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-  const int j = blk * kPullBlock + tid;
-  const int base = blk * kPullBlock;
-  const bool mine = j < a.n && a.is_hub[j] == 0;
-  float* acc = s_acc + tid * pitch;
-  for (int b = 0; b < a.B; ++b) acc[b] = 0.0f;
-  s_head[tid] = -1;
-  if (tid < kPullMaxSamples) {
-    s_spk[tid] = 0;
-    s_ev[tid] = 0;
+// Adds one round of 64 column entries (lane k holds entry k's fired word and
+// value) to every sample's sum, in column order. The fired bits are
+// transposed through the wave's 64 LDS masks (s_mask, all zero on entry and
+// on return): entry lane k ors bit k into the mask of every sample its word
+// has, then sample lane b walks its mask in entry order and adds those
+// entries' values (+0 on the rounds where it has none left). The work
+// follows the set bits, not the entries with any bit set.
+template <class Word>
+__device__ __forceinline__ float add_round(Word word, float w, float acc,
+                                           unsigned long long* s_mask) {
+  uint64_t bits = (uint64_t)word;
+  if (ballot(bits != 0) == 0) return acc;
+  const int lane = lane_id();
+  const unsigned long long me = 1ull << lane;
+  while (bits) {
+    atomicOr(&s_mask[__builtin_ctzll(bits)], me);
+    bits &= bits - 1;
   }
-  if (tid == 0) *s_pool_n = 0;
-  __syncthreads();
-
-  // Injections of step t of every sample into the LDS pool, linked per target.
-  for (int b = wave; b < a.B; b += kPullBlock / kWave) {
-    const int32_t* bp = a.inj_blk_ptr + (size_t)b * a.inj_stride + (size_t)t * (a.n_light_blocks + 1) + blk;
-    const int k0 = bp[0], cnt = bp[1] - k0;
-    if (cnt <= 0) continue;
-    int at = 0;
-    if (lane == 0) at = atomicAdd(s_pool_n, cnt);
-    at = __shfl(at, 0, kWave);
-    for (int q = lane; q < cnt; q += kWave) {
-      const int slot = at + q;
-      if (slot >= a.pool) break;  // the plan build sizes the pool for every (step, block)
-      const int loc = a.blk_nid[k0 + q] - base;
-      s_pool[slot] = make_int2(loc | (b << 16), __float_as_int(a.blk_amp[k0 + q]));
-      s_next[slot] = atomicExch(&s_head[loc], slot);
-    }
+  wave_lds_sync();
+  uint64_t m = s_mask[lane];
+  s_mask[lane] = 0ull;
+  while (ballot(m != 0)) {  // every lane active: lanes with nothing left add +0
+    const int k = m ? __builtin_ctzll(m) : 0;
+    const float vk = __shfl(w, k, kWave);
+    acc = __fadd_rn(acc, m ? vk : 0.0f);
+    m &= m - 1;
   }
+  return acc;
+}
 
-  // The column walk, kPullWalk entries per round: the next round's entries
-  // and values load while this round's fired words are gathered, so a round
-  // costs about one memory round trip. Additions stay in column order.
-  const int s = blk * (kPullBlock / kWave) + wave;
+// LDS of a step block: the halo's fired words, every wave's 64 sample masks,
+// then the injection pool, its per-entry links and per-target list heads.
+constexpr size_t kMaskBytes = (size_t)kPullWaves * kWave * 8;
+__host__ __device__ inline size_t lds_bytes(int word_bytes, int halo, int pool) {
+  return ((size_t)(halo + 1) * word_bytes + 15) / 16 * 16 + kMaskBytes + (size_t)pool * 12 +
+         (size_t)kPullBlock * 4 + 16;
+}
+
+// Hub chunks of step t: wave c sums chunk c per sample, in column order.
+template <class Word>
+__global__ __launch_bounds__(kHubThreads) void k_pull_hub_chunks(PullArgs a, int t) {
+  __shared__ unsigned long long s_masks[kHubThreads];
+  const int lane = lane_id();
+  const int c = blockIdx.x * (kHubThreads / kWave) + (threadIdx.x >> 6);
+  unsigned long long* s_mask = s_masks + (threadIdx.x & ~(kWave - 1));
+  s_mask[lane] = 0ull;
+  wave_lds_sync();
+  if (c >= a.n_chunks) return;
   const int tm = t % a.L;
-  if (mine) {
-    const int len = a.slice_len[s];
-    const uint32_t* col = a.slab + a.slice_ptr[s] + lane;
-    const float* colw = a.slab_w + a.slice_ptr[s] + lane;
-    uint32_t pk[kPullWalk];
-    float w[kPullWalk];
+  const int p0 = a.chunk_pos[c], len = a.chunk_len[c];
+  uint32_t pk[kHubRounds];
+  float w[kHubRounds];
+  Word wd[kHubRounds];
 #pragma unroll
-    for (int u = 0; u < kPullWalk; ++u) {
-      const bool in = u < len;
-      pk[u] = in ? col[(size_t)u * 64] : 0u;
-      w[u] = in ? colw[(size_t)u * 64] : 0.0f;
-    }
-    for (int k0 = 0; k0 < len; k0 += kPullWalk) {
-      uint32_t wd[kPullWalk], pn[kPullWalk];
-      float wn[kPullWalk];
-#pragma unroll
-      for (int u = 0; u < kPullWalk; ++u) wd[u] = pk[u] & 0xFFu ? fired_word(a, pk[u], tm) : 0u;
-#pragma unroll
-      for (int u = 0; u < kPullWalk; ++u) {
-        const int k = k0 + kPullWalk + u;
-        const bool in = k < len;
-        pn[u] = in ? col[(size_t)k * 64] : 0u;
-        wn[u] = in ? colw[(size_t)k * 64] : 0.0f;
-      }
-#pragma unroll
-      for (int u = 0; u < kPullWalk; ++u) {
-        uint32_t m = wd[u];
-        while (m) {
-          const int b = __builtin_ctz(m);
-          m &= m - 1u;
-          acc[b] = __fadd_rn(acc[b], w[u]);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kPullWalk; ++u) {
-        pk[u] = pn[u];
-        w[u] = wn[u];
-      }
-    }
+  for (int u = 0; u < kHubRounds; ++u) {
+    const int k = u * kWave + lane;
+    const uint32_t o = (uint32_t)(k < len ? p0 + k : p0);  // in the chunk either way
+    pk[u] = ld32(a.hub_pk, o);
+    w[u] = ld32(a.hub_w, o);
+    if (k >= len) pk[u] = kHaloNone;
   }
-  __syncthreads();
-
-  // Integrate/fire every sample of this target, kPullLifGroup samples per
-  // round of loads.
-  uint32_t word = 0;
-  if (mine) {
-    uint32_t inj = 0;
-    for (int q = s_head[tid]; q >= 0; q = s_next[q]) inj |= 1u << (s_pool[q].x >> 16);
-    const int deg = a.row_ptr[j + 1] - a.row_ptr[j];
-    for (int b0 = 0; b0 < a.B; b0 += kPullLifGroup) {
-      float vv[kPullLifGroup];
-      int rr[kPullLifGroup];
 #pragma unroll
-      for (int u = 0; u < kPullLifGroup; ++u) {
-        const int b = b0 + u;
-        vv[u] = b < a.B ? a.v[(size_t)b * a.n + j] : 0.0f;
-        rr[u] = b < a.B ? a.refr[(size_t)b * a.n + j] : 0;
-      }
+  for (int u = 0; u < kHubRounds; ++u)
+    wd[u] = pk[u] != kHaloNone ? direct_word<Word>(a, pk[u], tm) : (Word)0;
+  float acc = 0.0f;
 #pragma unroll
-      for (int u = 0; u < kPullLifGroup; ++u) {
-        const int b = b0 + u;
-        if (b >= a.B) break;
-        const size_t sj = (size_t)b * a.n + j;
-        float ext = 0.0f;
-        if ((inj >> b) & 1u)
-          for (int q = s_head[tid]; q >= 0; q = s_next[q])
-            if ((s_pool[q].x >> 16) == b) ext = __int_as_float(s_pool[q].y);
-        const bool f = lif_step(vv[u], rr[u], ext, acc[b], a.lif);
-        a.v[sj] = vv[u];
-        a.refr[sj] = rr[u];
-        if (f) {
-          word |= 1u << b;
-          atomicAdd(a.nspk + sj, 1);
-          atomicAdd(&s_spk[b], 1ull);
-          atomicAdd(&s_ev[b], (unsigned long long)deg);
-        }
-      }
-    }
-    a.F[(size_t)tm * a.n + j] = word;
-  }
-  if (a.raster) {  // sample b's raster words of this wave's 64 targets
-    for (int b = 0; b < a.B; ++b) {
-      const uint64_t m = ballot((word >> b) & 1u);
-      if (lane == 0) {
-        uint32_t* row = a.raster + (size_t)b * a.raster_stride + (size_t)t * a.words;
-        const int w0 = (base + wave * kWave) >> 5;
-        if (w0 < a.words) row[w0] = (uint32_t)m;
-        if (w0 + 1 < a.words) row[w0 + 1] = (uint32_t)(m >> 32);
-      }
-    }
-  }
-  __syncthreads();
-  if (tid < a.B && s_spk[tid]) {
-    atomicAdd(a.blk_spikes + (size_t)tid * a.blk_stride + blk, s_spk[tid]);
-    atomicAdd(a.blk_events + (size_t)tid * a.blk_stride + blk, s_ev[tid]);
-  }
+  for (int u = 0; u < kHubRounds; ++u) acc = add_round<Word>(wd[u], w[u], acc, s_mask);
+  a.hub_part[(size_t)c * kPullMaxSamples + lane] = acc;
 }
 
-// Hub chunk block: per-sample partial sums of one chunk of a hub column.
-// Thread q takes entries q, q + 256, ... (at most kPullHubPerThread), loads
-// them and their fired words in two rounds, sums them into its own LDS row;
-// rows are then added in a fixed tree order.
-__device__ void pull_hub_chunk(const PullArgs& a, int t, int c, unsigned char* lds) {
-  const int pitch = acc_pitch(a.B);
-  float* s_acc = reinterpret_cast<float*>(lds);
-  const int tid = threadIdx.x;
-  float* acc = s_acc + tid * pitch;
-  for (int b = 0; b < a.B; ++b) acc[b] = 0.0f;
-  const int64_t e0 = a.chunk_beg[c], e1 = a.chunk_beg[c + 1];
+// Hub h of step t (one wave, lane b = sample b): its chunks' partials in
+// chunk order, then integrate/fire.
+template <class Word>
+__device__ void pull_hub_step(const PullArgs& a, int t, int h) {
+  const int lane = lane_id();
   const int tm = t % a.L;
-  uint32_t pk[kPullHubPerThread], wd[kPullHubPerThread];
-  float w[kPullHubPerThread];
-#pragma unroll
-  for (int u = 0; u < kPullHubPerThread; ++u) {
-    const int64_t e = e0 + tid + (int64_t)u * kPullBlock;
-    const bool in = e < e1;
-    pk[u] = in ? a.hub_e[e] : 0u;
-    w[u] = in ? a.hub_w[e] : 0.0f;
-  }
-#pragma unroll
-  for (int u = 0; u < kPullHubPerThread; ++u) wd[u] = pk[u] & 0xFFu ? fired_word(a, pk[u], tm) : 0u;
-#pragma unroll
-  for (int u = 0; u < kPullHubPerThread; ++u) {
-    uint32_t m = wd[u];
-    while (m) {
-      const int b = __builtin_ctz(m);
-      m &= m - 1u;
-      acc[b] = __fadd_rn(acc[b], w[u]);
-    }
-  }
-  __syncthreads();
-  for (int half = kPullBlock / 2; half > 0; half >>= 1) {
-    if (tid < half) {
-      const float* other = s_acc + (tid + half) * pitch;
-      for (int b = 0; b < a.B; ++b) acc[b] = __fadd_rn(acc[b], other[b]);
-    }
-    __syncthreads();
-  }
-  if (tid < a.B) a.hub_part[(size_t)c * kPullMaxSamples + tid] = s_acc[tid];
-}
-
-__global__ __launch_bounds__(kPullBlock) void k_pull_step(PullArgs a, int t) {
-  extern __shared__ __align__(16) unsigned char s_dyn[];
-  if ((int)blockIdx.x < a.n_chunks)  // hub chunks first
-    pull_hub_chunk(a, t, blockIdx.x, s_dyn);
-  else
-    pull_light(a, t, (int)blockIdx.x - a.n_chunks, s_dyn);
-}
-
-// One wave per hub, lane b = sample b: the hub's partials in chunk order,
-// then integrate/fire.
-__global__ __launch_bounds__(kWave) void k_pull_hub_step(PullArgs a, int t) {
-  const int hidx = blockIdx.x, b = lane_id();
-  const int h = a.hub_ids[hidx];
+  const int hid = a.hub_ids[h];
+  const int c0 = a.hub_chunk_ptr[h], c1 = a.hub_chunk_ptr[h + 1];
   bool f = false;
-  if (b < a.B) {
+  if (lane < a.B) {
+    // the partials in chunk order, 16 loads in flight at a time
     float slot = 0.0f;
-    for (int c = a.hub_chunk_ptr[hidx]; c < a.hub_chunk_ptr[hidx + 1]; ++c)
-      slot = __fadd_rn(slot, a.hub_part[(size_t)c * kPullMaxSamples + b]);
-    const size_t sj = (size_t)b * a.n + h;
+    for (int cc = c0; cc < c1; cc += 16) {
+      float pt[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        pt[u] = cc + u < c1 ? a.hub_part[(size_t)(cc + u) * kPullMaxSamples + lane] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (cc + u < c1) slot = __fadd_rn(slot, pt[u]);
+    }
+    const size_t sj = (size_t)hid * a.B + lane;
     float v = a.v[sj];
     int r = a.refr[sj];
-    const float ext = a.inj_hub[(size_t)b * a.hinj_stride + (size_t)t * a.n_hub + hidx];
+    const int r0 = r;
+    const float ext = a.inj_hub[(size_t)lane * a.hinj_stride + (size_t)t * a.n_hub + h];
     f = lif_step(v, r, ext, slot, a.lif);
     a.v[sj] = v;
-    a.refr[sj] = r;
+    if (r != r0) a.refr[sj] = (uint8_t)r;
     if (f) {
-      atomicAdd(a.nspk + sj, 1);
-      const size_t slot_id = (size_t)b * a.blk_stride + a.n_light_blocks + hidx;
-      atomicAdd(a.blk_spikes + slot_id, 1ull);
-      atomicAdd(a.blk_events + slot_id, (unsigned long long)(a.row_ptr[h + 1] - a.row_ptr[h]));
+      a.nspk[sj] += 1;
       if (a.raster)
-        atomicOr(a.raster + (size_t)b * a.raster_stride + (size_t)t * a.words + (h >> 5),
-                 1u << (h & 31));
+        atomicOr(a.raster + (size_t)lane * a.raster_stride + (size_t)t * a.words + (hid >> 5),
+                 1u << (hid & 31));
     }
   }
   const uint64_t m = ballot(f);
-  if (b == 0) a.F[(size_t)(t % a.L) * a.n + h] = (uint32_t)m;
+  if (lane == 0) static_cast<Word*>(a.F)[(size_t)tm * a.n + hid] = (Word)m;
+}
+
+// A light target's first two rounds of entries and its state.
+struct Tgt {
+  uint32_t h0, h1;
+  float w0, w1, v;
+  int r, len, c0;
+};
+
+// One step t: hub integrate/fire (partials from the chunk launch), then the
+// light targets, 16 per wave in batches of kBatch whose loads go out
+// together. Eight waves per SIMD: two blocks share a CU, halos included.
+template <class Word, bool DIRECT>
+__global__ __launch_bounds__(kPullThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+k_pull_step(PullArgs a, int t) {
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  Word* s_halo = reinterpret_cast<Word*>(s_dyn);
+  unsigned long long* s_masks =
+      reinterpret_cast<unsigned long long*>(s_dyn + ((size_t)(a.halo + 1) * sizeof(Word) + 15) / 16 * 16);
+  int2* s_pool = reinterpret_cast<int2*>(reinterpret_cast<unsigned char*>(s_masks) + kMaskBytes);
+  int* s_next = reinterpret_cast<int*>(s_pool + a.pool);
+  int* s_head = s_next + a.pool;  // [kPullBlock]
+  int* s_pool_n = s_head + kPullBlock;
+
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+  // run of target blocks so the halos it stages overlap in its own L2.
+  const int R = a.n_blocks, x = blockIdx.x;
+  const int q8 = R >> 3, r8 = R & 7, xcd = x & 7;
+  const int blk = xcd * q8 + min(xcd, r8) + (x >> 3);
+  const int base = blk * kPullBlock;
+  const int tm = t % a.L;
+
+  // Halo: the fired word of every (pre, delay) the block's light columns
+  // use; each thread's loads go out together.
+  {
+    const int64_t h0 = a.halo_ptr[blk];
+    const int nh = (int)(a.halo_ptr[blk + 1] - h0);
+    if (nh > 0) {
+      uint32_t pk[kHaloPer];
+      Word wd[kHaloPer];
+#pragma unroll
+      for (int u = 0; u < kHaloPer; ++u) pk[u] = a.halo_pk[h0 + min(tid + u * kPullThreads, nh - 1)];
+#pragma unroll
+      for (int u = 0; u < kHaloPer; ++u) wd[u] = direct_word<Word>(a, pk[u], tm);
+#pragma unroll
+      for (int u = 0; u < kHaloPer; ++u)
+        if (tid + u * kPullThreads < nh) s_halo[tid + u * kPullThreads] = wd[u];
+    }
+  }
+  unsigned long long* s_mask = s_masks + wave * kWave;
+  s_mask[lane] = 0ull;
+  if (tid == 0) s_halo[a.halo] = (Word)0;  // the zero slot
+  if (tid < kPullBlock) s_head[tid] = -1;
+  if (tid == 0) *s_pool_n = 0;
+  __syncthreads();
+  // Injections of step t of every sample into the pool, linked per target.
+  // Wave w stages samples w, w + 16, ...: their list bounds load together,
+  // then their first 64 entries each.
+  {
+    constexpr int kPer = kPullMaxSamples / kPullWaves;
+    int k0[kPer], cnt[kPer], nid[kPer];
+    float amp[kPer];
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
+      const int b = wave + s * kPullWaves;
+      k0[s] = 0;
+      cnt[s] = 0;
+      if (b < a.B) {
+        const int32_t* bp =
+            a.inj_blk_ptr + (size_t)b * a.inj_stride + (size_t)t * (a.n_blocks + 1) + blk;
+        k0[s] = bp[0];
+        cnt[s] = bp[1] - k0[s];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
+      nid[s] = base;
+      amp[s] = 0.0f;
+      if (lane < cnt[s]) {
+        nid[s] = a.blk_nid[k0[s] + lane];
+        amp[s] = a.blk_amp[k0[s] + lane];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
+      if (cnt[s] <= 0) continue;
+      const int b = wave + s * kPullWaves;
+      int at = 0;
+      if (lane == 0) at = atomicAdd(s_pool_n, cnt[s]);
+      at = __shfl(at, 0, kWave);
+      for (int q = lane; q < cnt[s]; q += kWave) {
+        const int slot = at + q;
+        if (slot >= a.pool) break;  // the plan build sizes the pool for every (step, block)
+        const int id = q == lane ? nid[s] : a.blk_nid[k0[s] + q];
+        const float am = q == lane ? amp[s] : a.blk_amp[k0[s] + q];
+        const int loc = id - base;
+        s_pool[slot] = make_int2(loc | (b << 16), __float_as_int(am));
+        s_next[slot] = atomicExch(&s_head[loc], slot);
+      }
+    }
+  }
+  __syncthreads();
+
+  // Hubs: wave slot x * 16 + wave steps hubs in strides of the grid's waves.
+  for (int h = x * kPullWaves + wave; h < a.n_hub; h += (int)gridDim.x * kPullWaves)
+    pull_hub_step<Word>(a, t, h);
+
+  // Light targets.
+  const int jb = base + wave * kPullWaveTargets;
+  int my_c0 = 0, my_len = -1;
+  if (lane < kPullWaveTargets && jb + lane < a.n) {
+    my_c0 = a.loff[jb + lane];
+    my_len = a.lcnt[jb + lane];
+  }
+  const bool smp = lane < a.B;
+  const uint32_t lane_b = (uint32_t)min(lane, a.B - 1);
+  const uint32_t uB = (uint32_t)a.B;
+  Word* Fw = static_cast<Word*>(a.F);
+  uint32_t rbits = 0;
+  for (int i0 = 0; i0 < kPullWaveTargets; i0 += kBatch) {
+    // Every load of the batch before any is used (no branches: lanes past a
+    // column's end read the next entries of the stream, which is padded past
+    // its end), so they are in flight together.
+    Tgt p[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int i = i0 + u;
+      p[u].c0 = __builtin_amdgcn_readlane(my_c0, i);
+      p[u].len = __builtin_amdgcn_readlane(my_len, i);
+      const uint32_t o0 = (uint32_t)(p[u].c0 + lane);
+      p[u].h0 = ld32(a.lh, o0);
+      p[u].w0 = ld32(a.lw, o0);
+      p[u].h1 = ld32(a.lh, o0 + kWave);
+      p[u].w1 = ld32(a.lw, o0 + kWave);
+      const uint32_t so = (uint32_t)min(jb + i, a.n - 1) * uB + lane_b;
+      p[u].v = ld32(a.v, so);
+      p[u].r = ld32(a.refr, so);
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int i = i0 + u;
+      const int j = jb + i;
+      const int len = p[u].len;
+      if (len < 0) continue;  // a hub (stepped above) or past the last neuron
+      const uint32_t o0 = (uint32_t)(p[u].c0 + lane);
+      float acc = 0.0f;
+      acc = add_round<Word>(entry_word<Word, DIRECT>(a, s_halo, lane < len, p[u].h0, o0, tm),
+                            p[u].w0, acc, s_mask);
+      acc = add_round<Word>(
+          entry_word<Word, DIRECT>(a, s_halo, lane + kWave < len, p[u].h1, o0 + kWave, tm),
+          p[u].w1, acc, s_mask);
+      for (int k0 = 2 * kWave; k0 < len; k0 += kWave) {  // longer columns
+        const uint32_t o = o0 + (uint32_t)k0;
+        acc = add_round<Word>(
+            entry_word<Word, DIRECT>(a, s_halo, k0 + lane < len, ld32(a.lh, o), o, tm),
+            ld32(a.lw, o), acc, s_mask);
+      }
+      float ext = 0.0f;
+      for (int q = s_head[j - base]; q >= 0; q = s_next[q]) {
+        const int2 e = s_pool[q];
+        if ((e.x >> 16) == lane) ext = __int_as_float(e.y);
+      }
+      bool f = false;
+      if (smp) {
+        const uint32_t so = (uint32_t)j * uB + (uint32_t)lane;
+        float v = p[u].v;
+        int r = p[u].r;
+        f = lif_step(v, r, ext, acc, a.lif);
+        st32(a.v, so, v);
+        if (r != p[u].r) st32(a.refr, so, (uint8_t)r);
+        if (f) st32(a.nspk, so, ld32(a.nspk, so) + 1);
+      }
+      const uint64_t m = ballot(f);
+      if (lane == 0) st32(Fw, (uint32_t)tm * (uint32_t)a.n + (uint32_t)j, (Word)m);
+      rbits |= (f ? 1u : 0u) << i;
+    }
+  }
+  // Sample b's raster bits of the wave's 16 targets (half a raster word; hub
+  // bits come from the hub waves, so the word is or-ed).
+  if (a.raster && smp && rbits && jb < a.n)
+    atomicOr(a.raster + (size_t)lane * a.raster_stride + (size_t)t * a.words + (jb >> 5),
+             rbits << (jb & 31));
+}
+
+template <class Word, bool DIRECT>
+void set_step_lds() {
+  (void)hipFuncSetAttribute((const void*)k_pull_step<Word, DIRECT>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPullMaxLds);
 }
 
 }  // namespace
 
 void PullGraph::release() {
-  dfree(slab);
-  dfree(slab_w);
-  dfree(slice_ptr);
-  dfree(slice_len);
+  dfree(lw);
+  dfree(lh);
+  dfree(ldir);
+  dfree(loff);
+  dfree(lcnt);
+  dfree(halo_ptr);
+  dfree(halo);
   dfree(hub_ids);
-  dfree(hub_e);
-  dfree(hub_w);
   dfree(chunk_hub);
-  dfree(chunk_beg);
+  dfree(hub_pk);
+  dfree(hub_w);
+  dfree(chunk_pos);
+  dfree(chunk_len);
   dfree(hub_chunk_ptr);
-  dfree(is_hub);
   hub_host.clear();
   cap = -1;
-  n_slices = n_hub = n_chunks = 0;
-  entries = 0;
+  n_blocks = n_hub = n_chunks = halo_words = 0;
+  direct_entries = 0;
+  direct = false;
 }
 
 int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
   if (pg.cap == cap) return SNC_OK;
   pg.release();
   const int n = g.n;
-  if (n >= (1 << 24)) return fail(SNC_E_UNSUPPORTED, "hip-pull packs pre ids in 24 bits");
-  std::vector<uint8_t> is_hub((size_t)n, 0);
-  std::vector<int32_t> hub_of((size_t)n, -1), hubs;
-  for (int j = 0; j < n; ++j)
-    if (g.in_degree[j] > cap) {
-      hub_of[j] = (int32_t)hubs.size();
-      hubs.push_back(j);
-      is_hub[j] = 1;
+  if (n >= (1 << 23)) return fail(SNC_E_UNSUPPORTED, "hip-pull packs pre ids in 23 bits");
+  if (g.L > 256) return fail(SNC_E_UNSUPPORTED, "hip-pull packs delays in 8 bits");
+  // Host copies of the column layout: each position's (pre << 8) | delay.
+  std::vector<int32_t> row_ptr, cpos, ccol;
+  std::vector<uint8_t> cdel;
+  int rc = dload(row_ptr, g.row_ptr, (size_t)n + 1);
+  if (!rc) rc = dload(cpos, g.cpos, (size_t)g.S);
+  if (!rc) rc = dload(ccol, g.ccol, (size_t)n + 1);
+  if (!rc) rc = dload(cdel, g.cdel, (size_t)g.P);
+  if (rc) return rc;
+  std::vector<float> cvh;
+  rc = dload(cvh, g.cval, (size_t)g.P);
+  if (rc) return rc;
+  std::vector<uint32_t> cpk((size_t)g.P, 0u);
+  for (int i = 0; i < n; ++i)
+    for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+      const int p = cpos[(size_t)e];
+      cpk[(size_t)p] = ((uint32_t)i << 8) | cdel[(size_t)p];
     }
-  const int n_slices = (n + 63) / 64;
-  std::vector<int64_t> slice_ptr((size_t)n_slices, 0);
-  std::vector<int32_t> slice_len((size_t)n_slices, 0);
-  int64_t entries = 0;
-  for (int s = 0; s < n_slices; ++s) {
-    int len = 0;
-    for (int j = s * 64; j < std::min(n, s * 64 + 64); ++j)
-      if (!is_hub[j]) len = std::max(len, g.in_degree[j]);
-    slice_ptr[s] = entries;
-    slice_len[s] = len;
-    entries += (int64_t)len * 64;
-  }
-  std::vector<int64_t> hub_off(hubs.size() + 1, 0);
-  std::vector<int32_t> chunk_hub, hub_chunk_ptr(hubs.size() + 1, 0);
-  std::vector<int64_t> chunk_beg;
-  for (size_t h = 0; h < hubs.size(); ++h) {
-    const int64_t deg = g.in_degree[hubs[h]];
-    hub_off[h + 1] = hub_off[h] + deg;
-    for (int64_t q = 0; q < deg; q += kPullHubChunk) {
-      chunk_hub.push_back((int32_t)h);
-      chunk_beg.push_back(hub_off[h] + q);
+
+  const int R = std::max(1, (n + kPullBlock - 1) / kPullBlock);
+  std::vector<int32_t> lcnt((size_t)n, 0), hubs, chunk_hub, chunk_pos, chunk_len;
+  std::vector<int32_t> hub_chunk_ptr(1, 0);
+  std::vector<uint32_t> hub_pk;
+  std::vector<float> hub_w;
+  std::vector<std::pair<uint64_t, int32_t>> ord;
+  for (int j = 0; j < n; ++j) {
+    const int deg = g.in_degree[j];
+    if (deg <= cap) {
+      lcnt[j] = deg;
+      continue;
     }
-    hub_chunk_ptr[h + 1] = (int32_t)chunk_hub.size();
+    lcnt[j] = -1;
+    const int h = (int)hubs.size();
+    hubs.push_back(j);
+    // the hub's entries by (delay, pre), ties in column order
+    ord.clear();
+    for (int k = 0; k < deg; ++k) {
+      const uint32_t pk = cpk[(size_t)ccol[j] + k];
+      ord.push_back({((uint64_t)(pk & 0xFFu) << 56) | ((uint64_t)(pk >> 8) << 32) | (uint32_t)k, k});
+    }
+    std::sort(ord.begin(), ord.end());
+    const int32_t e0 = (int32_t)hub_pk.size();
+    for (const auto& o : ord) {
+      hub_pk.push_back(cpk[(size_t)ccol[j] + o.second]);
+      hub_w.push_back(cvh[(size_t)ccol[j] + o.second]);
+    }
+    for (int k = 0; k < deg; k += kPullHubChunk) {
+      chunk_hub.push_back(h);
+      chunk_pos.push_back(e0 + k);
+      chunk_len.push_back(std::min(kPullHubChunk, deg - k));
+    }
+    hub_chunk_ptr.push_back((int32_t)chunk_hub.size());
   }
-  chunk_beg.push_back(hub_off[hubs.size()]);
-  pg.n_slices = n_slices;
+  // The light stream: every light target's entries in column order, back to
+  // back. Per block: the distinct (delay, pre) of its light columns, sorted;
+  // the first kPullHaloCap are staged, the rest read F directly.
+  int64_t n_light = 0;
+  std::vector<int32_t> loff((size_t)n, 0);
+  for (int j = 0; j < n; ++j) {
+    loff[j] = (int32_t)n_light;
+    n_light += std::max(0, lcnt[j]);
+  }
+  if (n_light + kPullStreamPad >= (1ll << 30)) return fail(SNC_E_UNSUPPORTED, "hip-pull: too many light entries");
+  std::vector<float> lw((size_t)(n_light + kPullStreamPad), 0.0f);
+  std::vector<uint16_t> lh((size_t)(n_light + kPullStreamPad), kHaloDirect);
+  std::vector<uint32_t> ldir;
+  std::vector<int64_t> halo_ptr((size_t)R + 1, 0);
+  std::vector<uint32_t> halo;
+  std::vector<uint32_t> keys;
+  int halo_max = 0;
+  int64_t direct = 0;
+  auto key_of = [](uint32_t pk) { return ((pk & 0xFFu) << 24) | (pk >> 8); };  // (delay, pre)
+  for (int q = 0; q < R; ++q) {
+    const int j0 = q * kPullBlock, j1 = std::min(n, j0 + kPullBlock);
+    keys.clear();
+    for (int j = j0; j < j1; ++j)
+      for (int k = 0; k < lcnt[j]; ++k) keys.push_back(key_of(cpk[(size_t)ccol[j] + k]));
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int keep = std::min<int>((int)keys.size(), kPullHaloCap);
+    halo_max = std::max(halo_max, keep);
+    for (int k = 0; k < keep; ++k) halo.push_back(((keys[k] & 0xFFFFFFu) << 8) | (keys[k] >> 24));
+    halo_ptr[q + 1] = (int64_t)halo.size();
+    for (int j = j0; j < j1; ++j)
+      for (int k = 0; k < lcnt[j]; ++k) {
+        const size_t p = (size_t)ccol[j] + k;
+        const size_t o = (size_t)loff[j] + k;
+        const int rank = (int)(std::lower_bound(keys.begin(), keys.end(), key_of(cpk[p])) - keys.begin());
+        lw[o] = cvh[p];
+        if (rank < keep) {
+          lh[o] = (uint16_t)rank;
+        } else {
+          if (ldir.empty()) ldir.assign(lw.size(), 0u);
+          lh[o] = kHaloDirect;
+          ldir[o] = cpk[p];
+          ++direct;
+        }
+      }
+  }
+  if (ldir.empty()) ldir.assign(1, 0u);
+  pg.n_blocks = R;
   pg.n_hub = (int)hubs.size();
   pg.n_chunks = (int)chunk_hub.size();
-  pg.entries = entries;
+  pg.halo_words = std::max(1, halo_max);
+  pg.direct_entries = direct;
+  pg.direct = direct > 0;
   pg.hub_host = hubs;
-  int rc = SNC_OK;
-  int32_t* d_hub_of = nullptr;
-  int64_t* d_hub_off = nullptr;
-  if (!rc) rc = dalloc(&pg.slab, (size_t)std::max<int64_t>(entries, 1));
-  if (!rc) rc = dalloc(&pg.slab_w, (size_t)std::max<int64_t>(entries, 1));
-  if (!rc) rc = dupload(&pg.slice_ptr, slice_ptr);
-  if (!rc) rc = dupload(&pg.slice_len, slice_len);
+  if (!rc) rc = dupload(&pg.lw, lw);
+  if (!rc) rc = dupload(&pg.lh, lh);
+  if (!rc) rc = dupload(&pg.ldir, ldir);
+  if (!rc) rc = dupload(&pg.loff, loff);
+  if (!rc) rc = dupload(&pg.lcnt, lcnt);
+  if (!rc) rc = dupload(&pg.halo_ptr, halo_ptr);
+  if (!rc) rc = dupload(&pg.halo, halo.empty() ? std::vector<uint32_t>(1, 0u) : halo);
   if (!rc) rc = dupload(&pg.hub_ids, hubs);
-  if (!rc) rc = dalloc(&pg.hub_e, (size_t)std::max<int64_t>(hub_off.back(), 1));
-  if (!rc) rc = dalloc(&pg.hub_w, (size_t)std::max<int64_t>(hub_off.back(), 1));
   if (!rc) rc = dupload(&pg.chunk_hub, chunk_hub);
-  if (!rc) rc = dupload(&pg.chunk_beg, chunk_beg);
+  if (!rc) rc = dupload(&pg.hub_pk, hub_pk);
+  if (!rc) rc = dupload(&pg.hub_w, hub_w);
+  if (!rc) rc = dupload(&pg.chunk_pos, chunk_pos);
+  if (!rc) rc = dupload(&pg.chunk_len, chunk_len);
   if (!rc) rc = dupload(&pg.hub_chunk_ptr, hub_chunk_ptr);
-  if (!rc) rc = dupload(&pg.is_hub, is_hub);
-  if (!rc) rc = dupload(&d_hub_of, hub_of);
-  if (!rc) rc = dupload(&d_hub_off, hub_off);
-  if (!rc) {
-    hipError_t e = hipMemset(pg.slab, 0, (size_t)std::max<int64_t>(entries, 1) * 4);
-    if (e == hipSuccess) e = hipMemset(pg.slab_w, 0, (size_t)std::max<int64_t>(entries, 1) * 4);
-    if (e != hipSuccess) rc = hip_fail(e, "pull column clear");
-  }
-  if (!rc && n > 0) {
-    hipLaunchKernelGGL(k_pull_build, dim3((n + 255) / 256), dim3(256), 0, nullptr, n, g.row_ptr,
-                       g.post, g.wsig, g.delay, g.cpos, g.ccol, d_hub_of, d_hub_off,
-                       pg.slice_ptr, pg.slab, pg.slab_w, pg.hub_e, pg.hub_w);
-    rc = hip_last("pull column build launch");
-    if (!rc) {
-      const hipError_t e = hipDeviceSynchronize();
-      if (e != hipSuccess) rc = hip_fail(e, "pull column build");
-    }
-  }
-  dfree(d_hub_of);
-  dfree(d_hub_off);
   if (rc) {
     pg.release();
     return rc;
@@ -426,25 +547,43 @@ int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
   return SNC_OK;
 }
 
-int pull_launch_step(const PullArgs& a, int t, hipStream_t st) {
-  hipLaunchKernelGGL(k_pull_step, dim3(a.n_chunks + a.n_light_blocks), dim3(kPullBlock),
-                     (unsigned)pull_lds_bytes(a.B, a.pool), st, a, t);
-  int rc = hip_last("pull step launch failed");
-  if (!rc && a.n_hub > 0) {
-    hipLaunchKernelGGL(k_pull_hub_step, dim3(a.n_hub), dim3(kWave), 0, st, a, t);
-    rc = hip_last("pull hub step launch failed");
-  }
-  return rc;
+size_t pull_lds_bytes(int B, int halo, int pool) {
+  return lds_bytes(pull_word_bytes(B), halo, pool);
 }
 
-int pull_set_lds(int B, int pool) {
-  const size_t bytes = pull_lds_bytes(B, pool);
-  if (bytes > kPullMaxLds) return fail(SNC_E_UNSUPPORTED, "hip-pull: injection pool exceeds LDS");
-  // The ceiling, not this plan's size: plans with other pools share the kernel.
-  const hipError_t e = hipFuncSetAttribute((const void*)k_pull_step,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)kPullMaxLds);
-  return e == hipSuccess ? SNC_OK : hip_fail(e, "hip-pull LDS request");
+int pull_launch_step(const PullArgs& a, int t, hipStream_t st) {
+  const bool wide = a.B > 32;
+  if (a.n_chunks > 0) {
+    const int hb = (a.n_chunks + kHubThreads / kWave - 1) / (kHubThreads / kWave);
+    if (wide)
+      hipLaunchKernelGGL(k_pull_hub_chunks<uint64_t>, dim3(hb), dim3(kHubThreads), 0, st, a, t);
+    else
+      hipLaunchKernelGGL(k_pull_hub_chunks<uint32_t>, dim3(hb), dim3(kHubThreads), 0, st, a, t);
+    const int rc = hip_last("pull hub chunk launch failed");
+    if (rc) return rc;
+  }
+  const size_t lds = pull_lds_bytes(a.B, a.halo, a.pool);
+  const dim3 grid(a.n_blocks), block(kPullThreads);
+  if (wide && a.direct)
+    hipLaunchKernelGGL((k_pull_step<uint64_t, true>), grid, block, lds, st, a, t);
+  else if (wide)
+    hipLaunchKernelGGL((k_pull_step<uint64_t, false>), grid, block, lds, st, a, t);
+  else if (a.direct)
+    hipLaunchKernelGGL((k_pull_step<uint32_t, true>), grid, block, lds, st, a, t);
+  else
+    hipLaunchKernelGGL((k_pull_step<uint32_t, false>), grid, block, lds, st, a, t);
+  return hip_last("pull step launch failed");
+}
+
+int pull_set_lds(int B, int halo, int pool) {
+  const size_t bytes = pull_lds_bytes(B, halo, pool);
+  if (bytes > kPullMaxLds) return fail(SNC_E_UNSUPPORTED, "hip-pull: halo and injection pool exceed LDS");
+  // The ceiling, not this plan's size: plans of other sizes share the kernels.
+  set_step_lds<uint32_t, false>();
+  set_step_lds<uint32_t, true>();
+  set_step_lds<uint64_t, false>();
+  set_step_lds<uint64_t, true>();
+  return hip_last("hip-pull LDS request");
 }
 
 void pull_free(PullGraph* pg) { delete pg; }

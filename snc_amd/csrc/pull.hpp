@@ -1,21 +1,38 @@
-// hip-pull: batched forward by gathering, for many samples at once.
+// hip-pull: batched forward by gathering, for up to 64 samples at once.
 //
-// Every neuron's firing of step t over the plan's B <= 32 samples is one
-// 32-bit word F[t % L][i] (bit b: sample b fired). A target's input at step t
-// is the sum, over its incoming synapses (pre i, delay d, value w) in column
-// order, of w for every sample whose bit is set in F[(t - d) % L][i]: the
-// deposits the CPU ring holds for slot t, gathered instead of scattered. No
-// atomics are needed and one pass over the synapses serves all B samples.
+// Every neuron's firing of step t over the plan's B samples is one word
+// F[t % L][i] (bit b: sample b fired; 32-bit words for B <= 32, 64-bit for
+// B <= 64). A target's input at step t is the sum, over its incoming
+// synapses (pre i, delay d, value w) in column order, of w for every sample
+// whose bit is set in F[(t - d) % L][i]: the deposits the CPU ring holds for
+// slot t, gathered instead of scattered. No atomics are needed and one pass
+// over the synapses serves all B samples.
 //
-// Light targets (in-degree up to the plan's cap) sit in 64-target slices,
-// stored sliced (SELL-64): entry k of every target of a slice side by side,
-// so a wave reads 64 consecutive words per step k. Each lane walks its own
-// column sequentially and adds into its own LDS row, so a light target's sum
-// is the CPU's, bit for bit. Hub targets (in-degree above the cap) are split
-// into chunks of kPullHubChunk synapses; a chunk block sums per-sample
-// partials in a fixed order and a second launch adds a hub's partials in
-// chunk order and steps the hub, so hub sums are deterministic but
-// associated differently from the CPU's (the atomic-class tolerance).
+// Light targets (in-degree up to the plan's cap). Their column entries are
+// stored back to back in column order as a value (fp32) and a 16-bit halo
+// index. One wave steps one target at a time with lane b = sample b: the
+// wave loads 64 column entries at once, looks up each entry's
+// fired word in LDS, and walks the entries whose word is nonzero in column
+// order; lane b adds the entry's value when bit b is set (and +0 otherwise,
+// which leaves a sum started from +0 unchanged). Every sample's sum is thus
+// the CPU's sequential slot sum, bit for bit. The fired words a block needs
+// are its halo: the distinct (pre, delay) pairs of its light targets'
+// columns, staged in LDS once per step (sorted by delay, then pre, so the
+// loads coalesce); entries beyond the halo capacity read F directly.
+//
+// Hub targets (in-degree above the cap) keep their entries sorted by
+// (delay, pre), so consecutive gathers of a chunk share F's cache lines, and
+// are cut into chunks of at most kPullHubChunk entries. A chunk is one wave's work in a launch of its own
+// before the step launch: it gathers the chunk's fired words from F, sums
+// them per sample in column order and stores the partials; in the step
+// launch one wave per hub adds its partials in chunk order and steps the hub.
+// Hub sums are deterministic but associated differently from the CPU's (the
+// atomic-class tolerance).
+//
+// State is sample-minor, [n][B]: a wave's 64 lanes touch one target's samples
+// in one 256-byte run. Spike and event counts are not accumulated per step:
+// the host derives them from the per-neuron spike counts (events = spikes of
+// neuron i times its out-degree, as cpu_backend.cpp:168-186 counts them).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,82 +41,102 @@
 #include <vector>
 
 #include "forward_plan.hpp"
+#include "wave.hpp"
 
 namespace snc_amd {
 
-constexpr int kPullBlock = 256;      // targets per light block: 4 slices of 64
-constexpr int kPullMaxSamples = 32;  // bits of one F word
-constexpr int kPullHubChunk = 2048;  // hub synapses per chunk block
-constexpr size_t kPullMaxLds = 150 * 1024;  // LDS a light block may take (pool included)
-constexpr int kPullMaxCap = 1024;    // largest light in-degree cap
+constexpr int kPullBlock = 256;        // targets per light block
+constexpr int kPullThreads = 1024;     // 16 waves, 16 targets each
+constexpr int kPullWaveTargets = kPullBlock / (kPullThreads / kWave);
+constexpr int kPullMaxSamples = 64;    // bits of the widest F word
+constexpr int kPullHubChunk = 512;     // hub column entries per chunk wave
+constexpr int kPullHaloCap = 9216;     // fired words a block stages in LDS
+constexpr size_t kPullMaxLds = 150 * 1024;
+constexpr int kPullMaxCap = 1024;      // largest light in-degree cap
 
-// Sliced incoming columns of one graph at one light cap (device arrays).
+// A light entry's halo index is its LDS slot; kHaloDirect marks an entry past
+// its block's halo capacity, which reads F directly ((pre << 8) | delay from
+// the side array). kHaloNone marks a lane past a hub chunk's end.
+constexpr uint16_t kHaloDirect = 0xFFFFu;
+constexpr uint32_t kHaloNone = 0xFFFFFFFFu;
+constexpr int kPullStreamPad = 128;    // entries past the light stream's end (over-reads)
+
+// Per-graph, per-cap arrays (device).
 struct PullGraph {
   int cap = -1;
-  int n_slices = 0;
+  int n_blocks = 0;
   int n_hub = 0, n_chunks = 0;
-  uint32_t* slab = nullptr;    // packed (pre << 8) | delay, 0 = padding
-  float* slab_w = nullptr;     // value of each entry
-  int64_t* slice_ptr = nullptr;  // [n_slices] first entry of slice s
-  int32_t* slice_len = nullptr;  // [n_slices] entries per lane
-  int32_t* hub_ids = nullptr;    // [n_hub]
-  uint32_t* hub_e = nullptr;     // hub columns, packed like slab, column order
-  float* hub_w = nullptr;
-  int32_t* chunk_hub = nullptr;  // [n_chunks] hub index of a chunk
-  int64_t* chunk_beg = nullptr;  // [n_chunks + 1] chunk bounds in hub_e
-  int32_t* hub_chunk_ptr = nullptr;  // [n_hub + 1] chunks of hub h
-  uint8_t* is_hub = nullptr;     // [n]
+  int halo_words = 0;             // largest staged halo (LDS words)
+  bool direct = false;            // some light entry reads F directly
+  int64_t direct_entries = 0;     // light entries past a block's halo capacity
+  float* lw = nullptr;            // light entries' values, target by target in column order
+  uint16_t* lh = nullptr;         // their halo indices (kHaloDirect: see ldir)
+  uint32_t* ldir = nullptr;       // (pre << 8) | delay of direct entries (else unused)
+  int32_t* loff = nullptr;        // [n] first light entry of a target
+  int32_t* lcnt = nullptr;        // [n] light entries of a target, -1 for hubs
+  int64_t* halo_ptr = nullptr;    // [n_blocks + 1]
+  uint32_t* halo = nullptr;       // (pre << 8) | delay, per block by (delay, pre)
+  int32_t* hub_ids = nullptr;     // [n_hub]
+  int32_t* chunk_hub = nullptr;   // [n_chunks] hub index
+  uint32_t* hub_pk = nullptr;     // hub columns: (pre << 8) | delay, by (delay, pre) per hub
+  float* hub_w = nullptr;         // the value of each hub entry
+  int32_t* chunk_pos = nullptr;   // [n_chunks] first entry in hub_pk / hub_w
+  int32_t* chunk_len = nullptr;   // [n_chunks] entries
+  int32_t* hub_chunk_ptr = nullptr;  // [n_hub + 1]
   std::vector<int32_t> hub_host;
-  int64_t entries = 0;
-  int64_t dev_bytes = 0;  // device memory of these arrays
+  int64_t dev_bytes = 0;
   void release();
   ~PullGraph() { release(); }
 };
 
 struct PullArgs {
-  int n, L, B, words, n_light_blocks;
-  int pool;  // injection entries a light block stages per step (the plan's largest (step, block))
+  int n, L, B, T, words, n_blocks;
+  int pool;   // injection entries a block stages per step
+  int halo;   // LDS halo words of this launch
+  int direct; // some light entry reads F directly
   LifK lif;
-  uint32_t sample_mask;
   // graph
-  const int32_t* row_ptr;
-  const uint32_t* slab;
-  const float* slab_w;
-  const int64_t* slice_ptr;
-  const int32_t* slice_len;
-  const uint8_t* is_hub;
+  const float* lw;
+  const uint16_t* lh;
+  const uint32_t* ldir;
+  const int32_t* loff;
+  const int32_t* lcnt;
+  const int64_t* halo_ptr;
+  const uint32_t* halo_pk;
   const int32_t* hub_ids;
-  const uint32_t* hub_e;
+  const uint32_t* hub_pk;
   const float* hub_w;
   const int32_t* chunk_hub;
-  const int64_t* chunk_beg;
+  const int32_t* chunk_pos;
+  const int32_t* chunk_len;
   const int32_t* hub_chunk_ptr;
   int n_hub, n_chunks;
-  // injections: per (sample, step, light block) lists and dense hub table
-  const int32_t* inj_blk_ptr;  // [B][T][n_light_blocks + 1]
+  // injections: per (sample, step, block) lists and a dense hub table
+  const int32_t* inj_blk_ptr;  // [B][T][n_blocks + 1]
   const int32_t* blk_nid;
   const float* blk_amp;
   const float* inj_hub;        // [B][T][n_hub]
   int64_t inj_stride, hinj_stride;
   // state
-  uint32_t* F;      // [L][n]
-  float* hub_part;  // [n_chunks][kPullMaxSamples]
-  float* v;         // [B][n]
-  int32_t* refr;    // [B][n]
-  int32_t* nspk;    // [B][n]
-  unsigned long long* blk_spikes;  // [B][blk_stride]
-  unsigned long long* blk_events;
-  int64_t blk_stride;
-  uint32_t* raster;  // [B][T][words] or null
+  void* F;            // [L][n] words
+  float* hub_part;    // [n_chunks][kPullMaxSamples]
+  float* v;           // [n][B]
+  uint8_t* refr;      // [n][B] refractory steps left (the refractory period is at most 255)
+  int32_t* nspk;      // [n][B]
+  uint32_t* raster;   // [B][T][words] or null
   int64_t raster_stride;
 };
 
-// Builds the sliced columns of `g` at light cap `cap` (nothing if pg already
-// holds that cap).
+// Builds the halo-indexed columns of `g` at light cap `cap` (nothing if pg
+// already holds that cap).
 int pull_prepare(const DevGraph& g, int cap, PullGraph& pg);
-// Allows the step kernel the LDS of B samples and a pool of `pool` entries.
-int pull_set_lds(int B, int pool);
-// One step: the light/hub-chunk launch, then the hub launch.
+// LDS bytes of a step launch.
+size_t pull_lds_bytes(int B, int halo, int pool);
+// Allows the step kernel of B samples the LDS it needs.
+int pull_set_lds(int B, int halo, int pool);
+// One step: the hub-chunk launch (if the graph has hubs), then the step launch.
 int pull_launch_step(const PullArgs& a, int t, hipStream_t st);
+// Bytes of one F word for B samples.
+inline int pull_word_bytes(int B) { return B > 32 ? 8 : 4; }
 
 }  // namespace snc_amd

@@ -70,6 +70,33 @@ def test_c1_full_size(c1, backend):
     plan.close()
 
 
+def test_c1_full_size_pull_64_samples(c1):
+    """64 independent drives of the 20M-synapse graph in one hip-pull plan
+    (64-bit fired words, default light cap, hubs in play): every checked
+    sample has the oracle's per-neuron counts, totals and membranes within
+    1e-5; the rest of the batch shares the launches."""
+    from snc_amd import inputs as I
+    w, hg, ref0 = c1
+    B = 64
+    evs = [w.samples[0]] + [I.background_drive(w.graph.num_neurons, w.num_steps, 0.01, 1.0,
+                                               seed=100 + b) for b in range(1, B)]
+    allev = np.concatenate([e.copy() for e in evs])
+    off = 0
+    for b, e in enumerate(evs):
+        allev["sample_id"][off:off + e.size] = b
+        off += e.size
+    print("c1 full size, hip-pull x64", flush=True)
+    plan = ForwardPlan(hg, allev, w.num_steps, w.lif, Backend.HipPull, num_samples=B)
+    r = plan.run(capture_v=True)
+    sp, ev = plan.sample_stats()
+    for b in (0, 1, 33, 63):
+        ref = ref0 if b == 0 else O.forward(w.graph, evs[b], w.num_steps, w.lif)
+        assert sp[b] == ref["spikes"] and ev[b] == ref["events"], b
+        assert np.array_equal(r.neuron_spikes[b], ref["neuron_spikes"]), b
+        np.testing.assert_allclose(r.final_v[b], ref["final_v"], rtol=V_RTOL, atol=V_ATOL)
+    plan.close()
+
+
 @pytest.mark.parametrize("cfg", [PlanConfig(Backend.HipBucket, 256, 512, True),
                                  PlanConfig(Backend.HipBucket, 512, 128, True),
                                  PlanConfig(Backend.HipAtomic, 512, 512, True),

@@ -1,10 +1,11 @@
-"""hip-pull (batched gather over sliced incoming columns) through the C ABI.
+"""hip-pull (batched gather over halo-indexed incoming columns) through the C ABI.
 
 Light targets (in-degree up to the plan's cap) sum their inputs in column
 order, so without hubs every sample's raster, counts, logits and membranes
-equal the CPU oracle's bit for bit (and hip-bucket's). Hub targets add
-per-chunk partial sums: counts identical, voltages within 1e-5 relative, and
-the same bits on every run.
+equal the CPU oracle's bit for bit (and hip-bucket's), with 32-bit fired words
+(up to 32 samples) and 64-bit ones (33 to 64). Hub targets add per-chunk
+partial sums: counts identical, voltages within 1e-5 relative, and the same
+bits on every run.
 """
 import numpy as np
 import pytest
@@ -37,7 +38,7 @@ def _pull(hg, allev, T, lif, B, cap):
                        num_samples=B)
 
 
-@pytest.mark.parametrize("B", [1, 7, 32])
+@pytest.mark.parametrize("B", [1, 7, 32, 33, 64])
 def test_pull_without_hubs_is_bit_exact(B):
     g = G.sheet_graph([400, 300, 20], 24, recurrent_fan_in=16, radius=3.0, sheet=20.0,
                       w_std=0.35, max_delay=9, seed=3)
@@ -63,15 +64,17 @@ def test_pull_without_hubs_is_bit_exact(B):
         assert np.array_equal(r.step_spikes[b], ref["step_spikes"])
 
 
-def test_pull_with_hubs_within_tolerance_and_deterministic():
+@pytest.mark.parametrize("B", [12, 50])
+def test_pull_with_hubs_within_tolerance_and_deterministic(B):
     g = G.hub_sheet_graph(60, 50, 40, hub_fraction=0.02, hub_min=40, hub_max=5000,
                           w_std=0.06, max_delay=20, seed=9)
     lif = K.lif_tau20()
-    T, B = 90, 12
+    T = 90
     evs, allev = _drives(g.num_neurons, T, B, seed=20)
     hg = HipGraph(g)
     cap = 64
-    assert int((g.in_degree() > cap).sum()) > 20  # hubs in play, some of several chunks
+    deg = g.in_degree()
+    assert int((deg > cap).sum()) > 20 and int(deg.max()) > 1024  # hubs, some of several chunks
     plan = _pull(hg, allev, T, lif, B, cap)
     r = plan.run(capture_raster=True, capture_v=True)
     sp, ev = plan.sample_stats()
@@ -90,9 +93,9 @@ def test_pull_rejects_what_it_cannot_hold():
     g = G.sheet_graph([64, 48, 8], 12, seed=2)
     hg = HipGraph(g)
     lif = K.lif_tau20()
-    _, allev = _drives(g.num_neurons, 10, 33, seed=1)
-    with pytest.raises(L.HipError, match="at most 32"):
-        _pull(hg, allev, 10, lif, 33, 64)
+    _, allev = _drives(g.num_neurons, 10, 65, seed=1)
+    with pytest.raises(L.HipError, match="at most 64"):
+        _pull(hg, allev, 10, lif, 65, 64)
     with pytest.raises(L.HipError, match="256-neuron"):
         ForwardPlan(hg, allev[allev["sample_id"] < 2], 10, lif,
                     config=PlanConfig(Backend.HipPull, 512, 64, False), num_samples=2)
@@ -136,3 +139,23 @@ def test_batched_autotuner_considers_pull():
     for b in range(B):
         ref = O.forward(g, evs[b], T, lif)
         assert np.array_equal(r.neuron_spikes[b], ref["neuron_spikes"])
+
+
+@pytest.mark.parametrize("B", [9, 40])
+def test_pull_halo_overflow_reads_fired_words_directly(B):
+    """Random recurrent columns (no locality): each 256-target block uses more
+    distinct (pre, delay) words than its LDS halo holds, so part of every
+    column reads F directly; still bit-exact without hubs."""
+    g = G.random_recurrent_graph(700, 1024, 20, 0.1, 0.1, 1, 31, 1.0 / np.sqrt(102.0), seed=4)
+    cap = int(g.in_degree().max())
+    assert cap <= 1024
+    lif = K.lif_tau20()
+    T = 70
+    evs, allev = _drives(g.num_neurons, T, B, seed=70, p0=0.04, dp=0.01)
+    hg = HipGraph(g)
+    r = _pull(hg, allev, T, lif, B, cap).run(capture_raster=True, capture_v=True)
+    for b in range(B):
+        ref = O.forward(g, evs[b], T, lif)
+        assert ref["spikes"] > 0
+        assert np.array_equal(r.raster[b], ref["raster"]), b
+        assert np.array_equal(r.final_v[b].view(np.uint32), ref["final_v"].view(np.uint32))
