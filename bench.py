@@ -110,9 +110,10 @@ def expansion_floor(g, hg, plan, num_steps, kernel_ms):
             "frac": us / k_us if k_us > 0 else None}
 
 
-def measured_traffic(kernel_symbol):
+def measured_traffic(kernel_symbol, label=None):
     """HBM bytes per dispatch of a kernel from the committed counter summaries
-    (profiles/*pmc_traffic*.md: calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    (profiles/*pmc_traffic*.md: calibrated FETCH_SIZE + WRITE_SIZE), or None.
+    label, when given, must match the summary's workload label too."""
     import glob
     import re
 
@@ -120,10 +121,14 @@ def measured_traffic(kernel_symbol):
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*pmc_traffic*.md"))):
         for line in open(path):
             m = pat.search(line)
-            if m and m.group(1) == kernel_symbol:
+            if m and m.group(1) == kernel_symbol and (label is None or m.group(2) == label):
                 return {"bytes_per_launch": float(m.group(3)),
                         "source": os.path.relpath(path, ROOT)}
     return None
+
+
+PULL_KERNELS = ("snc_amd::(anonymous namespace)::k_pull_step<{w}, false>",
+                "snc_amd::(anonymous namespace)::k_pull_hub_chunks<{w}>")
 
 
 def model_bytes(neuron_steps, events, fired):
@@ -506,6 +511,16 @@ def batched_forward(w, hg, B, steps, warmup, variant):
            "whole_run": {"gbs": step_bytes / wall / 1e9,
                          "frac": step_bytes / wall / 1e9 / HBM_PEAK_GBS},
            "kernels": kernels}
+    if plan.variant == parse_backend("hip-pull"):
+        # counted HBM traffic of a step (the hub-chunk launch plus the step
+        # launch), from the committed counter summary of this workload
+        word = "unsigned long" if B > 32 else "unsigned int"
+        parts = [measured_traffic(k.format(w=word), f"hip-pull_x{B}") for k in PULL_KERNELS]
+        if all(parts):
+            by = sum(p["bytes_per_launch"] for p in parts)
+            out["counted_traffic"] = {"bytes_per_step": by, "source": parts[0]["source"],
+                                      "gbs": by * w.num_steps / wall / 1e9,
+                                      "frac": by * w.num_steps / wall / 1e9 / HBM_PEAK_GBS}
     log(f"batched forward x{B} ({out['variant']}): {wall * 1e3:.2f} ms/run, "
         f"{out['GSynOps/s']:.2f} GSynOps/s, whole-run {out['whole_run']['frac'] * 100:.1f}% of HBM")
     plan.close()
@@ -538,8 +553,11 @@ def main():
     ap.add_argument("--eprop-epochs", type=int, default=2)
     ap.add_argument("--plain-batches", type=int, default=8,
                     help="minibatches the plain per-sample path is timed on (0: skip it)")
-    ap.add_argument("--batch-samples", default="8,32",
+    ap.add_argument("--batch-samples", default="8,32,64",
                     help="sample counts of the batched-forward lines (empty: none)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched-forward lines (--batch-samples x --batched-variants)")
+    ap.add_argument("--batched-variants", default="hip-auto,hip-auto-exact,hip-pull")
     ap.add_argument("--no-eprop", action="store_true")
     ap.add_argument("--no-grow", action="store_true",
                     help="skip the learned-vs-heuristic grow comparison")
@@ -642,6 +660,17 @@ def main():
     log(f"workload {w.name}: generated in {time.time() - t0:.1f}s, "
         f"{w.samples[0].size} drive events")
     hg = HipGraph(g)
+
+    def batched_lines():
+        res = {}
+        for bs in [int(x) for x in args.batch_samples.split(",") if x]:
+            for vn in [v for v in args.batched_variants.split(",") if v]:
+                res[f"{vn}_x{bs}"] = batched_forward(w, hg, bs, max(1, args.steps // 2), 1, vn)
+        return res
+
+    if args.batched_only:
+        print(json.dumps({"batched_forward": batched_lines()}))
+        return
 
     def make_plan(name):
         if args.block and name == args.variant and not name.startswith("hip-auto"):
@@ -760,11 +789,7 @@ def main():
                       "graph_bytes_per_synapse": hg.memory() / max(1, g.num_synapses),
                       "plan_bytes_per_neuron": plan.memory() / max(1, g.num_neurons),
                       "overflow": ref.stats.overflow}}
-    out["batched_forward"] = {}
-    for bs in [int(x) for x in args.batch_samples.split(",") if x]:
-        for vn in ("hip-auto", "hip-auto-exact", "hip-pull"):
-            out["batched_forward"][f"{vn}_x{bs}"] = batched_forward(w, hg, bs, max(1, args.steps // 2),
-                                                                    1, vn)
+    out["batched_forward"] = batched_lines()
     if not args.no_eprop:
         out["eprop"] = eprop_all()
     if not args.no_grow:

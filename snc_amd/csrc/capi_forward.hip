@@ -127,6 +127,8 @@ struct snc_hip_plan {
   uint32_t* F = nullptr;      // hip-pull: fired words [L][n], 4 or 8 bytes each
   float* hub_part = nullptr;  // hip-pull: hub chunk partials
   size_t F_words = 0;         // hip-pull: 32-bit words of F
+  uint16_t* fsum = nullptr;   // hip-pull: fired summary [L][sum_words]
+  size_t fsum_total = 0;
   PullArgs pa{};
   size_t n_blk_slots = 0;
   int halves = 1;
@@ -174,6 +176,7 @@ struct snc_hip_plan {
     dfree(inj_tab);
     dfree(F);
     dfree(hub_part);
+    dfree(fsum);
     dfree(inj_heavy);
     dfree(heavy_ids);
     dfree(is_heavy);
@@ -685,6 +688,10 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
       return fail(SNC_E_UNSUPPORTED, "hip-pull: graph or batch too large for 32-bit offsets");
     int prc = dalloc(&P->F, (size_t)g.L * n * word_u32);
     if (!prc) prc = dalloc(&P->hub_part, (size_t)std::max(1, pg.n_chunks) * kPullMaxSamples);
+    const int sum_words = (n + 15) / 16;
+    // an even count: the fill kernel clears whole 32-bit words
+    P->fsum_total = ((size_t)g.L * sum_words + 1) / 2 * 2;
+    if (!prc) prc = dalloc(&P->fsum, P->fsum_total);
     if (prc) return prc;
     P->F_words = (size_t)g.L * n * word_u32;
     PullArgs& pa = P->pa;
@@ -703,6 +710,9 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     pa.ldir = pg.ldir;
     pa.loff = pg.loff;
     pa.lcnt = pg.lcnt;
+    pa.hub_of = pg.hub_of;
+    pa.fsum = P->fsum;
+    pa.sum_words = sum_words;
     pa.halo_ptr = pg.halo_ptr;
     pa.halo_pk = pg.halo;
     pa.hub_ids = pg.hub_ids;
@@ -822,6 +832,7 @@ int reset_state(snc_hip_plan* P, bool raster) {
   fill(P->dirty, P->dirty_total);
   fill(P->bits, sizeof(uint32_t) * P->bits_total);
   if (P->F) fill(P->F, sizeof(uint32_t) * P->F_words);
+  if (P->fsum) fill(P->fsum, sizeof(uint16_t) * P->fsum_total);
   fill(P->blk_spikes, sizeof(unsigned long long) * B * P->n_blk_slots);
   fill(P->blk_events, sizeof(unsigned long long) * B * P->n_blk_slots);
   if (raster) fill(P->raster, sizeof(uint32_t) * B * (size_t)P->T * P->a.words);

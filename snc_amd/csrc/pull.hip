@@ -151,53 +151,24 @@ __global__ __launch_bounds__(kHubThreads) void k_pull_hub_chunks(PullArgs a, int
     w[u] = ld32(a.hub_w, o);
     if (k >= len) pk[u] = kHaloNone;
   }
+  // the fired summary bit of each entry's (origin step, pre) first: most are
+  // clear, and a clear bit needs no word
+  uint32_t sb[kHubRounds];
+#pragma unroll
+  for (int u = 0; u < kHubRounds; ++u) {
+    int s = tm - (int)(pk[u] & 0xFFu);
+    if (s < 0) s += a.L;
+    const uint32_t pre = (pk[u] >> 8) & 0x7FFFFFu;
+    const uint32_t q = pk[u] != kHaloNone ? (uint32_t)s * (uint32_t)a.sum_words + (pre >> 4) : 0u;
+    sb[u] = (uint32_t)ld32(a.fsum, q) >> (pre & 15u);
+  }
 #pragma unroll
   for (int u = 0; u < kHubRounds; ++u)
-    wd[u] = pk[u] != kHaloNone ? direct_word<Word>(a, pk[u], tm) : (Word)0;
+    wd[u] = pk[u] != kHaloNone && (sb[u] & 1u) ? direct_word<Word>(a, pk[u], tm) : (Word)0;
   float acc = 0.0f;
 #pragma unroll
   for (int u = 0; u < kHubRounds; ++u) acc = add_round<Word>(wd[u], w[u], acc, s_mask);
   a.hub_part[(size_t)c * kPullMaxSamples + lane] = acc;
-}
-
-// Hub h of step t (one wave, lane b = sample b): its chunks' partials in
-// chunk order, then integrate/fire.
-template <class Word>
-__device__ void pull_hub_step(const PullArgs& a, int t, int h) {
-  const int lane = lane_id();
-  const int tm = t % a.L;
-  const int hid = a.hub_ids[h];
-  const int c0 = a.hub_chunk_ptr[h], c1 = a.hub_chunk_ptr[h + 1];
-  bool f = false;
-  if (lane < a.B) {
-    // the partials in chunk order, 16 loads in flight at a time
-    float slot = 0.0f;
-    for (int cc = c0; cc < c1; cc += 16) {
-      float pt[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u)
-        pt[u] = cc + u < c1 ? a.hub_part[(size_t)(cc + u) * kPullMaxSamples + lane] : 0.0f;
-#pragma unroll
-      for (int u = 0; u < 16; ++u)
-        if (cc + u < c1) slot = __fadd_rn(slot, pt[u]);
-    }
-    const size_t sj = (size_t)hid * a.B + lane;
-    float v = a.v[sj];
-    int r = a.refr[sj];
-    const int r0 = r;
-    const float ext = a.inj_hub[(size_t)lane * a.hinj_stride + (size_t)t * a.n_hub + h];
-    f = lif_step(v, r, ext, slot, a.lif);
-    a.v[sj] = v;
-    if (r != r0) a.refr[sj] = (uint8_t)r;
-    if (f) {
-      a.nspk[sj] += 1;
-      if (a.raster)
-        atomicOr(a.raster + (size_t)lane * a.raster_stride + (size_t)t * a.words + (hid >> 5),
-                 1u << (hid & 31));
-    }
-  }
-  const uint64_t m = ballot(f);
-  if (lane == 0) static_cast<Word*>(a.F)[(size_t)tm * a.n + hid] = (Word)m;
 }
 
 // A light target's first two rounds of entries and its state.
@@ -302,22 +273,20 @@ k_pull_step(PullArgs a, int t) {
   }
   __syncthreads();
 
-  // Hubs: wave slot x * 16 + wave steps hubs in strides of the grid's waves.
-  for (int h = x * kPullWaves + wave; h < a.n_hub; h += (int)gridDim.x * kPullWaves)
-    pull_hub_step<Word>(a, t, h);
-
-  // Light targets.
+  // This wave's 16 targets: light ones sum their columns, hubs their chunk
+  // partials.
   const int jb = base + wave * kPullWaveTargets;
-  int my_c0 = 0, my_len = -1;
+  int my_c0 = 0, my_len = -1, my_hub = -1;
   if (lane < kPullWaveTargets && jb + lane < a.n) {
     my_c0 = a.loff[jb + lane];
     my_len = a.lcnt[jb + lane];
+    my_hub = a.hub_of[jb + lane];
   }
   const bool smp = lane < a.B;
   const uint32_t lane_b = (uint32_t)min(lane, a.B - 1);
   const uint32_t uB = (uint32_t)a.B;
   Word* Fw = static_cast<Word*>(a.F);
-  uint32_t rbits = 0;
+  uint32_t rbits = 0, sbits = 0;
   for (int i0 = 0; i0 < kPullWaveTargets; i0 += kBatch) {
     // Every load of the batch before any is used (no branches: lanes past a
     // column's end read the next entries of the stream, which is padded past
@@ -342,9 +311,9 @@ k_pull_step(PullArgs a, int t) {
       const int i = i0 + u;
       const int j = jb + i;
       const int len = p[u].len;
-      if (len < 0) continue;  // a hub (stepped above) or past the last neuron
+      if (len < 0) continue;  // a hub (stepped below) or past the last neuron
       const uint32_t o0 = (uint32_t)(p[u].c0 + lane);
-      float acc = 0.0f;
+      float acc = 0.0f, ext = 0.0f;
       acc = add_round<Word>(entry_word<Word, DIRECT>(a, s_halo, lane < len, p[u].h0, o0, tm),
                             p[u].w0, acc, s_mask);
       acc = add_round<Word>(
@@ -356,7 +325,6 @@ k_pull_step(PullArgs a, int t) {
             entry_word<Word, DIRECT>(a, s_halo, k0 + lane < len, ld32(a.lh, o), o, tm),
             ld32(a.lw, o), acc, s_mask);
       }
-      float ext = 0.0f;
       for (int q = s_head[j - base]; q >= 0; q = s_next[q]) {
         const int2 e = s_pool[q];
         if ((e.x >> 16) == lane) ext = __int_as_float(e.y);
@@ -374,10 +342,48 @@ k_pull_step(PullArgs a, int t) {
       const uint64_t m = ballot(f);
       if (lane == 0) st32(Fw, (uint32_t)tm * (uint32_t)a.n + (uint32_t)j, (Word)m);
       rbits |= (f ? 1u : 0u) << i;
+      sbits |= (m != 0 ? 1u : 0u) << i;
     }
   }
-  // Sample b's raster bits of the wave's 16 targets (half a raster word; hub
-  // bits come from the hub waves, so the word is or-ed).
+  // The wave's hubs: each one's chunk partials in chunk order (16 loads in
+  // flight at a time), then integrate/fire.
+  for (uint64_t hm = ballot(my_hub >= 0); hm; hm &= hm - 1) {
+    const int i = __builtin_ctzll(hm);
+    const int hub = __builtin_amdgcn_readlane(my_hub, i);
+    const int j = jb + i;
+    const int c0 = a.hub_chunk_ptr[hub], c1 = a.hub_chunk_ptr[hub + 1];
+    const uint32_t so = (uint32_t)j * uB + lane_b;
+    float v = ld32(a.v, so);
+    const int r0 = ld32(a.refr, so);
+    float acc = 0.0f;
+    for (int cc = c0; cc < c1; cc += 16) {
+      float pt[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        pt[q] = cc + q < c1 ? a.hub_part[(size_t)(cc + q) * kPullMaxSamples + lane] : 0.0f;
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (cc + q < c1) acc = __fadd_rn(acc, pt[q]);
+    }
+    bool f = false;
+    if (smp) {
+      const float ext = a.inj_hub[(size_t)lane * a.hinj_stride + (size_t)t * a.n_hub + hub];
+      int r = r0;
+      f = lif_step(v, r, ext, acc, a.lif);
+      st32(a.v, so, v);
+      if (r != r0) st32(a.refr, so, (uint8_t)r);
+      if (f) st32(a.nspk, so, ld32(a.nspk, so) + 1);
+    }
+    const uint64_t m = ballot(f);
+    if (lane == 0) st32(Fw, (uint32_t)tm * (uint32_t)a.n + (uint32_t)j, (Word)m);
+    rbits |= (f ? 1u : 0u) << i;
+    sbits |= (m != 0 ? 1u : 0u) << i;
+  }
+  // The fired summary half-word of the wave's 16 targets.
+  if (lane == 0 && jb < a.n)
+    st32(a.fsum, (uint32_t)tm * (uint32_t)a.sum_words + (uint32_t)(jb >> 4), (uint16_t)sbits);
+  // Sample b's raster bits of the wave's 16 targets (half a raster word, so
+  // the word is or-ed).
   if (a.raster && smp && rbits && jb < a.n)
     atomicOr(a.raster + (size_t)lane * a.raster_stride + (size_t)t * a.words + (jb >> 5),
              rbits << (jb & 31));
@@ -397,6 +403,7 @@ void PullGraph::release() {
   dfree(ldir);
   dfree(loff);
   dfree(lcnt);
+  dfree(hub_of);
   dfree(halo_ptr);
   dfree(halo);
   dfree(hub_ids);
@@ -438,7 +445,8 @@ int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
     }
 
   const int R = std::max(1, (n + kPullBlock - 1) / kPullBlock);
-  std::vector<int32_t> lcnt((size_t)n, 0), hubs, chunk_hub, chunk_pos, chunk_len;
+  std::vector<int32_t> lcnt((size_t)n, 0), hub_of((size_t)n, -1), hubs, chunk_hub, chunk_pos,
+      chunk_len;
   std::vector<int32_t> hub_chunk_ptr(1, 0);
   std::vector<uint32_t> hub_pk;
   std::vector<float> hub_w;
@@ -451,6 +459,7 @@ int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
     }
     lcnt[j] = -1;
     const int h = (int)hubs.size();
+    hub_of[j] = h;
     hubs.push_back(j);
     // the hub's entries by (delay, pre), ties in column order
     ord.clear();
@@ -529,6 +538,7 @@ int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
   if (!rc) rc = dupload(&pg.lh, lh);
   if (!rc) rc = dupload(&pg.ldir, ldir);
   if (!rc) rc = dupload(&pg.loff, loff);
+  if (!rc) rc = dupload(&pg.hub_of, hub_of);
   if (!rc) rc = dupload(&pg.lcnt, lcnt);
   if (!rc) rc = dupload(&pg.halo_ptr, halo_ptr);
   if (!rc) rc = dupload(&pg.halo, halo.empty() ? std::vector<uint32_t>(1, 0u) : halo);

@@ -22,12 +22,15 @@
 //
 // Hub targets (in-degree above the cap) keep their entries sorted by
 // (delay, pre), so consecutive gathers of a chunk share F's cache lines, and
-// are cut into chunks of at most kPullHubChunk entries. A chunk is one wave's work in a launch of its own
-// before the step launch: it gathers the chunk's fired words from F, sums
-// them per sample in column order and stores the partials; in the step
-// launch one wave per hub adds its partials in chunk order and steps the hub.
-// Hub sums are deterministic but associated differently from the CPU's (the
-// atomic-class tolerance).
+// are cut into chunks of at most kPullHubChunk entries. A chunk is one wave's
+// work in a launch of its own before the step launch: it gathers the chunk's
+// fired words from F, sums them per sample in column order and stores the
+// partials; in the step launch the wave that steps the hub's id range adds
+// its partials in chunk order and steps the hub. A chunk gathers only the
+// words whose bit is set in the fired summary: one bit per neuron and step,
+// set when the neuron fired in any sample, written 16 neurons to a half-word
+// by the wave that steps them (no atomics). Hub sums are deterministic but
+// associated differently from the CPU's (the atomic-class tolerance).
 //
 // State is sample-minor, [n][B]: a wave's 64 lanes touch one target's samples
 // in one 256-byte run. Spike and event counts are not accumulated per step:
@@ -74,6 +77,7 @@ struct PullGraph {
   uint32_t* ldir = nullptr;       // (pre << 8) | delay of direct entries (else unused)
   int32_t* loff = nullptr;        // [n] first light entry of a target
   int32_t* lcnt = nullptr;        // [n] light entries of a target, -1 for hubs
+  int32_t* hub_of = nullptr;      // [n] hub index of a target, -1 for light targets
   int64_t* halo_ptr = nullptr;    // [n_blocks + 1]
   uint32_t* halo = nullptr;       // (pre << 8) | delay, per block by (delay, pre)
   int32_t* hub_ids = nullptr;     // [n_hub]
@@ -101,6 +105,7 @@ struct PullArgs {
   const uint32_t* ldir;
   const int32_t* loff;
   const int32_t* lcnt;
+  const int32_t* hub_of;
   const int64_t* halo_ptr;
   const uint32_t* halo_pk;
   const int32_t* hub_ids;
@@ -119,6 +124,8 @@ struct PullArgs {
   int64_t inj_stride, hinj_stride;
   // state
   void* F;            // [L][n] words
+  uint16_t* fsum;     // [L][sum_words] fired summary: bit i of word q = neuron 16 q + i
+  int sum_words;
   float* hub_part;    // [n_chunks][kPullMaxSamples]
   float* v;           // [n][B]
   uint8_t* refr;      // [n][B] refractory steps left (the refractory period is at most 255)
