@@ -127,8 +127,8 @@ def measured_traffic(kernel_symbol, label=None):
     return None
 
 
-PULL_KERNELS = ("snc_amd::(anonymous namespace)::k_pull_step<{w}, false>",
-                "snc_amd::(anonymous namespace)::k_pull_hub_chunks<{w}>")
+PULL_KERNELS = ("void snc_amd::(anonymous namespace)::k_pull_step<{w}, false>(snc_amd::PullArgs, int)",
+                "void snc_amd::(anonymous namespace)::k_pull_hub_chunks<{w}>(snc_amd::PullArgs, int)")
 
 
 def model_bytes(neuron_steps, events, fired):

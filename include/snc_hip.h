@@ -169,7 +169,9 @@ int snc_hip_plan_get_config(const snc_hip_plan* p, snc_plan_config* out);
  * a trial window of the first trial_steps steps of the input that triggered
  * the search, and the one chosen (the fastest). exact_only selects the
  * SNC_VARIANT_AUTO_EXACT search, batched the search of batched plans (kept
- * apart from single-sample plans). Fails when no such search has run yet. */
+ * apart from single-sample plans; the latest one, as a batched search is
+ * redone when a plan has another sample count). Fails when no such search
+ * has run yet. */
 #define SNC_TUNE_MAX_CANDIDATES 24
 #define SNC_TUNE_ERROR_LEN 120
 typedef struct snc_tune_report {

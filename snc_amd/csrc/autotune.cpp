@@ -17,7 +17,9 @@
 //     (delivery stress: 29 vs 7.3 us per step with float atomics, 55 vs 12.1
 //     with buckets), so it runs only when a plan asks for it explicitly.
 //   * hip-pull for batched plans of up to 64 samples, at the 99th-percentile
-//     in-degree cap and at twice it.
+//     in-degree cap with 256- and 128-target blocks (the larger won with 64
+//     samples on the delivery-stress graph, the smaller with 8), and at
+//     twice the cap.
 #include "autotune.hpp"
 #include "pull.hpp"
 
@@ -82,6 +84,7 @@ std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_o
   if (!exact_only && num_samples > 1 && num_samples <= kPullMaxSamples) {
     const int cap = pull_default_cap(g);
     add(SNC_VARIANT_PULL, 256, cap, 0);
+    add(SNC_VARIANT_PULL, 128, cap, 0);
     if (std::min(1024, 2 * cap) != cap) add(SNC_VARIANT_PULL, 256, std::min(1024, 2 * cap), 0);
   }
   return out;

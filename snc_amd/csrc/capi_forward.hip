@@ -412,8 +412,8 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     return fail(SNC_E_INVALID, "unknown variant");
   const bool pull = variant == SNC_VARIANT_PULL;
   if (pull) {
-    if (cfg->block_neurons != kPullBlock)
-      return fail(SNC_E_INVALID, "hip-pull runs 256-neuron blocks");
+    if (!pull_block_ok(cfg->block_neurons))
+      return fail(SNC_E_INVALID, "hip-pull blocks hold 64 to 256 neurons, a multiple of 16");
     if (cfg->heavy_threshold < 1 || cfg->heavy_threshold > kPullMaxCap)
       return fail(SNC_E_INVALID, "hip-pull heavy_threshold (light in-degree cap) must be in [1, 1024]");
     if (cfg->persistent) return fail(SNC_E_UNSUPPORTED, "hip-pull has no persistent kernel");
@@ -667,11 +667,11 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     // while it lives (plans of different caps may be alive together).
     PullGraph* pgp = nullptr;
     for (PullGraph* q : G->pulls)
-      if (q->cap == cfg->heavy_threshold) pgp = q;
+      if (q->cap == cfg->heavy_threshold && q->blk == nb) pgp = q;
     if (!pgp) {
       std::unique_ptr<PullGraph> fresh(new PullGraph());
       AllocAccount gacct(&fresh->dev_bytes);
-      const int brc = pull_prepare(g, cfg->heavy_threshold, *fresh);
+      const int brc = pull_prepare(g, cfg->heavy_threshold, nb, *fresh);
       if (brc) return brc;
       pgp = fresh.release();
       G->pulls.push_back(pgp);
@@ -679,7 +679,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     PullGraph& pg = *pgp;
     if (pg.hub_host != P->heavy_host) return fail(SNC_E_UNSUPPORTED, "hip-pull hub sets differ");
     if (pg.n_blocks != R) return fail(SNC_E_INVALID, "hip-pull block count differs from the plan's");
-    int lrc = pull_set_lds(B, pg.halo_words, (int)pool);
+    int lrc = pull_set_lds(B, pg.halo_words, (int)pool, nb);
     if (lrc) return lrc;
     const size_t word_u32 = (size_t)pull_word_bytes(B) / 4;
     // the step kernels index every array with 32-bit byte offsets
@@ -701,6 +701,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     pa.T = num_steps;
     pa.words = a.words;
     pa.n_blocks = R;
+    pa.blk = nb;
     pa.pool = (int)pool;
     pa.halo = pg.halo_words;
     pa.direct = pg.direct ? 1 : 0;
@@ -1119,14 +1120,15 @@ int snc_hip_forward(const snc_graph_view* gv, const snc_event* events, int64_t n
 }  // extern "C"
 
 // The autotuner's search (cached per graph, separately for single-sample
-// and batched plans): every candidate runs the first trial_steps steps of
-// this input twice (the first run captures and instantiates its graph); the
-// device time of the second run decides.
+// and batched plans; a batched search is redone for another sample count,
+// since the fastest strategy changes with it): every candidate runs the
+// first trial_steps steps of this input twice (the first run captures and
+// instantiates its graph); the device time of the second run decides.
 static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_events,
                            int num_samples, int num_steps, const snc_lif* lif, bool exact_only,
                            snc_plan_config* out) {
   const int k = (exact_only ? 1 : 0) + (num_samples > 1 ? 2 : 0);
-  if (G->tuned[k]) {
+  if (G->tuned[k] && G->tuned_samples[k] == num_samples) {
     *out = G->tune[k].cand[G->tune[k].chosen];
     return SNC_OK;
   }
@@ -1169,6 +1171,7 @@ static int autotune_search(snc_hip_graph* G, const snc_event* events, int64_t n_
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   G->tune[k] = rep;
   G->tuned[k] = true;
+  G->tuned_samples[k] = num_samples;
   *out = rep.cand[best];
   return SNC_OK;
 }

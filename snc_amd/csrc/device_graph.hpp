@@ -70,4 +70,5 @@ struct snc_hip_graph {
   // Autotuner results: [exact_only + 2 * batched].
   bool tuned[4] = {false, false, false, false};
   snc_tune_report tune[4] = {};
+  int tuned_samples[4] = {0, 0, 0, 0};  // the sample count each search ran with
 };

@@ -14,11 +14,10 @@ namespace snc_amd {
 
 namespace {
 
-constexpr int kPullWaves = kPullThreads / kWave;
 constexpr int kHubRounds = kPullHubChunk / kWave;
 constexpr int kHubThreads = 256;  // chunk waves per hub-chunk block: 4
 constexpr int kBatch = 4;         // light targets whose loads go out together
-constexpr int kHaloPer = (kPullHaloCap + kPullThreads - 1) / kPullThreads;
+constexpr int kHaloRound = 8;     // halo words a thread loads per round
 static_assert(kPullHubChunk % kWave == 0, "hub chunks are whole rounds of a wave");
 static_assert(kPullWaveTargets % kBatch == 0, "whole batches per wave");
 
@@ -122,10 +121,14 @@ __device__ __forceinline__ float add_round(Word word, float w, float acc,
 
 // LDS of a step block: the halo's fired words, every wave's 64 sample masks,
 // then the injection pool, its per-entry links and per-target list heads.
-constexpr size_t kMaskBytes = (size_t)kPullWaves * kWave * 8;
-__host__ __device__ inline size_t lds_bytes(int word_bytes, int halo, int pool) {
-  return ((size_t)(halo + 1) * word_bytes + 15) / 16 * 16 + kMaskBytes + (size_t)pool * 12 +
-         (size_t)kPullBlock * 4 + 16;
+__host__ __device__ inline size_t halo_bytes(int word_bytes, int halo) {
+  return ((size_t)(halo + 1) * word_bytes + 15) / 16 * 16;
+}
+__host__ __device__ inline size_t mask_bytes(int blk) {
+  return (size_t)(blk / kPullWaveTargets) * kWave * 8;
+}
+__host__ __device__ inline size_t lds_bytes(int word_bytes, int halo, int pool, int blk) {
+  return halo_bytes(word_bytes, halo) + mask_bytes(blk) + (size_t)pool * 12 + (size_t)blk * 4 + 16;
 }
 
 // Hub chunks of step t: wave c sums chunk c per sample, in column order.
@@ -178,28 +181,30 @@ struct Tgt {
   int r, len, c0;
 };
 
-// One step t: hub integrate/fire (partials from the chunk launch), then the
-// light targets, 16 per wave in batches of kBatch whose loads go out
-// together. Eight waves per SIMD: two blocks share a CU, halos included.
+// One step t: the block's 16-target waves, in batches of kBatch targets
+// whose loads go out together, then the hubs among each wave's targets
+// (partials from the chunk launch). Eight waves per SIMD: two blocks share a
+// CU, halos included.
 template <class Word, bool DIRECT>
-__global__ __launch_bounds__(kPullThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+__global__ __launch_bounds__(kPullMaxThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 k_pull_step(PullArgs a, int t) {
   extern __shared__ __align__(16) unsigned char s_dyn[];
   Word* s_halo = reinterpret_cast<Word*>(s_dyn);
   unsigned long long* s_masks =
-      reinterpret_cast<unsigned long long*>(s_dyn + ((size_t)(a.halo + 1) * sizeof(Word) + 15) / 16 * 16);
-  int2* s_pool = reinterpret_cast<int2*>(reinterpret_cast<unsigned char*>(s_masks) + kMaskBytes);
+      reinterpret_cast<unsigned long long*>(s_dyn + halo_bytes((int)sizeof(Word), a.halo));
+  int2* s_pool = reinterpret_cast<int2*>(reinterpret_cast<unsigned char*>(s_masks) + mask_bytes(a.blk));
   int* s_next = reinterpret_cast<int*>(s_pool + a.pool);
-  int* s_head = s_next + a.pool;  // [kPullBlock]
-  int* s_pool_n = s_head + kPullBlock;
+  int* s_head = s_next + a.pool;  // [a.blk]
+  int* s_pool_n = s_head + a.blk;
 
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  const int nthr = (int)blockDim.x, waves = nthr >> 6;
   // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous
   // run of target blocks so the halos it stages overlap in its own L2.
   const int R = a.n_blocks, x = blockIdx.x;
   const int q8 = R >> 3, r8 = R & 7, xcd = x & 7;
   const int blk = xcd * q8 + min(xcd, r8) + (x >> 3);
-  const int base = blk * kPullBlock;
+  const int base = blk * a.blk;
   const int tm = t % a.L;
 
   // Halo: the fired word of every (pre, delay) the block's light columns
@@ -207,34 +212,34 @@ k_pull_step(PullArgs a, int t) {
   {
     const int64_t h0 = a.halo_ptr[blk];
     const int nh = (int)(a.halo_ptr[blk + 1] - h0);
-    if (nh > 0) {
-      uint32_t pk[kHaloPer];
-      Word wd[kHaloPer];
+    for (int k0 = 0; k0 < nh; k0 += kHaloRound * nthr) {
+      uint32_t pk[kHaloRound];
+      Word wd[kHaloRound];
 #pragma unroll
-      for (int u = 0; u < kHaloPer; ++u) pk[u] = a.halo_pk[h0 + min(tid + u * kPullThreads, nh - 1)];
+      for (int u = 0; u < kHaloRound; ++u) pk[u] = a.halo_pk[h0 + min(k0 + tid + u * nthr, nh - 1)];
 #pragma unroll
-      for (int u = 0; u < kHaloPer; ++u) wd[u] = direct_word<Word>(a, pk[u], tm);
+      for (int u = 0; u < kHaloRound; ++u) wd[u] = direct_word<Word>(a, pk[u], tm);
 #pragma unroll
-      for (int u = 0; u < kHaloPer; ++u)
-        if (tid + u * kPullThreads < nh) s_halo[tid + u * kPullThreads] = wd[u];
+      for (int u = 0; u < kHaloRound; ++u)
+        if (k0 + tid + u * nthr < nh) s_halo[k0 + tid + u * nthr] = wd[u];
     }
   }
   unsigned long long* s_mask = s_masks + wave * kWave;
   s_mask[lane] = 0ull;
   if (tid == 0) s_halo[a.halo] = (Word)0;  // the zero slot
-  if (tid < kPullBlock) s_head[tid] = -1;
+  if (tid < a.blk) s_head[tid] = -1;
   if (tid == 0) *s_pool_n = 0;
   __syncthreads();
   // Injections of step t of every sample into the pool, linked per target.
-  // Wave w stages samples w, w + 16, ...: their list bounds load together,
-  // then their first 64 entries each.
-  {
-    constexpr int kPer = kPullMaxSamples / kPullWaves;
+  // Wave w stages samples w, w + waves, ...: four samples' list bounds load
+  // together, then their first 64 entries each.
+  for (int b0 = wave; b0 < a.B; b0 += 4 * waves) {
+    constexpr int kPer = 4;
     int k0[kPer], cnt[kPer], nid[kPer];
     float amp[kPer];
 #pragma unroll
     for (int s = 0; s < kPer; ++s) {
-      const int b = wave + s * kPullWaves;
+      const int b = b0 + s * waves;
       k0[s] = 0;
       cnt[s] = 0;
       if (b < a.B) {
@@ -256,7 +261,7 @@ k_pull_step(PullArgs a, int t) {
 #pragma unroll
     for (int s = 0; s < kPer; ++s) {
       if (cnt[s] <= 0) continue;
-      const int b = wave + s * kPullWaves;
+      const int b = b0 + s * waves;
       int at = 0;
       if (lane == 0) at = atomicAdd(s_pool_n, cnt[s]);
       at = __shfl(at, 0, kWave);
@@ -420,8 +425,13 @@ void PullGraph::release() {
   direct = false;
 }
 
-int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
-  if (pg.cap == cap) return SNC_OK;
+bool pull_block_ok(int blk) {
+  return blk >= kPullMinBlock && blk <= kPullMaxBlock && blk % kPullWaveTargets == 0;
+}
+
+int pull_prepare(const DevGraph& g, int cap, int blk, PullGraph& pg) {
+  if (pg.cap == cap && pg.blk == blk) return SNC_OK;
+  if (!pull_block_ok(blk)) return fail(SNC_E_INVALID, "hip-pull block size");
   pg.release();
   const int n = g.n;
   if (n >= (1 << 23)) return fail(SNC_E_UNSUPPORTED, "hip-pull packs pre ids in 23 bits");
@@ -444,7 +454,7 @@ int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
       cpk[(size_t)p] = ((uint32_t)i << 8) | cdel[(size_t)p];
     }
 
-  const int R = std::max(1, (n + kPullBlock - 1) / kPullBlock);
+  const int R = std::max(1, (n + blk - 1) / blk);
   std::vector<int32_t> lcnt((size_t)n, 0), hub_of((size_t)n, -1), hubs, chunk_hub, chunk_pos,
       chunk_len;
   std::vector<int32_t> hub_chunk_ptr(1, 0);
@@ -500,7 +510,7 @@ int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
   int64_t direct = 0;
   auto key_of = [](uint32_t pk) { return ((pk & 0xFFu) << 24) | (pk >> 8); };  // (delay, pre)
   for (int q = 0; q < R; ++q) {
-    const int j0 = q * kPullBlock, j1 = std::min(n, j0 + kPullBlock);
+    const int j0 = q * blk, j1 = std::min(n, j0 + blk);
     keys.clear();
     for (int j = j0; j < j1; ++j)
       for (int k = 0; k < lcnt[j]; ++k) keys.push_back(key_of(cpk[(size_t)ccol[j] + k]));
@@ -554,11 +564,12 @@ int pull_prepare(const DevGraph& g, int cap, PullGraph& pg) {
     return rc;
   }
   pg.cap = cap;
+  pg.blk = blk;
   return SNC_OK;
 }
 
-size_t pull_lds_bytes(int B, int halo, int pool) {
-  return lds_bytes(pull_word_bytes(B), halo, pool);
+size_t pull_lds_bytes(int B, int halo, int pool, int blk) {
+  return lds_bytes(pull_word_bytes(B), halo, pool, blk);
 }
 
 int pull_launch_step(const PullArgs& a, int t, hipStream_t st) {
@@ -572,8 +583,8 @@ int pull_launch_step(const PullArgs& a, int t, hipStream_t st) {
     const int rc = hip_last("pull hub chunk launch failed");
     if (rc) return rc;
   }
-  const size_t lds = pull_lds_bytes(a.B, a.halo, a.pool);
-  const dim3 grid(a.n_blocks), block(kPullThreads);
+  const size_t lds = pull_lds_bytes(a.B, a.halo, a.pool, a.blk);
+  const dim3 grid(a.n_blocks), block(a.blk / kPullWaveTargets * kWave);
   if (wide && a.direct)
     hipLaunchKernelGGL((k_pull_step<uint64_t, true>), grid, block, lds, st, a, t);
   else if (wide)
@@ -585,8 +596,8 @@ int pull_launch_step(const PullArgs& a, int t, hipStream_t st) {
   return hip_last("pull step launch failed");
 }
 
-int pull_set_lds(int B, int halo, int pool) {
-  const size_t bytes = pull_lds_bytes(B, halo, pool);
+int pull_set_lds(int B, int halo, int pool, int blk) {
+  const size_t bytes = pull_lds_bytes(B, halo, pool, blk);
   if (bytes > kPullMaxLds) return fail(SNC_E_UNSUPPORTED, "hip-pull: halo and injection pool exceed LDS");
   // The ceiling, not this plan's size: plans of other sizes share the kernels.
   set_step_lds<uint32_t, false>();

@@ -48,9 +48,10 @@
 
 namespace snc_amd {
 
-constexpr int kPullBlock = 256;        // targets per light block
-constexpr int kPullThreads = 1024;     // 16 waves, 16 targets each
-constexpr int kPullWaveTargets = kPullBlock / (kPullThreads / kWave);
+constexpr int kPullWaveTargets = 16;   // targets one wave steps
+constexpr int kPullMaxBlock = 256;     // targets per block (a multiple of 16, at least 64)
+constexpr int kPullMinBlock = 64;
+constexpr int kPullMaxThreads = kPullMaxBlock / kPullWaveTargets * kWave;
 constexpr int kPullMaxSamples = 64;    // bits of the widest F word
 constexpr int kPullHubChunk = 512;     // hub column entries per chunk wave
 constexpr int kPullHaloCap = 9216;     // fired words a block stages in LDS
@@ -67,6 +68,7 @@ constexpr int kPullStreamPad = 128;    // entries past the light stream's end (o
 // Per-graph, per-cap arrays (device).
 struct PullGraph {
   int cap = -1;
+  int blk = 0;                    // targets per block
   int n_blocks = 0;
   int n_hub = 0, n_chunks = 0;
   int halo_words = 0;             // largest staged halo (LDS words)
@@ -95,6 +97,7 @@ struct PullGraph {
 
 struct PullArgs {
   int n, L, B, T, words, n_blocks;
+  int blk;    // targets per block (16 per wave)
   int pool;   // injection entries a block stages per step
   int halo;   // LDS halo words of this launch
   int direct; // some light entry reads F directly
@@ -134,13 +137,15 @@ struct PullArgs {
   int64_t raster_stride;
 };
 
-// Builds the halo-indexed columns of `g` at light cap `cap` (nothing if pg
-// already holds that cap).
-int pull_prepare(const DevGraph& g, int cap, PullGraph& pg);
+// Builds the halo-indexed columns of `g` at light cap `cap` for blocks of
+// `blk` targets (nothing if pg already holds them).
+int pull_prepare(const DevGraph& g, int cap, int blk, PullGraph& pg);
 // LDS bytes of a step launch.
-size_t pull_lds_bytes(int B, int halo, int pool);
+size_t pull_lds_bytes(int B, int halo, int pool, int blk);
 // Allows the step kernel of B samples the LDS it needs.
-int pull_set_lds(int B, int halo, int pool);
+int pull_set_lds(int B, int halo, int pool, int blk);
+// Whether blk is a valid block size.
+bool pull_block_ok(int blk);
 // One step: the hub-chunk launch (if the graph has hubs), then the step launch.
 int pull_launch_step(const PullArgs& a, int t, hipStream_t st);
 // Bytes of one F word for B samples.

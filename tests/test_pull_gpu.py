@@ -32,14 +32,15 @@ def _drives(n, T, B, seed, p0=0.01, dp=0.003):
     return evs, allev
 
 
-def _pull(hg, allev, T, lif, B, cap):
-    print(f"hip-pull plan: B={B} T={T} cap={cap}", flush=True)
-    return ForwardPlan(hg, allev, T, lif, config=PlanConfig(Backend.HipPull, 256, cap, False),
+def _pull(hg, allev, T, lif, B, cap, blk=256):
+    print(f"hip-pull plan: B={B} T={T} cap={cap} block={blk}", flush=True)
+    return ForwardPlan(hg, allev, T, lif, config=PlanConfig(Backend.HipPull, blk, cap, False),
                        num_samples=B)
 
 
-@pytest.mark.parametrize("B", [1, 7, 32, 33, 64])
-def test_pull_without_hubs_is_bit_exact(B):
+@pytest.mark.parametrize("B,blk", [(1, 256), (7, 256), (32, 256), (33, 256), (64, 256),
+                                   (33, 208), (20, 64)])
+def test_pull_without_hubs_is_bit_exact(B, blk):
     g = G.sheet_graph([400, 300, 20], 24, recurrent_fan_in=16, radius=3.0, sheet=20.0,
                       w_std=0.35, max_delay=9, seed=3)
     cap = int(g.in_degree().max())
@@ -48,8 +49,8 @@ def test_pull_without_hubs_is_bit_exact(B):
     evs, allev = _drives(g.num_neurons, T, B, seed=40, p0=0.05, dp=0.01)
     hg = HipGraph(g)
     m0 = hg.memory()
-    plan = _pull(hg, allev, T, lif, B, cap)
-    assert hg.memory() > m0  # the sliced columns count toward the graph
+    plan = _pull(hg, allev, T, lif, B, cap, blk)
+    assert hg.memory() > m0  # the light stream and halos count toward the graph
     p0 = plan.memory()
     r = plan.run(capture_raster=True, capture_v=True)
     assert plan.memory() >= p0 + B * T * ((g.num_neurons + 31) // 32) * 4  # the raster
@@ -96,9 +97,10 @@ def test_pull_rejects_what_it_cannot_hold():
     _, allev = _drives(g.num_neurons, 10, 65, seed=1)
     with pytest.raises(L.HipError, match="at most 64"):
         _pull(hg, allev, 10, lif, 65, 64)
-    with pytest.raises(L.HipError, match="256-neuron"):
-        ForwardPlan(hg, allev[allev["sample_id"] < 2], 10, lif,
-                    config=PlanConfig(Backend.HipPull, 512, 64, False), num_samples=2)
+    for blk in (512, 48, 200):
+        with pytest.raises(L.HipError, match="64 to 256 neurons"):
+            ForwardPlan(hg, allev[allev["sample_id"] < 2], 10, lif,
+                        config=PlanConfig(Backend.HipPull, blk, 64, False), num_samples=2)
 
 
 def test_pull_dense_input_fills_the_largest_pool():
@@ -159,3 +161,26 @@ def test_pull_halo_overflow_reads_fired_words_directly(B):
         assert ref["spikes"] > 0
         assert np.array_equal(r.raster[b], ref["raster"]), b
         assert np.array_equal(r.final_v[b].view(np.uint32), ref["final_v"].view(np.uint32))
+
+
+def test_batched_autotuner_searches_again_for_another_sample_count():
+    """The batched search is cached per sample count: a second plan with the
+    same count reuses it, a plan with another count searches again (the
+    fastest strategy depends on how many samples share a step)."""
+    g = G.hub_sheet_graph(60, 50, 40, hub_fraction=0.02, hub_min=40, hub_max=1500,
+                          w_std=0.06, max_delay=20, seed=9)
+    lif = K.lif_tau20()
+    T = 40
+    hg = HipGraph(g)
+    _, ev4 = _drives(g.num_neurons, T, 4, seed=8)
+    _, ev16 = _drives(g.num_neurons, T, 16, seed=8)
+    ForwardPlan(hg, ev4, T, lif, Backend.HipAuto, num_samples=4).close()
+    r1 = hg.tune_report(batched=True)
+    ForwardPlan(hg, ev4, T, lif, Backend.HipAuto, num_samples=4).close()
+    r2 = hg.tune_report(batched=True)
+    assert [m for _, m in r2["candidates"]] == [m for _, m in r1["candidates"]]
+    assert r2["search_ms"] == r1["search_ms"]
+    ForwardPlan(hg, ev16, T, lif, Backend.HipAuto, num_samples=16).close()
+    r3 = hg.tune_report(batched=True)
+    assert r3["search_ms"] != r1["search_ms"]
+    assert [m for _, m in r3["candidates"]] != [m for _, m in r1["candidates"]]
