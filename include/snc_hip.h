@@ -48,10 +48,11 @@ extern "C" {
 #define SNC_VARIANT_AUTO 8   /* hip-auto: autotuner picks per graph           */
 #define SNC_VARIANT_AUTO_EXACT 9 /* hip-auto-exact: autotuner over the bit-exact
                                     strategies (bucket, sort) only            */
-#define SNC_VARIANT_PULL 10  /* hip-pull: batched gather over sliced incoming
-                                columns, all samples of a plan in one pass;
-                                light targets bit-exact, hub targets within
-                                the atomic tolerance (deterministic) */
+#define SNC_VARIANT_PULL 10  /* hip-pull: batched gather over the incoming
+                                columns, all samples of a plan (up to 64) in
+                                one pass; light targets bit-exact, hub
+                                targets within the atomic tolerance
+                                (deterministic) */
 
 /* ---- graph roles (GraphRole, snn_graph.hpp:29) ---------------------------- */
 #define SNC_ROLE_INTERNAL 0
@@ -235,8 +236,8 @@ int snc_hip_plan_profile(snc_hip_plan* p, double* kernel_ms_total, int64_t* laun
 /* Wait until all work the library queued on the device has finished. */
 int snc_hip_synchronize(void);
 
-/* Device memory in bytes held by a graph (its arrays, plus hip-pull's sliced
- * columns once a hip-pull plan built them) and by a plan (state, schedule,
+/* Device memory in bytes held by a graph (its arrays, plus hip-pull's light
+ * columns and halos once a hip-pull plan built them) and by a plan (state, schedule,
  * scratch, the raster once captured). No counterpart in the reference: the
  * memory fields its observability plan asks for (docs/new-plan.md:727-741). */
 int snc_hip_graph_memory(const snc_hip_graph* g, int64_t* bytes);

@@ -63,7 +63,7 @@ void pull_free(PullGraph* pg);
 
 struct snc_hip_graph {
   snc_amd::DevGraph g;
-  std::vector<snc_amd::PullGraph*> pulls;  // hip-pull's sliced columns, one per light cap
+  std::vector<snc_amd::PullGraph*> pulls;  // hip-pull light columns and halos, per (light cap, block size)
   ~snc_hip_graph() {
     for (snc_amd::PullGraph* p : pulls) snc_amd::pull_free(p);
   }
