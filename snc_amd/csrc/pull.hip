@@ -92,27 +92,38 @@ __device__ __forceinline__ Word entry_word(const PullArgs& a, const Word* s_halo
 // Adds one round of 64 column entries (lane k holds entry k's fired word and
 // value) to every sample's sum, in column order. The fired bits are
 // transposed through the wave's 64 LDS masks (s_mask, all zero on entry and
-// on return): entry lane k ors bit k into the mask of every sample its word
-// has, then sample lane b walks its mask in entry order and adds those
-// entries' values (+0 on the rounds where it has none left). The work
-// follows the set bits, not the entries with any bit set.
+// on return; sample b's mask is two 32-bit words, entries 0-31 and 32-63):
+// entry lane k ors its bit into the mask of every sample its word has, then
+// sample lane b walks its mask in entry order and adds those entries'
+// values (+0 on the rounds where it has none left). The work follows the set
+// bits, not the entries with any bit set, and every bit loop runs on 32-bit
+// halves (one ffbl and one and per bit, no 64-bit emulation).
 template <class Word>
 __device__ __forceinline__ float add_round(Word word, float w, float acc,
                                            unsigned long long* s_mask) {
-  uint64_t bits = (uint64_t)word;
-  if (ballot(bits != 0) == 0) return acc;
+  if (ballot(word != (Word)0) == 0) return acc;
   const int lane = lane_id();
-  const unsigned long long me = 1ull << lane;
-  while (bits) {
-    atomicOr(&s_mask[__builtin_ctzll(bits)], me);
-    bits &= bits - 1;
-  }
+  uint32_t* m32 = reinterpret_cast<uint32_t*>(s_mask);
+  const uint32_t me = 1u << (lane & 31);
+  uint32_t* mine = m32 + (lane >> 5);  // this entry's half of every sample's mask
+  for (uint32_t x = (uint32_t)word; x; x &= x - 1) atomicOr(mine + 2 * __builtin_ctz(x), me);
+  if constexpr (sizeof(Word) == 8)
+    for (uint32_t x = (uint32_t)(word >> 32); x; x &= x - 1)
+      atomicOr(mine + 2 * (32 + __builtin_ctz(x)), me);
   wave_lds_sync();
-  uint64_t m = s_mask[lane];
+  const unsigned long long both = s_mask[lane];
   s_mask[lane] = 0ull;
-  while (ballot(m != 0)) {  // every lane active: lanes with nothing left add +0
-    const int k = m ? __builtin_ctzll(m) : 0;
-    const float vk = __shfl(w, k, kWave);
+  uint32_t m = (uint32_t)both;  // entries 0-31, then 32-63: column order
+  while (ballot(m != 0)) {      // every lane active: lanes with nothing left add +0
+    const int k = m ? __builtin_ctz(m) : 0;
+    const float vk = __int_as_float(__builtin_amdgcn_ds_bpermute(k << 2, __float_as_int(w)));
+    acc = __fadd_rn(acc, m ? vk : 0.0f);
+    m &= m - 1;
+  }
+  m = (uint32_t)(both >> 32);
+  while (ballot(m != 0)) {
+    const int k = m ? 32 + __builtin_ctz(m) : 0;
+    const float vk = __int_as_float(__builtin_amdgcn_ds_bpermute(k << 2, __float_as_int(w)));
     acc = __fadd_rn(acc, m ? vk : 0.0f);
     m &= m - 1;
   }
