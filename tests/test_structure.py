@@ -234,3 +234,56 @@ def test_negative_grow_init_starts_grown_synapses_at_the_survivors_median():
     assert new_pair.any() and np.all(w1[new_pair] == med)
     # every grown synapse (a regrown pruned pair included) carries the median
     assert np.count_nonzero(w1 == med) >= rep["grown"]
+
+
+def test_sheet_heuristic_grow_equals_brute_force_rule():
+    """The sheet connectome's heuristic step against a brute-force
+    restatement of its rule: prune the 30 smallest (|w|, CSR position); each
+    freed post, in the pruned synapses' CSR order, regrows from the open
+    member of its window (previous layer, and its own when hidden, within the
+    locality distance, self excluded; layer then id order) that fired most,
+    the first such on ties. Enough window candidates are left that no post
+    goes without, so the tournament fill never draws; the resulting edges and
+    the grown synapses' start weight must match exactly."""
+    layers = [81, 49, 9]
+    con = Connectome.sheet_recurrent(layers, fan_in=10, rec_fan_in=5, radius=0.2, seed=7)
+    x, y, lay = _sheet_xy(layers)
+    g0 = con.to_graph()
+    n, m = g0.num_neurons, g0.num_synapses
+    rng = np.random.default_rng(11)
+    w = (rng.standard_normal(m) * 0.3).astype(np.float32)
+    con.set_edge_weights(w)
+    fires = rng.integers(0, 12, n).astype(np.int64)  # many ties
+    win = 0.35
+    cfg = StructConfig(grow_per_epoch=30, protect_rounds=0, locality_window=win)
+    rep = con.structural_update(np.zeros(m, np.int64), fires, cfg, mode=GROW_HEURISTIC)
+
+    order = np.lexsort((np.arange(m), np.abs(w)))
+    gone = np.zeros(m, bool)
+    gone[order[:30]] = True
+    pre0, post0 = g0.pre_ids(), np.asarray(g0.post_ids)
+    conn = set(zip(pre0[~gone].tolist(), post0[~gone].tolist()))
+    limit = float(np.float32(win))
+    hidden_top = len(layers) - 2
+    grown = []
+    for p in post0[gone].tolist():  # the freed posts, in CSR order
+        lp = int(lay[p])
+        best, most = -1, -1
+        for q in range(n):  # ids run layer by layer, so id order is layer order
+            ok_layer = lay[q] == lp - 1 or (lay[q] == lp and 1 <= lp <= hidden_top)
+            if q == p or not ok_layer or (q, p) in conn:
+                continue
+            if _dist(x, y, q, p) <= limit and fires[q] > most:
+                best, most = q, int(fires[q])
+        assert best >= 0
+        conn.add((best, p))
+        grown.append((best, p))
+    assert rep["pruned"] == 30 and rep["grown"] == 30 == len(grown)
+    g1 = con.to_graph()
+    got = sorted(zip(g1.pre_ids().tolist(), np.asarray(g1.post_ids).tolist()))
+    expect = sorted(list(zip(pre0[~gone].tolist(), post0[~gone].tolist())) + grown)
+    assert got == expect
+    w1 = con.edge_weights()
+    pairs = list(zip(g1.pre_ids().tolist(), np.asarray(g1.post_ids).tolist()))
+    new = np.array([pr in set(grown) for pr in pairs])
+    assert new.sum() == 30 and np.all(w1[new] == np.float32(cfg.w_grow_init))
