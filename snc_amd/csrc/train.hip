@@ -294,10 +294,14 @@ __device__ __forceinline__ void phase_mark(unsigned long long* s_ph, long long* 
 //   E_s = sum_t psi_t[post_s] * tr_t[pre_s],   t ascending, one fma per step,
 // exactly the per-step accumulation of trainer.cpp:132-139 (the product of
 // two floats is exact in double, so fma == add of the exact product).
-// The histories stream through LDS in chunks of CH steps, transposed: row j
-// holds CH consecutive steps of neuron j's psi (and, in a second array, its
-// trace), so one 16-byte load at a constant offset gives a lane four steps of
-// one neuron. Each lane takes one work unit: up to kEligPerThread synapses of
+// The histories stream through LDS in chunks of CH steps, transposed in
+// groups of four steps: float4 q * n + j holds steps 4q .. 4q + 3 of neuron
+// j's psi (and, in a second array, its trace), so one 16-byte load gives a
+// lane four steps of one neuron, and that load's LDS bank group (16-byte
+// granule mod 8) is j mod 8 whatever CH is. The units deal their synapses
+// so that the eight lanes one LDS pass serves read pres of different
+// classes mod 8, so those gathers do not conflict; the transposing writes
+// (consecutive j) do not either. Each lane takes one work unit: up to kEligPerThread synapses of
 // one post, in column order (a.eunit), with their sums in registers; it reads
 // the post's four psi values once and reuses them for every synapse of the
 // unit, so a fused multiply-add costs about 4 bytes of LDS reads (the
@@ -309,8 +313,8 @@ __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, cons
                              long long* s_last) {
   static_assert(CH % 4 == 0, "chunk rows are read four steps at a time");
   const int hn = a.hn, n = a.n;
-  float* ps_t = s_chunk;                   // [n][CH]
-  float* tr_t = s_chunk + (size_t)n * CH;  // [n][CH]
+  float4* ps_t = reinterpret_cast<float4*>(s_chunk);                   // [CH / 4][n]
+  float4* tr_t = reinterpret_cast<float4*>(s_chunk + (size_t)n * CH);  // [CH / 4][n]
   const float* hb = a.hist + (size_t)b * a.T_max * 2 * hn;
   {
     const int u = u0 + tid;
@@ -321,15 +325,14 @@ __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, cons
       len = (unit.x & 31) + 1;
       q0 = unit.y;
     }
-    const int pro = post * CH;
-    // row offsets of the unit's pres; a unit shorter than kEligPerThread
-    // repeats its last pre, so the sum loop has no per-synapse branches (the
-    // extra sums are never stored)
+    // the unit's pres; a unit shorter than kEligPerThread repeats its last
+    // pre, so the sum loop has no per-synapse branches (the extra sums are
+    // never stored)
     int tro[kEligPerThread];
     ET E[kEligPerThread];
 #pragma unroll
     for (int k = 0; k < kEligPerThread; ++k) {
-      tro[k] = (len > 0 ? a.ecol_pre[q0 + min(k, len - 1)] : 0) * CH;
+      tro[k] = len > 0 ? a.ecol_pre[q0 + min(k, len - 1)] : 0;
       E[k] = (ET)0;
     }
     for (int t0 = 0; t0 < T; t0 += CH) {
@@ -346,8 +349,8 @@ __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, cons
             pv[v] = c + v < tc ? row[j] : 0.0f;
             tv[v] = c + v < tc ? row[hn + j] : 0.0f;
           }
-          *(float4*)(ps_t + (size_t)j * CH + c) = make_float4(pv[0], pv[1], pv[2], pv[3]);
-          *(float4*)(tr_t + (size_t)j * CH + c) = make_float4(tv[0], tv[1], tv[2], tv[3]);
+          ps_t[(size_t)(c / 4) * n + j] = make_float4(pv[0], pv[1], pv[2], pv[3]);
+          tr_t[(size_t)(c / 4) * n + j] = make_float4(tv[0], tv[1], tv[2], tv[3]);
         }
       }
       __syncthreads();
@@ -356,12 +359,12 @@ __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, cons
         constexpr int kBatch = 8;  // trace rows loaded together before their sums
 #pragma unroll 1
         for (int q = 0; q < CH / 4; ++q) {
-          const float4 p = ((const float4*)(ps_t + pro))[q];
+          const float4 p = ps_t[(size_t)q * n + post];
 #pragma unroll
           for (int k0 = 0; k0 < kEligPerThread; k0 += kBatch) {
             float4 r[kBatch];
 #pragma unroll
-            for (int k = 0; k < kBatch; ++k) r[k] = ((const float4*)(tr_t + tro[k0 + k]))[q];
+            for (int k = 0; k < kBatch; ++k) r[k] = tr_t[(size_t)q * n + tro[k0 + k]];
 #pragma unroll
             for (int k = 0; k < kBatch; ++k) {
               ET e[1] = {E[k0 + k]};
