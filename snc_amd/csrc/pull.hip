@@ -89,6 +89,19 @@ __device__ __forceinline__ Word entry_word(const PullArgs& a, const Word* s_halo
   }
 }
 
+// Adds, in entry order, the values (lane k holds entry k's) of the entries
+// set in m (bit i: entry first + i) to acc. Every lane stays active and lanes
+// with nothing left add +0, which leaves a sum started from +0 as it is.
+__device__ __forceinline__ float walk_half(float acc, uint32_t m, int first, float w) {
+  while (ballot(m != 0)) {
+    const int k = m ? first + __builtin_ctz(m) : 0;
+    const float vk = __int_as_float(__builtin_amdgcn_ds_bpermute(k << 2, __float_as_int(w)));
+    acc = __fadd_rn(acc, m ? vk : 0.0f);
+    m &= m - 1;
+  }
+  return acc;
+}
+
 // Adds one round of 64 column entries (lane k holds entry k's fired word and
 // value) to every sample's sum, in column order. The fired bits are
 // transposed through the wave's 64 LDS masks (s_mask, all zero on entry and
@@ -113,21 +126,8 @@ __device__ __forceinline__ float add_round(Word word, float w, float acc,
   wave_lds_sync();
   const unsigned long long both = s_mask[lane];
   s_mask[lane] = 0ull;
-  uint32_t m = (uint32_t)both;  // entries 0-31, then 32-63: column order
-  while (ballot(m != 0)) {      // every lane active: lanes with nothing left add +0
-    const int k = m ? __builtin_ctz(m) : 0;
-    const float vk = __int_as_float(__builtin_amdgcn_ds_bpermute(k << 2, __float_as_int(w)));
-    acc = __fadd_rn(acc, m ? vk : 0.0f);
-    m &= m - 1;
-  }
-  m = (uint32_t)(both >> 32);
-  while (ballot(m != 0)) {
-    const int k = m ? 32 + __builtin_ctz(m) : 0;
-    const float vk = __int_as_float(__builtin_amdgcn_ds_bpermute(k << 2, __float_as_int(w)));
-    acc = __fadd_rn(acc, m ? vk : 0.0f);
-    m &= m - 1;
-  }
-  return acc;
+  acc = walk_half(acc, (uint32_t)both, 0, w);  // entries 0-31, then 32-63: column order
+  return walk_half(acc, (uint32_t)(both >> 32), 32, w);
 }
 
 // LDS of a step block: the halo's fired words, every wave's 64 sample masks,
@@ -322,6 +322,10 @@ k_pull_step(PullArgs a, int t) {
       p[u].v = ld32(a.v, so);
       p[u].r = ld32(a.refr, so);
     }
+    // The loads above are issued here, not where their values are first used
+    // (inside branches the compiler would otherwise sink them into, each then
+    // waiting out a full memory latency).
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const int i = i0 + u;
@@ -330,11 +334,12 @@ k_pull_step(PullArgs a, int t) {
       if (len < 0) continue;  // a hub (stepped below) or past the last neuron
       const uint32_t o0 = (uint32_t)(p[u].c0 + lane);
       float acc = 0.0f, ext = 0.0f;
-      acc = add_round<Word>(entry_word<Word, DIRECT>(a, s_halo, lane < len, p[u].h0, o0, tm),
-                            p[u].w0, acc, s_mask);
-      acc = add_round<Word>(
-          entry_word<Word, DIRECT>(a, s_halo, lane + kWave < len, p[u].h1, o0 + kWave, tm),
-          p[u].w1, acc, s_mask);
+      // both rounds' fired words looked up before either is used
+      const Word wd0 = entry_word<Word, DIRECT>(a, s_halo, lane < len, p[u].h0, o0, tm);
+      const Word wd1 =
+          entry_word<Word, DIRECT>(a, s_halo, lane + kWave < len, p[u].h1, o0 + kWave, tm);
+      acc = add_round<Word>(wd0, p[u].w0, acc, s_mask);
+      acc = add_round<Word>(wd1, p[u].w1, acc, s_mask);
       for (int k0 = 2 * kWave; k0 < len; k0 += kWave) {  // longer columns
         const uint32_t o = o0 + (uint32_t)k0;
         acc = add_round<Word>(
@@ -353,7 +358,9 @@ k_pull_step(PullArgs a, int t) {
         f = lif_step(v, r, ext, acc, a.lif);
         st32(a.v, so, v);
         if (r != p[u].r) st32(a.refr, so, (uint8_t)r);
-        if (f) st32(a.nspk, so, ld32(a.nspk, so) + 1);
+        // a no-return atomic: nothing waits for it (a load-add-store here
+        // waited out a memory latency whenever any sample fired)
+        if (f) atomicAdd(a.nspk + so, 1);
       }
       const uint64_t m = ballot(f);
       if (lane == 0) st32(Fw, (uint32_t)tm * (uint32_t)a.n + (uint32_t)j, (Word)m);
@@ -388,7 +395,7 @@ k_pull_step(PullArgs a, int t) {
       f = lif_step(v, r, ext, acc, a.lif);
       st32(a.v, so, v);
       if (r != r0) st32(a.refr, so, (uint8_t)r);
-      if (f) st32(a.nspk, so, ld32(a.nspk, so) + 1);
+      if (f) atomicAdd(a.nspk + so, 1);
     }
     const uint64_t m = ballot(f);
     if (lane == 0) st32(Fw, (uint32_t)tm * (uint32_t)a.n + (uint32_t)j, (Word)m);
