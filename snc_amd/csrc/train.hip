@@ -356,7 +356,7 @@ __device__ void elig_contrib(const TrainArgs& a, int b, int T, float scale, cons
       __syncthreads();
       phase_mark<PROF>(s_ph, s_last, 8, tid);
       if (len > 0) {
-        constexpr int kBatch = 8;  // trace rows loaded together before their sums
+        constexpr int kBatch = 4;  // trace rows loaded together before their sums (8 spilled VGPRs)
 #pragma unroll 1
         for (int q = 0; q < CH / 4; ++q) {
           const float4 p = ps_t[(size_t)q * n + post];
