@@ -47,12 +47,12 @@ extern "C" {
 #define SNC_VARIANT_SORT 7   /* hip-sort: per-segment counting sort of each step's events */
 #define SNC_VARIANT_AUTO 8   /* hip-auto: autotuner picks per graph           */
 #define SNC_VARIANT_AUTO_EXACT 9 /* hip-auto-exact: autotuner over the bit-exact
-                                    strategies (bucket, sort) only            */
+                                    strategies (bucket, sort; pull for
+                                    batched plans) only                       */
 #define SNC_VARIANT_PULL 10  /* hip-pull: batched gather over the incoming
                                 columns, all samples of a plan (up to 64) in
-                                one pass; light targets bit-exact, hub
-                                targets within the atomic tolerance
-                                (deterministic) */
+                                one pass, bit-exact (hub targets walk their
+                                chunks' arrival lists in column order) */
 
 /* ---- graph roles (GraphRole, snn_graph.hpp:29) ---------------------------- */
 #define SNC_ROLE_INTERNAL 0

@@ -16,10 +16,10 @@
 //     per step cost more than a graph-launched step on every measured graph
 //     (delivery stress: 29 vs 7.3 us per step with float atomics, 55 vs 12.1
 //     with buckets), so it runs only when a plan asks for it explicitly.
-//   * hip-pull for batched plans of up to 64 samples, at the 99th-percentile
-//     in-degree cap with 256- and 128-target blocks (the larger won with 64
-//     samples on the delivery-stress graph, the smaller with 8), and at
-//     twice the cap.
+//   * hip-pull (bit-exact) for batched plans of up to 64 samples, at the
+//     99th-percentile in-degree cap with 256- and 128-target blocks (the
+//     larger won with 64 samples on the delivery-stress graph, the smaller
+//     with 8), and at twice the cap.
 #include "autotune.hpp"
 #include "pull.hpp"
 
@@ -81,7 +81,7 @@ std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_o
   if (!exact_only)
     for (int blk : {128, 256, 512}) add(SNC_VARIANT_ATOMIC, blk, kLightLimit, 0);
   if (allow_sort) add(SNC_VARIANT_SORT, 256, kLightLimit, 0);
-  if (!exact_only && num_samples > 1 && num_samples <= kPullMaxSamples) {
+  if (num_samples > 1 && num_samples <= kPullMaxSamples) {
     const int cap = pull_default_cap(g);
     add(SNC_VARIANT_PULL, 256, cap, 0);
     add(SNC_VARIANT_PULL, 128, cap, 0);

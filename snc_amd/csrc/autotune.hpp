@@ -22,10 +22,10 @@ snc_plan_config default_plan_config(const DevGraph& g, int variant);
 // stepped as hubs.
 int pull_default_cap(const DevGraph& g);
 
-// Candidates for the search. exact_only drops hip-atomic and hip-pull;
-// allow_sort says whether hip-sort may run (single-sample plans only);
-// num_samples > 1 adds hip-pull (plans of up to 32 samples). Persistent
-// launches are never candidates.
+// Candidates for the search. exact_only drops hip-atomic; allow_sort says
+// whether hip-sort may run (single-sample plans only); num_samples > 1 adds
+// hip-pull (plans of up to 64 samples). Persistent launches are never
+// candidates.
 std::vector<snc_plan_config> autotune_candidates(const DevGraph& g, bool exact_only,
                                                  bool allow_sort, int num_samples);
 

@@ -125,7 +125,8 @@ struct snc_hip_plan {
   float *blk_amp = nullptr, *inj_heavy = nullptr;
   int2* inj_tab = nullptr;
   uint32_t* F = nullptr;      // hip-pull: fired words [L][n], 4 or 8 bytes each
-  float* hub_part = nullptr;  // hip-pull: hub chunk partials
+  float* hub_list = nullptr;  // hip-pull: hub arrivals per chunk and sample
+  uint16_t* hub_cnt = nullptr;  // hip-pull: their counts
   size_t F_words = 0;         // hip-pull: 32-bit words of F
   uint16_t* fsum = nullptr;   // hip-pull: fired summary [L][sum_words]
   size_t fsum_total = 0;
@@ -175,7 +176,8 @@ struct snc_hip_plan {
     dfree(blk_amp);
     dfree(inj_tab);
     dfree(F);
-    dfree(hub_part);
+    dfree(hub_list);
+    dfree(hub_cnt);
     dfree(fsum);
     dfree(inj_heavy);
     dfree(heavy_ids);
@@ -687,7 +689,14 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
         (uint64_t)g.L * n * word_u32 * 4 >= (1ull << 32))
       return fail(SNC_E_UNSUPPORTED, "hip-pull: graph or batch too large for 32-bit offsets");
     int prc = dalloc(&P->F, (size_t)g.L * n * word_u32);
-    if (!prc) prc = dalloc(&P->hub_part, (size_t)std::max(1, pg.n_chunks) * kPullMaxSamples);
+    // hub lists: a row of lstride values per hub entry, plus the rows (and
+    // one row's lanes) that loads past a chunk's list may touch
+    const int lstride = (B + 7) / 8 * 8;
+    if ((uint64_t)(pg.hub_entries + 16) * lstride * 4 >= (1ull << 32))
+      return fail(SNC_E_UNSUPPORTED, "hip-pull: hub lists too large for 32-bit offsets");
+    if (!prc) prc = dalloc(&P->hub_list, (size_t)(pg.hub_entries + 16) * lstride + kPullMaxSamples);
+    if (!prc)
+      prc = dalloc(&P->hub_cnt, (size_t)std::max(1, pg.n_chunks) * lstride + kPullMaxSamples);
     const int sum_words = (n + 15) / 16;
     // an even count: the fill kernel clears whole 32-bit words
     P->fsum_total = ((size_t)g.L * sum_words + 1) / 2 * 2;
@@ -723,6 +732,7 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     pa.chunk_pos = pg.chunk_pos;
     pa.chunk_len = pg.chunk_len;
     pa.hub_chunk_ptr = pg.hub_chunk_ptr;
+    pa.blk_order = pg.blk_order;
     pa.n_hub = pg.n_hub;
     pa.n_chunks = pg.n_chunks;
     pa.inj_blk_ptr = P->inj_blk_ptr;
@@ -732,7 +742,9 @@ static int plan_build(snc_hip_graph* G, const snc_event* events, int64_t n_event
     pa.inj_stride = a.inj_stride;
     pa.hinj_stride = a.hinj_stride;
     pa.F = P->F;
-    pa.hub_part = P->hub_part;
+    pa.hub_list = P->hub_list;
+    pa.hub_cnt = P->hub_cnt;
+    pa.lstride = lstride;
     pa.v = P->v;  // hip-pull state is sample-minor: [n][B]
     pa.refr = reinterpret_cast<uint8_t*>(P->refr);  // one byte per neuron and sample
     pa.nspk = P->nspk;

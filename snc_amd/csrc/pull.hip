@@ -18,6 +18,8 @@ constexpr int kHubRounds = kPullHubChunk / kWave;
 constexpr int kHubThreads = 256;  // chunk waves per hub-chunk block: 4
 constexpr int kBatch = 4;         // light targets whose loads go out together
 constexpr int kHaloRound = 8;     // halo words a thread loads per round
+constexpr int kHubGroup = 4;      // hub chunks whose list loads go out together
+constexpr int kHubPre = 8;        // list values per chunk loaded with its count
 static_assert(kPullHubChunk % kWave == 0, "hub chunks are whole rounds of a wave");
 static_assert(kPullWaveTargets % kBatch == 0, "whole batches per wave");
 
@@ -112,9 +114,8 @@ __device__ __forceinline__ float walk_half(float acc, uint32_t m, int first, flo
 // bits, not the entries with any bit set, and every bit loop runs on 32-bit
 // halves (one ffbl and one and per bit, no 64-bit emulation).
 template <class Word>
-__device__ __forceinline__ float add_round(Word word, float w, float acc,
-                                           unsigned long long* s_mask) {
-  if (ballot(word != (Word)0) == 0) return acc;
+__device__ __forceinline__ unsigned long long transpose_round(Word word,
+                                                              unsigned long long* s_mask) {
   const int lane = lane_id();
   uint32_t* m32 = reinterpret_cast<uint32_t*>(s_mask);
   const uint32_t me = 1u << (lane & 31);
@@ -126,8 +127,44 @@ __device__ __forceinline__ float add_round(Word word, float w, float acc,
   wave_lds_sync();
   const unsigned long long both = s_mask[lane];
   s_mask[lane] = 0ull;
+  return both;
+}
+
+template <class Word>
+__device__ __forceinline__ float add_round(Word word, float w, float acc,
+                                           unsigned long long* s_mask) {
+  if (ballot(word != (Word)0) == 0) return acc;
+  const unsigned long long both = transpose_round<Word>(word, s_mask);
   acc = walk_half(acc, (uint32_t)both, 0, w);  // entries 0-31, then 32-63: column order
   return walk_half(acc, (uint32_t)(both >> 32), 32, w);
+}
+
+// As walk_half, but appends the values to this lane's list (row pos of lst,
+// lane's column) instead of adding them; returns the new count.
+__device__ __forceinline__ int list_half(int pos, uint32_t m, int first, float w, float* lst,
+                                         int lstride) {
+  const int lane = lane_id();
+  while (ballot(m != 0)) {
+    const int k = m ? first + __builtin_ctz(m) : 0;
+    const float vk = __int_as_float(__builtin_amdgcn_ds_bpermute(k << 2, __float_as_int(w)));
+    if (m) {
+      lst[(size_t)pos * lstride + lane] = vk;
+      ++pos;
+    }
+    m &= m - 1;
+  }
+  return pos;
+}
+
+// One round of 64 hub entries: every sample's arriving values, in column
+// order, appended to its list (lane b: sample b, pos its count so far).
+template <class Word>
+__device__ __forceinline__ int list_round(Word word, float w, int pos, unsigned long long* s_mask,
+                                          float* lst, int lstride) {
+  if (ballot(word != (Word)0) == 0) return pos;
+  const unsigned long long both = transpose_round<Word>(word, s_mask);
+  pos = list_half(pos, (uint32_t)both, 0, w, lst, lstride);
+  return list_half(pos, (uint32_t)(both >> 32), 32, w, lst, lstride);
 }
 
 // LDS of a step block: the halo's fired words, every wave's 64 sample masks,
@@ -142,7 +179,8 @@ __host__ __device__ inline size_t lds_bytes(int word_bytes, int halo, int pool, 
   return halo_bytes(word_bytes, halo) + mask_bytes(blk) + (size_t)pool * 12 + (size_t)blk * 4 + 16;
 }
 
-// Hub chunks of step t: wave c sums chunk c per sample, in column order.
+// Hub chunks of step t: wave c lists chunk c's arrivals per sample, in
+// column order, and their counts.
 template <class Word>
 __global__ __launch_bounds__(kHubThreads) void k_pull_hub_chunks(PullArgs a, int t) {
   __shared__ unsigned long long s_masks[kHubThreads];
@@ -165,6 +203,7 @@ __global__ __launch_bounds__(kHubThreads) void k_pull_hub_chunks(PullArgs a, int
     w[u] = ld32(a.hub_w, o);
     if (k >= len) pk[u] = kHaloNone;
   }
+  asm volatile("" ::: "memory");  // the value loads are issued here, not where first used
   // the fired summary bit of each entry's (origin step, pre) first: most are
   // clear, and a clear bit needs no word
   uint32_t sb[kHubRounds];
@@ -179,10 +218,11 @@ __global__ __launch_bounds__(kHubThreads) void k_pull_hub_chunks(PullArgs a, int
 #pragma unroll
   for (int u = 0; u < kHubRounds; ++u)
     wd[u] = pk[u] != kHaloNone && (sb[u] & 1u) ? direct_word<Word>(a, pk[u], tm) : (Word)0;
-  float acc = 0.0f;
+  float* lst = a.hub_list + (size_t)p0 * a.lstride;  // the chunk's list rows
+  int pos = 0;
 #pragma unroll
-  for (int u = 0; u < kHubRounds; ++u) acc = add_round<Word>(wd[u], w[u], acc, s_mask);
-  a.hub_part[(size_t)c * kPullMaxSamples + lane] = acc;
+  for (int u = 0; u < kHubRounds; ++u) pos = list_round<Word>(wd[u], w[u], pos, s_mask, lst, a.lstride);
+  if (lane < a.lstride) a.hub_cnt[(size_t)c * a.lstride + lane] = (uint16_t)pos;
 }
 
 // A light target's first two rounds of entries and its state.
@@ -211,10 +251,11 @@ k_pull_step(PullArgs a, int t) {
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const int nthr = (int)blockDim.x, waves = nthr >> 6;
   // Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous
-  // run of target blocks so the halos it stages overlap in its own L2.
+  // run of target blocks so the halos it stages overlap in its own L2, the
+  // blocks with the most hub chunks to walk first (blk_order).
   const int R = a.n_blocks, x = blockIdx.x;
   const int q8 = R >> 3, r8 = R & 7, xcd = x & 7;
-  const int blk = xcd * q8 + min(xcd, r8) + (x >> 3);
+  const int blk = a.blk_order[xcd * q8 + min(xcd, r8) + (x >> 3)];
   const int base = blk * a.blk;
   const int tm = t % a.L;
 
@@ -368,8 +409,10 @@ k_pull_step(PullArgs a, int t) {
       sbits |= (m != 0 ? 1u : 0u) << i;
     }
   }
-  // The wave's hubs: each one's chunk partials in chunk order (16 loads in
-  // flight at a time), then integrate/fire.
+  // The wave's hubs: each one's chunk lists in chunk order, lane b adding
+  // sample b's arrivals one by one (counts and first values of kHubGroup
+  // chunks loaded together; the lists are padded past their end, and values
+  // past a count are loaded but not added), then integrate/fire.
   for (uint64_t hm = ballot(my_hub >= 0); hm; hm &= hm - 1) {
     const int i = __builtin_ctzll(hm);
     const int hub = __builtin_amdgcn_readlane(my_hub, i);
@@ -378,15 +421,37 @@ k_pull_step(PullArgs a, int t) {
     const uint32_t so = (uint32_t)j * uB + lane_b;
     float v = ld32(a.v, so);
     const int r0 = ld32(a.refr, so);
+    const uint32_t ls = (uint32_t)a.lstride;
     float acc = 0.0f;
-    for (int cc = c0; cc < c1; cc += 16) {
-      float pt[16];
+    for (int cc = c0; cc < c1; cc += kHubGroup) {
+      int cnt[kHubGroup];
+      uint32_t row[kHubGroup];
+      float x[kHubGroup][kHubPre];
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        pt[q] = cc + q < c1 ? a.hub_part[(size_t)(cc + q) * kPullMaxSamples + lane] : 0.0f;
+      for (int q = 0; q < kHubGroup; ++q) {
+        const bool in = cc + q < c1;
+        row[q] = in ? (uint32_t)a.chunk_pos[cc + q] : 0u;
+        cnt[q] = ld32(a.hub_cnt, (uint32_t)(in ? cc + q : c0) * ls + (uint32_t)lane);
+        if (!in || lane >= a.lstride) cnt[q] = 0;
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (cc + q < c1) acc = __fadd_rn(acc, pt[q]);
+        for (int k = 0; k < kHubPre; ++k)
+          x[q][k] = ld32(a.hub_list, (row[q] + (uint32_t)k) * ls + (uint32_t)lane);
+      }
+#pragma unroll
+      for (int q = 0; q < kHubGroup; ++q) {
+#pragma unroll
+        for (int k = 0; k < kHubPre; ++k)
+          if (k < cnt[q]) acc = __fadd_rn(acc, x[q][k]);
+        for (int k0 = kHubPre; k0 < cnt[q]; k0 += kHubPre) {  // longer lists, kHubPre at a time
+          float y[kHubPre];
+#pragma unroll
+          for (int k = 0; k < kHubPre; ++k)
+            y[k] = ld32(a.hub_list, (row[q] + (uint32_t)(k0 + k)) * ls + (uint32_t)lane);
+#pragma unroll
+          for (int k = 0; k < kHubPre; ++k)
+            if (k0 + k < cnt[q]) acc = __fadd_rn(acc, y[k]);
+        }
+      }
     }
     bool f = false;
     if (smp) {
@@ -436,9 +501,11 @@ void PullGraph::release() {
   dfree(chunk_pos);
   dfree(chunk_len);
   dfree(hub_chunk_ptr);
+  dfree(blk_order);
   hub_host.clear();
   cap = -1;
   n_blocks = n_hub = n_chunks = halo_words = 0;
+  hub_entries = 0;
   direct_entries = 0;
   direct = false;
 }
@@ -478,7 +545,6 @@ int pull_prepare(const DevGraph& g, int cap, int blk, PullGraph& pg) {
   std::vector<int32_t> hub_chunk_ptr(1, 0);
   std::vector<uint32_t> hub_pk;
   std::vector<float> hub_w;
-  std::vector<std::pair<uint64_t, int32_t>> ord;
   for (int j = 0; j < n; ++j) {
     const int deg = g.in_degree[j];
     if (deg <= cap) {
@@ -489,17 +555,11 @@ int pull_prepare(const DevGraph& g, int cap, int blk, PullGraph& pg) {
     const int h = (int)hubs.size();
     hub_of[j] = h;
     hubs.push_back(j);
-    // the hub's entries by (delay, pre), ties in column order
-    ord.clear();
-    for (int k = 0; k < deg; ++k) {
-      const uint32_t pk = cpk[(size_t)ccol[j] + k];
-      ord.push_back({((uint64_t)(pk & 0xFFu) << 56) | ((uint64_t)(pk >> 8) << 32) | (uint32_t)k, k});
-    }
-    std::sort(ord.begin(), ord.end());
+    // the hub's entries in column order: (delay descending, pre, CSR position)
     const int32_t e0 = (int32_t)hub_pk.size();
-    for (const auto& o : ord) {
-      hub_pk.push_back(cpk[(size_t)ccol[j] + o.second]);
-      hub_w.push_back(cvh[(size_t)ccol[j] + o.second]);
+    for (int k = 0; k < deg; ++k) {
+      hub_pk.push_back(cpk[(size_t)ccol[j] + k]);
+      hub_w.push_back(cvh[(size_t)ccol[j] + k]);
     }
     for (int k = 0; k < deg; k += kPullHubChunk) {
       chunk_hub.push_back(h);
@@ -555,9 +615,23 @@ int pull_prepare(const DevGraph& g, int cap, int blk, PullGraph& pg) {
       }
   }
   if (ldir.empty()) ldir.assign(1, 0u);
+  // Launch order: each XCD's run of target blocks (as the step kernel deals
+  // them), the blocks whose hubs have the most chunks first -- a hub's wave
+  // walks its chunks' lists one after another, so those blocks run longest
+  // and should not start in the launch's last round.
+  std::vector<int32_t> blk_order((size_t)R), blk_chunks((size_t)R, 0);
+  for (size_t h = 0; h < hubs.size(); ++h)
+    blk_chunks[(size_t)(hubs[h] / blk)] += hub_chunk_ptr[h + 1] - hub_chunk_ptr[h];
+  for (int xcd = 0, q8 = R >> 3, r8 = R & 7; xcd < 8; ++xcd) {
+    const int b0 = xcd * q8 + std::min(xcd, r8), b1 = b0 + q8 + (xcd < r8 ? 1 : 0);
+    for (int q = b0; q < b1; ++q) blk_order[(size_t)q] = q;
+    std::stable_sort(blk_order.begin() + b0, blk_order.begin() + b1,
+                     [&](int32_t u, int32_t v) { return blk_chunks[(size_t)u] > blk_chunks[(size_t)v]; });
+  }
   pg.n_blocks = R;
   pg.n_hub = (int)hubs.size();
   pg.n_chunks = (int)chunk_hub.size();
+  pg.hub_entries = (int64_t)hub_pk.size();
   pg.halo_words = std::max(1, halo_max);
   pg.direct_entries = direct;
   pg.direct = direct > 0;
@@ -577,6 +651,7 @@ int pull_prepare(const DevGraph& g, int cap, int blk, PullGraph& pg) {
   if (!rc) rc = dupload(&pg.chunk_pos, chunk_pos);
   if (!rc) rc = dupload(&pg.chunk_len, chunk_len);
   if (!rc) rc = dupload(&pg.hub_chunk_ptr, hub_chunk_ptr);
+  if (!rc) rc = dupload(&pg.blk_order, blk_order);
   if (rc) {
     pg.release();
     return rc;

@@ -20,17 +20,20 @@
 // columns, staged in LDS once per step (sorted by delay, then pre, so the
 // loads coalesce); entries beyond the halo capacity read F directly.
 //
-// Hub targets (in-degree above the cap) keep their entries sorted by
-// (delay, pre), so consecutive gathers of a chunk share F's cache lines, and
-// are cut into chunks of at most kPullHubChunk entries. A chunk is one wave's
-// work in a launch of its own before the step launch: it gathers the chunk's
-// fired words from F, sums them per sample in column order and stores the
-// partials; in the step launch the wave that steps the hub's id range adds
-// its partials in chunk order and steps the hub. A chunk gathers only the
-// words whose bit is set in the fired summary: one bit per neuron and step,
-// set when the neuron fired in any sample, written 16 neurons to a half-word
-// by the wave that steps them (no atomics). Hub sums are deterministic but
-// associated differently from the CPU's (the atomic-class tolerance).
+// Hub targets (in-degree above the cap) keep their entries in column order
+// too -- (delay descending, pre, CSR position), so consecutive gathers of a
+// chunk share F's cache lines -- cut into chunks of at most kPullHubChunk
+// entries. A chunk is one wave's work in a launch of its own before the step
+// launch: it gathers the chunk's fired words from F and, through the same bit
+// transpose, lists every sample's arriving values in column order (hub_list:
+// entry slot i of the chunk, sample b at (chunk start + i) * lstride + b;
+// their count per chunk and sample in hub_cnt). In the step launch the wave
+// that steps the hub's id range walks those lists chunk after chunk, lane b
+// adding sample b's values one by one from +0: the CPU's slot sum bit for
+// bit, as for light targets. A chunk gathers only the words whose bit is set
+// in the fired summary: one bit per neuron and step, set when the neuron
+// fired in any sample, written 16 neurons to a half-word by the wave that
+// steps them (no atomics).
 //
 // State is sample-minor, [n][B]: a wave's 64 lanes touch one target's samples
 // in one 256-byte run. Spike and event counts are not accumulated per step:
@@ -84,11 +87,13 @@ struct PullGraph {
   uint32_t* halo = nullptr;       // (pre << 8) | delay, per block by (delay, pre)
   int32_t* hub_ids = nullptr;     // [n_hub]
   int32_t* chunk_hub = nullptr;   // [n_chunks] hub index
-  uint32_t* hub_pk = nullptr;     // hub columns: (pre << 8) | delay, by (delay, pre) per hub
+  int64_t hub_entries = 0;        // entries of all hub columns
+  uint32_t* hub_pk = nullptr;     // hub columns: (pre << 8) | delay, in column order
   float* hub_w = nullptr;         // the value of each hub entry
   int32_t* chunk_pos = nullptr;   // [n_chunks] first entry in hub_pk / hub_w
   int32_t* chunk_len = nullptr;   // [n_chunks] entries
   int32_t* hub_chunk_ptr = nullptr;  // [n_hub + 1]
+  int32_t* blk_order = nullptr;   // [n_blocks] target block of each launch slot
   std::vector<int32_t> hub_host;
   int64_t dev_bytes = 0;
   void release();
@@ -118,6 +123,7 @@ struct PullArgs {
   const int32_t* chunk_pos;
   const int32_t* chunk_len;
   const int32_t* hub_chunk_ptr;
+  const int32_t* blk_order;
   int n_hub, n_chunks;
   // injections: per (sample, step, block) lists and a dense hub table
   const int32_t* inj_blk_ptr;  // [B][T][n_blocks + 1]
@@ -129,7 +135,9 @@ struct PullArgs {
   void* F;            // [L][n] words
   uint16_t* fsum;     // [L][sum_words] fired summary: bit i of word q = neuron 16 q + i
   int sum_words;
-  float* hub_part;    // [n_chunks][kPullMaxSamples]
+  float* hub_list;    // [hub entries][lstride] arriving values per chunk and sample
+  uint16_t* hub_cnt;  // [n_chunks][lstride] how many of them
+  int lstride;        // samples per list row: B rounded up to a multiple of 8
   float* v;           // [n][B]
   uint8_t* refr;      // [n][B] refractory steps left (the refractory period is at most 255)
   int32_t* nspk;      // [n][B]

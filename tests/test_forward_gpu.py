@@ -346,7 +346,7 @@ def test_batched_forward_equals_one_sample_at_a_time(cfg):
         ref = O.forward(g, evs[b], T, lif)
         assert sp[b] == ref["spikes"] and ev[b] == ref["events"]
         assert np.array_equal(r.neuron_spikes[b], ref["neuron_spikes"])
-        if variant in (Backend.HipAtomic, Backend.HipPull):
+        if variant == Backend.HipAtomic:
             np.testing.assert_allclose(r.final_v[b], ref["final_v"], rtol=V_RTOL, atol=V_ATOL)
         else:
             assert np.array_equal(r.raster[b], ref["raster"])

@@ -1,11 +1,10 @@
 """hip-pull (batched gather over halo-indexed incoming columns) through the C ABI.
 
 Light targets (in-degree up to the plan's cap) sum their inputs in column
-order, so without hubs every sample's raster, counts, logits and membranes
-equal the CPU oracle's bit for bit (and hip-bucket's), with 32-bit fired words
-(up to 32 samples) and 64-bit ones (33 to 64). Hub targets add per-chunk
-partial sums: counts identical, voltages within 1e-5 relative, and the same
-bits on every run.
+order, and hub targets walk their chunks' per-sample arrival lists in column
+order too, so every sample's raster, counts, logits and membranes equal the
+CPU oracle's bit for bit (and hip-bucket's), with 32-bit fired words (up to
+32 samples) and 64-bit ones (33 to 64), with and without hubs.
 """
 import numpy as np
 import pytest
@@ -19,7 +18,6 @@ from snc_amd.runtime import Backend, ForwardPlan, HipGraph, PlanConfig
 
 pytestmark = pytest.mark.gpu
 
-V_RTOL, V_ATOL = 1e-5, 1e-6
 
 
 def _drives(n, T, B, seed, p0=0.01, dp=0.003):
@@ -66,7 +64,10 @@ def test_pull_without_hubs_is_bit_exact(B, blk):
 
 
 @pytest.mark.parametrize("B", [12, 50])
-def test_pull_with_hubs_within_tolerance_and_deterministic(B):
+def test_pull_with_hubs_is_bit_exact(B):
+    """Hub columns of up to 5000 entries (up to ten 512-entry chunks), at 32-
+    and 64-bit fired words: raster, counts, logits and membranes bit for bit,
+    and the same again on a second run of the plan."""
     g = G.hub_sheet_graph(60, 50, 40, hub_fraction=0.02, hub_min=40, hub_max=5000,
                           w_std=0.06, max_delay=20, seed=9)
     lif = K.lif_tau20()
@@ -83,8 +84,11 @@ def test_pull_with_hubs_within_tolerance_and_deterministic(B):
     for b in range(B):
         ref = O.forward(g, evs[b], T, lif)
         assert sp[b] == ref["spikes"] and ev[b] == ref["events"]
+        assert np.array_equal(r.raster[b], ref["raster"]), b
         assert np.array_equal(r.neuron_spikes[b], ref["neuron_spikes"])
-        np.testing.assert_allclose(r.final_v[b], ref["final_v"], rtol=V_RTOL, atol=V_ATOL)
+        assert np.array_equal(r.logits[b], ref["logits"])
+        assert np.array_equal(r.final_v[b].view(np.uint32), ref["final_v"].view(np.uint32)), b
+        assert np.array_equal(r.step_spikes[b], ref["step_spikes"])
     r2 = plan.run(capture_raster=True, capture_v=True)
     assert np.array_equal(r2.raster, r.raster)
     assert np.array_equal(r2.final_v.view(np.uint32), r.final_v.view(np.uint32))
@@ -184,3 +188,25 @@ def test_batched_autotuner_searches_again_for_another_sample_count():
     r3 = hg.tune_report(batched=True)
     assert r3["search_ms"] != r1["search_ms"]
     assert [m for _, m in r3["candidates"]] != [m for _, m in r1["candidates"]]
+
+
+def test_batched_exact_autotuner_considers_pull():
+    """hip-auto-exact on a batched plan searches hip-bucket and hip-pull (both
+    bit-exact), never hip-atomic; whichever it picks matches the oracle bit
+    for bit."""
+    g = G.hub_sheet_graph(60, 50, 40, hub_fraction=0.02, hub_min=40, hub_max=1500,
+                          w_std=0.06, max_delay=20, seed=9)
+    lif = K.lif_tau20()
+    T, B = 60, 16
+    evs, allev = _drives(g.num_neurons, T, B, seed=6)
+    hg = HipGraph(g)
+    plan = ForwardPlan(hg, allev, T, lif, Backend.HipAutoExact, num_samples=B)
+    rep = hg.tune_report(exact_only=True, batched=True)
+    variants = {c.variant for c, _ in rep["candidates"]}
+    assert Backend.HipPull in variants and Backend.HipAtomic not in variants
+    assert plan.variant in (Backend.HipPull, Backend.HipBucket)
+    r = plan.run(capture_raster=True, capture_v=True)
+    for b in range(B):
+        ref = O.forward(g, evs[b], T, lif)
+        assert np.array_equal(r.raster[b], ref["raster"])
+        assert np.array_equal(r.final_v[b].view(np.uint32), ref["final_v"].view(np.uint32))
