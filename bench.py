@@ -191,7 +191,7 @@ def eprop_bench(w, epochs, plain_batches, elig, cpu_samples=0):
     samples/s are wall-clock rates of whole training steps (host loop,
     launches, synchronisation and the per-step readback included)."""
     import numpy as np
-    from snc_amd.train import BATCHED, PLAIN
+    from snc_amd.train import BATCHED, PLAIN, SERIAL
 
     n = w.labels.size
     tr = np.arange(w.n_train, dtype=np.int32)
@@ -225,12 +225,19 @@ def eprop_bench(w, epochs, plain_batches, elig, cpu_samples=0):
         w_plain = sp.get_weights()
         acc_plain = sp.evaluate(te)
         sp.close()
+        # the per-sample persistent path: the batched kernel launched once per
+        # sample (a minibatch of one), same minibatches, same start weights
+        ss = _train_session(w, elig)
+        wall_s, dev_s, n_s, _, _, _ = run(ss, SERIAL, batches[:nb], 1)
+        w_serial = ss.get_weights()
+        ss.close()
     # batched path over the same minibatches, then the rest of the epochs
     wall_b, dev_b, n_b, spk_b, ev_b, st_b = run(sb, BATCHED, batches[:nb], 1)
     nmb = len(batches[:nb])
     w_batch = sb.get_weights()
     acc_same = sb.evaluate(te)
     identical = bool(np.array_equal(w_plain, w_batch)) if nb > 0 else None
+    identical_serial = bool(np.array_equal(w_serial, w_batch)) if nb > 0 else None
     if nb < len(batches):
         r = run(sb, BATCHED, batches[nb:], 1)
         wall_b, dev_b, n_b = wall_b + r[0], dev_b + r[1], n_b + r[2]
@@ -267,11 +274,18 @@ def eprop_bench(w, epochs, plain_batches, elig, cpu_samples=0):
         out["plain"] = {"samples_per_s": psps, "device_samples_per_s": n_p / (dev_p / 1e3),
                         "samples": n_p, "GSynOps/s": ev_p / wall_p / 1e9}
         out["speedup_batched_vs_plain"] = bsps / psps
+        ssps = n_s / wall_s
+        out["serial"] = {"samples_per_s": ssps, "device_samples_per_s": n_s / (dev_s / 1e3),
+                         "samples": n_s,
+                         "what": "one launch of the batched kernel per sample (minibatch of one)"}
+        out["speedup_batched_vs_serial"] = bsps / ssps
         out["equal_accuracy_check"] = {"minibatches": nb, "weights_bit_identical": identical,
+                                       "serial_weights_bit_identical": identical_serial,
                                        "test_acc_plain": acc_plain, "test_acc_batched": acc_same,
                                        "acc_diff_pt": 100.0 * (acc_same - acc_plain)}
         log(f"e-prop {w.name}: batched {bsps:.1f} samples/s, plain {psps:.1f} samples/s "
-            f"(x{bsps / psps:.1f}), identical={identical}, acc {acc0:.3f} -> {curve}")
+            f"(x{bsps / psps:.1f}), serial {ssps:.1f} samples/s (x{bsps / ssps:.1f}), "
+            f"identical={identical}/{identical_serial}, acc {acc0:.3f} -> {curve}")
     else:
         log(f"e-prop {w.name}: batched {bsps:.1f} samples/s, acc {acc0:.3f} -> {curve}")
     out["global_T_max"] = eprop_global_t(w, epochs, elig)

@@ -313,6 +313,13 @@ typedef struct snc_train_config {
 #define SNC_TRAIN_PLAIN 0   /* one sample at a time, per-step kernel launches       */
 #define SNC_TRAIN_BATCHED 1 /* whole minibatch in one persistent launch, one
                                workgroup per sample running its own T_b steps    */
+#define SNC_TRAIN_SERIAL 2  /* one sample at a time, each one launch of the
+                               persistent kernel as a minibatch of one (its
+                               eligibility rounds on otherwise idle CUs); the
+                               update is the batched one, so the weights are
+                               bit-identical to SNC_TRAIN_BATCHED's. No
+                               counterpart in the reference: it is the
+                               per-sample baseline of the batched launch      */
 
 #define SNC_ELIG_F32 0 /* eligibility accumulated in float (fused multiply-add)    */
 #define SNC_ELIG_F64 1 /* in double, exactly the CPU trainer's sums (trainer.cpp:137) */

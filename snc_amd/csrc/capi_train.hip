@@ -658,6 +658,31 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
     if (!rc)
       rc = train_launch_batched(a, B, train_batched_grid(a, B, s->n_cu), s->f64, s->ring_lds,
                                 s->lds_max, sm);
+  } else if (mode == SNC_TRAIN_SERIAL) {
+    // One launch of the batched kernel per sample, in minibatch order. Each
+    // launch sees sample b's rows of every per-minibatch array as a
+    // minibatch of one, so the contributions land in row b as the batched
+    // launch writes them, and the update below sums the same rows in the
+    // same order.
+    if (a.C) rc = fill0(sm, s->wq, sizeof(int) * 2 * (size_t)B);
+    for (int b = 0; b < B && !rc; ++b) {
+      TrainArgs ab = a;
+      ab.idx = a.idx + b;
+      ab.hist = a.hist + (size_t)b * a.T_max * 2 * a.hn;
+      if (a.C) ab.C = a.C + (size_t)b * a.S;
+      if (a.ring_g) ab.ring_g = a.ring_g + (size_t)b * a.L * a.n;
+      if (a.arq) ab.arq = a.arq + (size_t)b * a.L * a.arq_cap;
+      ab.logits = a.logits + (size_t)b * a.classes;
+      ab.loss = a.loss + b;
+      ab.correct = a.correct + b;
+      ab.blk_spikes = a.blk_spikes + b;
+      ab.blk_events = a.blk_events + b;
+      if (a.raster) ab.raster = a.raster + (size_t)b * a.T_max * a.words;
+      ab.wq = a.wq + 2 * (size_t)b;
+      ab.Ls_g = a.Ls_g + (size_t)b * a.n;
+      rc = train_launch_batched(ab, 1, train_batched_grid(ab, 1, s->n_cu), s->f64, s->ring_lds,
+                                s->lds_max, sm);
+    }
   } else if (mode == SNC_TRAIN_PLAIN) {
     const size_t esz = s->f64 ? sizeof(double) : sizeof(float);
     rc = fill0(sm, s->p_spikes, 8);
@@ -745,7 +770,7 @@ int snc_hip_train_step(snc_hip_train* s, const int32_t* idx, int32_t B, int32_t 
     return fail(SNC_E_HIP, "raster readback failed");
   if (st) {
     std::memset(st, 0, sizeof(*st));
-    if (mode == SNC_TRAIN_BATCHED) {
+    if (mode != SNC_TRAIN_PLAIN) {
       (void)hipMemcpy(sp.data(), s->blk_spikes, 8 * B, hipMemcpyDeviceToHost);
       (void)hipMemcpy(evn.data(), s->blk_events, 8 * B, hipMemcpyDeviceToHost);
       for (int b = 0; b < B; ++b) {

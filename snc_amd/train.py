@@ -2,9 +2,11 @@
 
 ``HipTrainSession`` keeps the reference lifecycle -- create, set/get weights,
 train_epoch, evaluate, reset/collect stats, destroy -- on top of the C ABI
-(snc_hip_train_*). Two execution modes share one definition of the math:
+(snc_hip_train_*). Three execution modes share one definition of the math:
 PLAIN runs one sample at a time with per-step kernel launches; BATCHED runs a
-whole minibatch of uneven-length samples in one persistent launch.
+whole minibatch of uneven-length samples in one persistent launch; SERIAL runs
+one sample at a time, each as one launch of that persistent kernel (the
+per-sample baseline of BATCHED, with bit-identical weights).
 """
 from __future__ import annotations
 
@@ -18,7 +20,7 @@ from .graph import SNNGraph
 from .inputs import EVENT_DTYPE
 from .runtime import LIFParams
 
-PLAIN, BATCHED = 0, 1
+PLAIN, BATCHED, SERIAL = 0, 1, 2
 ELIG_F32, ELIG_F64 = 0, 1
 READOUT_SPIKING, READOUT_LEAKY = 0, 1
 

@@ -22,7 +22,7 @@ import pytest
 import oracle as O
 from snc_amd import configs as K
 from snc_amd.runtime import (Backend, ForwardPlan, HipGraph, PlanConfig, forward_oneshot)
-from snc_amd.train import BATCHED, ELIG_F32, ELIG_F64, PLAIN, HipTrainSession
+from snc_amd.train import BATCHED, ELIG_F32, ELIG_F64, PLAIN, SERIAL, HipTrainSession
 
 pytestmark = pytest.mark.gpu
 
@@ -204,7 +204,7 @@ def eprop_refs():
 
 @pytest.mark.parametrize("name", ["c2", "c3"])
 @pytest.mark.parametrize("mode,elig", [(BATCHED, ELIG_F64), (PLAIN, ELIG_F64),
-                                       (BATCHED, ELIG_F32)])
+                                       (SERIAL, ELIG_F64), (BATCHED, ELIG_F32)])
 def test_eprop_full_minibatch(eprop_refs, name, mode, elig):
     w, idx, ref = eprop_refs[name]
     print(f"{name} full minibatch, mode {mode}, elig {elig}", flush=True)

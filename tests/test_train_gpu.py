@@ -17,7 +17,7 @@ from snc_amd import graph as G
 from snc_amd import inputs as I
 from snc_amd.runtime import LIFParams
 from snc_amd.train import (BATCHED, ELIG_F32, ELIG_F64, PLAIN, READOUT_LEAKY, READOUT_SPIKING,
-                           HipTrainSession, TrainConfig)
+                           SERIAL, HipTrainSession, TrainConfig)
 
 pytestmark = pytest.mark.gpu
 
@@ -67,7 +67,7 @@ def _compare(sess, orc_tr, w0, idx, ev_orc, lengths, labels, mode, want_exact_w=
     return wg
 
 
-@pytest.mark.parametrize("mode", [BATCHED, PLAIN])
+@pytest.mark.parametrize("mode", [BATCHED, PLAIN, SERIAL])
 @pytest.mark.parametrize("elig", [ELIG_F64, ELIG_F32])
 def test_minibatch_step_matches_oracle(mode, elig):
     g = G.static_snc([64, 48, 8], 1500, 0.5, 1, 0.2, 4)
@@ -87,7 +87,10 @@ def test_minibatch_step_matches_oracle(mode, elig):
     assert np.array_equal(neur, oneur) and np.array_equal(syn, osyn)
 
 
-def test_plain_and_batched_are_bit_identical():
+def test_plain_serial_and_batched_are_bit_identical():
+    """The three modes on one minibatch: weights, logits, rasters and totals
+    bit for bit (SERIAL launches the batched kernel once per sample and sums
+    the same contribution rows in the same order)."""
     g = G.sheet_graph([100, 64, 10], 16, recurrent_fan_in=8, radius=3.0, sheet=10.0, w_std=0.3,
                       max_delay=5, seed=2)
     B = 16
@@ -96,13 +99,14 @@ def test_plain_and_batched_are_bit_identical():
     lengths = np.random.default_rng(1).integers(10, 40, B).astype(np.int32)
     gcfg, _ = _cfgs(40, readout=READOUT_LEAKY, reset_mode=1, encoder=1, max_rate=0.4)
     out = []
-    for mode in (PLAIN, BATCHED):
+    for mode in (PLAIN, BATCHED, SERIAL):
         s = HipTrainSession(g, gcfg, batch=B, max_steps=40)
         s.upload_features(Xf, y, lengths)
         st, lg, ras = s.step(np.arange(B), epoch=2, mode=mode, want_logits=True, want_raster=True)
         out.append((s.get_weights(), lg, ras, st.spikes, st.synaptic_events))
-    for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b)
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
 
 
 def test_plain_graph_replay_equals_batched_over_minibatches():
@@ -176,7 +180,7 @@ def test_device_latency_matches_oracle(mode):
     _compare(sess, orc, w0, np.arange(B), ev, lengths, y, mode)
 
 
-@pytest.mark.parametrize("mode", [BATCHED, PLAIN])
+@pytest.mark.parametrize("mode", [BATCHED, PLAIN, SERIAL])
 @pytest.mark.parametrize("readout,elig", [(READOUT_SPIKING, ELIG_F64), (READOUT_LEAKY, ELIG_F64),
                                           (READOUT_LEAKY, ELIG_F32)])
 def test_reward_mode_matches_oracle_over_minibatches(mode, readout, elig):
@@ -247,8 +251,9 @@ def test_single_sample_steps_match_the_compiled_reference_trainer(reward, mode):
     assert np.abs(ref.weights - start).max() > 1e-3
 
 
+@pytest.mark.parametrize("mode", [BATCHED, SERIAL])
 @pytest.mark.parametrize("delay_store", [None, "queues", "hbm"])
-def test_delay_rich_recurrent_ring_outside_lds(monkeypatch, delay_store):
+def test_delay_rich_recurrent_ring_outside_lds(monkeypatch, delay_store, mode):
     """Recurrent 10 % graph with delays 1..31, with the deposits in flight kept
     where the session chooses (the ring in LDS here), in per-slot arrival
     queues, or in a ring in HBM (SNC_HIP_TRAIN_DELAY forces the last two)."""
@@ -263,7 +268,7 @@ def test_delay_rich_recurrent_ring_outside_lds(monkeypatch, delay_store):
     sess.upload_events(ev, B, y, lengths)
     orc = O.Trainer(g, ocfg)
     w0 = orc.weights
-    _compare(sess, orc, w0, np.arange(B), ev, lengths, y, BATCHED)
+    _compare(sess, orc, w0, np.arange(B), ev, lengths, y, mode)
 
 
 def test_delay_stores_give_identical_weights(monkeypatch):
