@@ -181,6 +181,7 @@ __device__ void sample_epilogue(const TrainArgs& a, int b, const float* logits_i
 constexpr int kFoldBatch = 8;       // staged arrivals a fold round loads together
 constexpr int kDeposit = 4;         // deposits a lane loads before queueing them
 constexpr int kHeld = 6;            // arrivals a thread keeps in registers while sorting
+constexpr int kRowsHeld = 4;        // fired rows a wave keeps in registers from bits to staging
 
 // Ordered fold of target j's arrivals into its ring slots. Columns are
 // ordered delay descending, so one step's arrivals for one ring slot are one
@@ -617,13 +618,19 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
         a.raster[((size_t)b * a.T_max + t) * a.words + w] = s_fw[w];
     const int nf = s_nf;
     const int wave = tid >> 6, lane = lane_id();
+    constexpr int kWaves = kTrainThreads / kWave;
     unsigned long long ev = 0;
-    for (int k = wave; k < nf; k += kTrainThreads / kWave) {
-      const int i = s_fired[k];
-      const int rb = s_rowp[i], re = s_rowp[i + 1];
-      if (queue) {
-        // each deposit joins the queue of the slot it arrives at; a lane
-        // loads kDeposit deposits before it reserves and stores any
+    // Bucket bits (no queues): the first 64 entries of the wave's first
+    // kRowsHeld fired rows keep their column position, weight and delay in
+    // registers until the staging below, which then reads no row twice.
+    int hp[kRowsHeld];
+    uint2 hv[kRowsHeld];
+    if (queue) {
+      // each deposit joins the queue of the slot it arrives at; a lane loads
+      // kDeposit deposits before it reserves and stores any
+      for (int k = wave; k < nf; k += kWaves) {
+        const int i = s_fired[k];
+        const int rb = s_rowp[i], re = s_rowp[i + 1];
         for (int e0 = rb + lane; e0 < re; e0 += kDeposit * kWave) {
           int sl[kDeposit];
           uint2 rec[kDeposit];
@@ -644,13 +651,36 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
               arq[(size_t)sl[u] * a.arq_cap + q] = rec[u];
             }
         }
-      } else {
-        for (int e = rb + lane; e < re; e += kWave) {
+        ev += (unsigned long long)(re - rb);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kRowsHeld; ++r) {
+        const int k = wave + r * kWaves;
+        hp[r] = -1;
+        if (k < nf) {
+          const int i = s_fired[k];
+          const int e = s_rowp[i] + lane;
+          if (e < s_rowp[i + 1]) {
+            hp[r] = a.cpos[e];
+            hv[r] = make_uint2(__float_as_uint(a.w_csr[e]), (uint32_t)a.sdel[e]);
+          }
+        }
+      }
+      asm volatile("" ::: "memory");  // every held row's loads in flight before any is used
+#pragma unroll
+      for (int r = 0; r < kRowsHeld; ++r)
+        if (hp[r] >= 0) atomicOr(&s_bits[hp[r] >> 5], 1u << (hp[r] & 31));
+      for (int k = wave; k < nf; k += kWaves) {
+        const int i = s_fired[k];
+        const int rb = s_rowp[i], re = s_rowp[i + 1];
+        const bool held = k < wave + kRowsHeld * kWaves;
+        for (int e = rb + lane + (held ? kWave : 0); e < re; e += kWave) {
           const int p = a.cpos[e];
           atomicOr(&s_bits[p >> 5], 1u << (p & 31));
         }
+        ev += (unsigned long long)(re - rb);
       }
-      ev += (unsigned long long)(re - rb);
     }
     if (lane == 0 && ev) {
       atomicAdd(&s_ev, ev);
@@ -769,12 +799,21 @@ __device__ void train_sample(const TrainArgs& a, unsigned char* smem, int b,
       if (hi == W) s_pref[W] = (uint16_t)run;  // the total (every such thread writes the same)
       __syncthreads();
       phase_mark<PROF>(s_ph, &s_last, 3, tid);
-      // (B3) every arrival's weight and delay at its rank, from the fired
-      // rows read in CSR order (coalesced)
-      for (int k = wave; k < nf; k += kTrainThreads / kWave) {
+      // (B3) every arrival's weight and delay at its rank: the held entries
+      // from registers, the rest from the fired rows read in CSR order
+      // (coalesced)
+#pragma unroll
+      for (int r = 0; r < kRowsHeld; ++r)
+        if (hp[r] >= 0) {
+          const int p = hp[r], q = p >> 5;
+          const int rk = (int)s_pref[q] + __popc(s_bits[q] & ((1u << (p & 31)) - 1u));
+          s_cval[rk] = hv[r];
+        }
+      for (int k = wave; k < nf; k += kWaves) {
         const int i = s_fired[k];
         const int rb = s_rowp[i], re = s_rowp[i + 1];
-        for (int e = rb + lane; e < re; e += kWave) {
+        const bool held = k < wave + kRowsHeld * kWaves;
+        for (int e = rb + lane + (held ? kWave : 0); e < re; e += kWave) {
           const int p = a.cpos[e];
           const float wv = a.w_csr[e];
           const uint32_t d = a.sdel[e];
