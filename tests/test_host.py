@@ -35,6 +35,40 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert set(declared) <= set(L.exported_symbols())
 
 
+def _header_defines():
+    out = {}
+    for h in ("snc_hip.h", "snc_gen.h", "snc_struct.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        for m in re.finditer(r"^#define\s+(SNC_\w+)\s+\(?(-?\d+)\)?", src, flags=re.M):
+            out[m.group(1)] = int(m.group(2))
+    return out
+
+
+def test_python_constants_mirror_the_header():
+    """Every integer the Python mirror passes across the C ABI (status codes,
+    strategies, reset modes, training modes, eligibility precisions, readouts,
+    kernel phases) has the value the header defines."""
+    from snc_amd import train as T
+
+    d = _header_defines()
+    pairs = {
+        "SNC_OK": L.SNC_OK, "SNC_E_INVALID": L.SNC_E_INVALID, "SNC_E_NODEVICE": L.SNC_E_NODEVICE,
+        "SNC_E_HIP": L.SNC_E_HIP, "SNC_E_NOMEM": L.SNC_E_NOMEM,
+        "SNC_E_UNSUPPORTED": L.SNC_E_UNSUPPORTED,
+        "SNC_VARIANT_ATOMIC": L.VARIANT_ATOMIC, "SNC_VARIANT_BUCKET": L.VARIANT_BUCKET,
+        "SNC_VARIANT_SORT": L.VARIANT_SORT, "SNC_VARIANT_AUTO": L.VARIANT_AUTO,
+        "SNC_VARIANT_AUTO_EXACT": L.VARIANT_AUTO_EXACT, "SNC_VARIANT_PULL": L.VARIANT_PULL,
+        "SNC_RESET_TO_VALUE": L.RESET_TO_VALUE, "SNC_RESET_SUBTRACT": L.RESET_SUBTRACT,
+        "SNC_TRAIN_PLAIN": T.PLAIN, "SNC_TRAIN_BATCHED": T.BATCHED, "SNC_TRAIN_SERIAL": T.SERIAL,
+        "SNC_ELIG_F32": T.ELIG_F32, "SNC_ELIG_F64": T.ELIG_F64,
+        "SNC_READOUT_SPIKING": T.READOUT_SPIKING, "SNC_READOUT_LEAKY": T.READOUT_LEAKY,
+        "SNC_TRAIN_PHASES": T.PHASES,
+    }
+    for name, value in pairs.items():
+        assert name in d, name
+        assert d[name] == value, (name, d[name], value)
+
+
 def test_encode_matches_reference_goldens():
     z = np.load(os.path.join(GOLDEN, "encode.npz"), allow_pickle=False)
     for k in range(3):
