@@ -796,7 +796,7 @@ def main():
                       "plan": {"block_neurons": plan.config.block_neurons,
                                "heavy_threshold": plan.config.heavy_threshold,
                                "persistent": plan.config.persistent},
-                      "spikes_per_step": spikes, "events_per_step": events,
+                      "spikes_per_run": spikes, "events_per_run": events,
                       "parallelism": "single-gpu"},
            "roofline": roofline, "variants": variants, "autotune": autotune,
            "memory": {"graph_bytes": hg.memory(), "plan_bytes": plan.memory(),
