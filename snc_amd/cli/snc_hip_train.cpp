@@ -6,6 +6,7 @@
 //   snc_hip_train --dataset prototype --structure sheet --epochs 3 --batch 256
 //   snc_hip_train --dataset events --structure recurrent --lr 3 --epochs 5
 //   snc_hip_train --train-mode plain ...      (same math, one sample at a time)
+//   snc_hip_train --train-mode serial ...     (the persistent kernel once per sample)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -34,7 +35,7 @@ void usage() {
   std::printf(
       "snc_hip_train -- e-prop training on the MI355X backend\n"
       "  --dataset prototype|synthetic|events   --structure sheet|static-snc|recurrent\n"
-      "  --train-mode batched|plain  --epochs N  --batch N  --num-steps T (max)  --min-steps T\n"
+      "  --train-mode batched|plain|serial  --epochs N  --batch N  --num-steps T (max)  --min-steps T\n"
       "  --num-train N  --num-test N  --hidden N  --dim N  --classes N\n"
       "  --fan-in K  --rec-fan-in K   (sheet: input and recurrent partners per neuron)\n"
       "  --lr F  --w-init F (0: the graph's weights)  --w-max F  --w-std F  --gamma F\n"
@@ -161,7 +162,13 @@ int main(int argc, char** argv) {
   std::vector<int32_t> tr(o.num_train), te(o.num_test);
   std::iota(tr.begin(), tr.end(), 0);
   std::iota(te.begin(), te.end(), o.num_train);
-  const int mode = o.train_mode == "plain" ? SNC_TRAIN_PLAIN : SNC_TRAIN_BATCHED;
+  const int mode = o.train_mode == "plain"    ? SNC_TRAIN_PLAIN
+                   : o.train_mode == "serial" ? SNC_TRAIN_SERIAL
+                                              : SNC_TRAIN_BATCHED;
+  if (mode == SNC_TRAIN_BATCHED && o.train_mode != "batched") {
+    std::fprintf(stderr, "unknown --train-mode: %s\n", o.train_mode.c_str());
+    return 2;
+  }
   std::printf("graph: neurons=%d synapses=%d  dataset=%s train=%d test=%d  mode=%s batch=%d\n",
               g.view.num_neurons, g.view.num_synapses, o.dataset.c_str(), o.num_train,
               o.num_test, o.train_mode.c_str(), o.batch);

@@ -76,6 +76,25 @@ def test_train_driver_learns():
     assert len(acc) == 5 and acc[-1] >= 0.4 and acc[-1] > acc[0] + 0.2
 
 
+def test_train_driver_serial_mode_equals_batched():
+    """--train-mode serial launches the persistent kernel once per sample; its
+    weights are the batched mode's bit for bit, so every epoch's held-out
+    accuracy is the same."""
+    accs = []
+    for mode in ("batched", "serial"):
+        r = _run([os.path.join(BIN, "snc_hip_train"), "--dim", "196", "--hidden", "96",
+                  "--num-train", "128", "--num-test", "64", "--num-steps", "30",
+                  "--min-steps", "20", "--batch", "32", "--epochs", "2", "--w-std", "0.3",
+                  "--max-rate", "0.3", "--fan-in", "24", "--rec-fan-in", "16",
+                  "--train-mode", mode])
+        assert r.returncode == 0, r.stderr
+        accs.append([l.split("test_acc=")[1].split()[0] for l in r.stdout.splitlines()
+                     if l.startswith("epoch")])
+    assert len(accs[0]) == 3 and accs[0] == accs[1]
+    r = _run([os.path.join(BIN, "snc_hip_train"), "--train-mode", "sideways", "--epochs", "0"])
+    assert r.returncode != 0 and "unknown --train-mode" in r.stderr
+
+
 def test_train_driver_reward_mode_learns():
     """--reward-mode 1 (train_main.cpp:119): the three-factor rule sees only
     right/wrong per sample, so it climbs slower than the supervised delta;
